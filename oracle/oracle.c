@@ -1,0 +1,945 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * An FP64 CPU restatement of PureAdsorb.jl's Widom insertion path, written from the
+ * reference's Julia source (file:line citations below point into the reference tree).
+ * It is the checker for the HIP product path: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and never as the thing measured or shipped.
+ * The product library never links, loads or calls anything in this directory.
+ *
+ * Pinning (what fixes each function independently of this restatement) is listed in
+ * tests/test_oracle_*.py; functions without an independent check are marked
+ * "parity unpinned" below.
+ *
+ * Build: oracle/Makefile -> oracle/_build/liboracle.so (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* src/constants.jl:3-4 */
+#define OR_KB (1.38064852e-23 / 1.6021766208e-19)
+#define OR_KE 14.399645351950548
+#define OR_PI 3.141592653589793
+#define OR_TWO_PI 6.283185307179586
+#define OR_MAXSITES 16
+
+/* ------------------------------------------------------------------------------------------
+ * Counter-based RNG: src/state.jl:225-230 (splitmix64), src/rng.jl:14-24 (mulhi64),
+ * src/rng.jl:43-81 (ChainRNG, next_bits, unit_from_bits, rand_uniform),
+ * src/rng.jl:93-97 (rand_range).
+ * ---------------------------------------------------------------------------------------- */
+uint64_t or_splitmix64(uint64_t seed, uint64_t index)
+{
+    uint64_t z = seed + index * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t or_mulhi64(uint64_t a, uint64_t b)
+{
+    uint64_t a_lo = a & 0xffffffffULL, a_hi = a >> 32;
+    uint64_t b_lo = b & 0xffffffffULL, b_hi = b >> 32;
+    uint64_t lo_lo = a_lo * b_lo, hi_lo = a_hi * b_lo, lo_hi = a_lo * b_hi, hi_hi = a_hi * b_hi;
+    uint64_t cross = (lo_lo >> 32) + (hi_lo & 0xffffffffULL) + (lo_hi & 0xffffffffULL);
+    return hi_hi + (hi_lo >> 32) + (lo_hi >> 32) + (cross >> 32);
+}
+
+typedef struct { uint64_t key, draw; } or_chain;
+
+static or_chain or_chain_new(uint64_t chain_seed, uint64_t cycle, uint64_t move)
+{
+    or_chain c;
+    c.key = or_splitmix64(or_splitmix64(chain_seed, cycle), move);
+    c.draw = 0;
+    return c;
+}
+
+static uint64_t or_next_bits(or_chain *c)
+{
+    uint64_t raw = or_splitmix64(c->key, c->draw);
+    c->draw += 1;
+    return raw;
+}
+
+double or_unit_f64(uint64_t raw) { return (double)(raw >> 11) * 1.1102230246251565e-16; }
+float or_unit_f32(uint64_t raw) { return (float)(raw >> 40) * 5.9604645e-8f; }
+
+/* Draws words 0..nwords-1 of ChainRNG(chain_seed, cycle, move) (test hook). */
+void or_chain_words(uint64_t chain_seed, uint64_t cycle, uint64_t move, int nwords, uint64_t *out)
+{
+    or_chain c = or_chain_new(chain_seed, cycle, move);
+    for (int i = 0; i < nwords; ++i) out[i] = or_next_bits(&c);
+}
+
+/* rand_range: 1-based index in 1:n (src/rng.jl:93-97). */
+int64_t or_rand_range(uint64_t raw, int64_t n) { return (int64_t)or_mulhi64(raw, (uint64_t)n) + 1; }
+
+/* ------------------------------------------------------------------------------------------
+ * sin(pi x), cos(pi x). The reference uses Julia Base `sinpi`/`cospi` (src/rng.jl:126-129);
+ * this build fixes its own contract: exact reduction to r in [-1/4, 1/4] by the nearest
+ * half-integer, then Taylor series of sin(pi r)/cos(pi r) in r^2 (coefficients
+ * (-1)^k pi^(2k+1)/(2k+1)!, (-1)^k pi^(2k)/(2k)!, rounded from 40-digit values), Horner with
+ * fused multiply-add. Pinned against mpmath to <= 1 ulp (tests/test_oracle_rng.py).
+ * ULP-level parity with Julia Base is unpinned.
+ * ---------------------------------------------------------------------------------------- */
+static const double OR_SINPI_C[9] = {
+    0x1.921fb54442d18p+1, -0x1.4abbce625be53p+2, 0x1.466bc6775aae2p+1,
+    -0x1.32d2cce62bd86p-1, 0x1.50783487ee782p-4, -0x1.e3074fde8871fp-8,
+    0x1.e8f434d018d63p-12, -0x1.6fadb9f155744p-16, 0x1.aaec32af93359p-21,
+};
+static const double OR_COSPI_C[10] = {
+    0x1.0000000000000p+0, -0x1.3bd3cc9be45dep+2, 0x1.03c1f081b5ac4p+2,
+    -0x1.55d3c7e3cbffap+0, 0x1.e1f506891babbp-3, -0x1.a6d1f2a204a8cp-6,
+    0x1.f9d38a3763cc3p-10, -0x1.b6e24f44b128fp-14, 0x1.20c62c2f2d7f5p-18,
+    -0x1.2a0c591af8314p-23,
+};
+
+void or_sincospi(double x, double *s_out, double *c_out)
+{
+    double n = rint(2.0 * x);          /* nearest multiple of 1/2, exact */
+    double r = x - 0.5 * n;            /* exact: |r| <= 1/4 */
+    double r2 = r * r;
+    double ps = OR_SINPI_C[8];
+    for (int k = 7; k >= 0; --k) ps = fma(ps, r2, OR_SINPI_C[k]);
+    double s = r * ps;
+    double pc = OR_COSPI_C[9];
+    for (int k = 8; k >= 0; --k) pc = fma(pc, r2, OR_COSPI_C[k]);
+    double c = pc;
+    int64_t q = (int64_t)n & 3;
+    double so, co;
+    switch (q) {
+    case 0: so = s; co = c; break;
+    case 1: so = c; co = -s; break;
+    case 2: so = -s; co = -c; break;
+    default: so = -c; co = s; break;
+    }
+    *s_out = so;
+    *c_out = co;
+}
+
+/* Shoemake (1992) quaternion in (x, y, z, w) order: src/rng.jl:126-129. */
+void or_shoemake(double u1, double u2, double u3, double q[4])
+{
+    double a = sqrt(1.0 - u1), b = sqrt(u1);
+    double s2, c2, s3, c3;
+    or_sincospi(2.0 * u2, &s2, &c2);
+    or_sincospi(2.0 * u3, &s3, &c3);
+    q[0] = a * c2;
+    q[1] = b * s3;
+    q[2] = b * c3;
+    q[3] = a * s2;
+}
+
+/* Column-major 3x3 times vector, summed left to right (StaticArrays order). */
+static inline void or_matvec(const double A[9], const double v[3], double out[3])
+{
+    for (int i = 0; i < 3; ++i) out[i] = A[i] * v[0] + A[i + 3] * v[1] + A[i + 6] * v[2];
+}
+
+/* Widom pose of system `sys` (0-based), per-system insertion `j` (0-based): the reference's
+ * counter-based rigid-guest placement `propose_reinsertion` (src/moves.jl:154-161) keyed as
+ * ChainRNG(splitmix64(seed, sys+1), j+1, 0); chain seeds follow SystemState's
+ * rng_seed[n] = splitmix64(seed, n) (src/state.jl:434). */
+void or_pose(uint64_t seed, int64_t sys, int64_t j, const double A[9], double pos[3], double q[4])
+{
+    or_chain c = or_chain_new(or_splitmix64(seed, (uint64_t)(sys + 1)), (uint64_t)(j + 1), 0);
+    double f[3];
+    for (int i = 0; i < 3; ++i) f[i] = or_unit_f64(or_next_bits(&c));
+    or_matvec(A, f, pos);
+    double u1 = or_unit_f64(or_next_bits(&c));
+    double u2 = or_unit_f64(or_next_bits(&c));
+    double u3 = or_unit_f64(or_next_bits(&c));
+    or_shoemake(u1, u2, u3, q);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Cell geometry: src/cell.jl:2-37.
+ * ---------------------------------------------------------------------------------------- */
+/* sin/cos of an angle in degrees with exact zeros at multiples of 90 (Julia `sind`/`cosd`
+ * semantics at those points; other values are unpinned at the ULP level). */
+static void or_sincosd(double deg, double *s, double *c)
+{
+    double n = rint(deg / 90.0);
+    double r = deg - 90.0 * n;   /* exact for the angles a cell uses */
+    double rs, rc;
+    or_sincospi(r / 180.0, &rs, &rc);
+    int64_t q = ((int64_t)n) & 3;
+    switch (q) {
+    case 0: *s = rs; *c = rc; break;
+    case 1: *s = rc; *c = -rs; break;
+    case 2: *s = -rs; *c = -rc; break;
+    default: *s = -rc; *c = rs; break;
+    }
+}
+
+int or_cell_matrix(double a, double b, double c, double al, double be, double ga, double A[9])
+{
+    if (!(a > 0 && b > 0 && c > 0)) return -1;
+    double sa, ca, sb, cb, sg, cg;
+    or_sincosd(al, &sa, &ca);
+    or_sincosd(be, &sb, &cb);
+    or_sincosd(ga, &sg, &cg);
+    (void)sa; (void)sb;
+    if (sg == 0.0) return -2;
+    double cy = (ca - cb * cg) / sg;
+    double rad = 1.0 - cb * cb - cy * cy;
+    if (!(rad > 0)) return -3;
+    double cz = sqrt(rad);
+    A[0] = a; A[1] = 0; A[2] = 0;
+    A[3] = b * cg; A[4] = b * sg; A[5] = 0;
+    A[6] = c * cb; A[7] = c * cy; A[8] = c * cz;
+    return 0;
+}
+
+double or_det3(const double A[9])
+{
+    /* A[r + 3c] */
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+           A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+/* Closed-form 3x3 inverse by cofactors (StaticArrays' `inv` is the reference; ULP-level
+ * agreement with it is unpinned, energies are checked at rtol 1e-12). */
+void or_inv3(const double A[9], double inv[9])
+{
+    double d = or_det3(A);
+    inv[0] = (A[4] * A[8] - A[7] * A[5]) / d;
+    inv[1] = -(A[1] * A[8] - A[7] * A[2]) / d;
+    inv[2] = (A[1] * A[5] - A[4] * A[2]) / d;
+    inv[3] = -(A[3] * A[8] - A[6] * A[5]) / d;
+    inv[4] = (A[0] * A[8] - A[6] * A[2]) / d;
+    inv[5] = -(A[0] * A[5] - A[3] * A[2]) / d;
+    inv[6] = (A[3] * A[7] - A[6] * A[4]) / d;
+    inv[7] = -(A[0] * A[7] - A[6] * A[1]) / d;
+    inv[8] = (A[0] * A[4] - A[3] * A[1]) / d;
+}
+
+static void or_cross(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double or_norm3(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* src/cell.jl:14-20 */
+void or_perpendicular_lengths(const double A[9], double L[3])
+{
+    double V = fabs(or_det3(A));
+    double x[3];
+    or_cross(A + 3, A + 6, x); L[0] = V / or_norm3(x);
+    or_cross(A + 0, A + 6, x); L[1] = V / or_norm3(x);
+    or_cross(A + 0, A + 3, x); L[2] = V / or_norm3(x);
+}
+
+/* src/cell.jl:34-37: Delta - A*round(invA*Delta), round = nearest-even. */
+void or_minimum_image(const double A[9], const double invA[9], const double d[3], double out[3])
+{
+    double f[3], r[3], s[3];
+    or_matvec(invA, d, f);
+    for (int i = 0; i < 3; ++i) r[i] = rint(f[i]);
+    or_matvec(A, r, s);
+    for (int i = 0; i < 3; ++i) out[i] = d[i] - s[i];
+}
+
+/* src/energy.jl:13-16: v + 2 u x (u x v + w v). */
+void or_rotate(const double q[4], const double v[3], double out[3])
+{
+    double u[3] = {q[0], q[1], q[2]}, w = q[3];
+    double t[3], uv[3];
+    or_cross(u, v, uv);
+    for (int i = 0; i < 3; ++i) t[i] = uv[i] + w * v[i];
+    double ut[3];
+    or_cross(u, t, ut);
+    for (int i = 0; i < 3; ++i) out[i] = v[i] + 2.0 * ut[i];
+}
+
+/* ------------------------------------------------------------------------------------------
+ * erfc series: src/ewald.jl:47-74 (erfc_dev, 28 terms) and src/ewald.jl:88-130
+ * (pair_erfc_dev, 17 terms in Float64, 8 in Float32). The coefficient literals below are
+ * the reference's (src/ewald.jl:47-56,103-113); tests/test_oracle_erfc.py regenerates them
+ * with mpmath from the construction the reference documents and checks them against scipy.
+ * ---------------------------------------------------------------------------------------- */
+static const double OR_ERFC_COF[28] = {
+    -1.3026537197817094, 6.419697923564902e-1, 1.9476473204185836e-2,
+    -9.561514786808632e-3, -9.465953444820369e-4, 3.6683949785276145e-4, 4.252332480690777e-5,
+    -2.0278578112534242e-5, -1.6242900046470256e-6, 1.3036558355805232e-6, 1.5626441722066142e-8,
+    -8.523809591492654e-8, 6.5290544390988515e-9, 5.0593434955514685e-9, -9.913641564930322e-10,
+    -2.2736512229318417e-10, 9.646791102014962e-11, 2.394038083065756e-12, -6.886027526536277e-12,
+    8.944879271730368e-13, 3.130921408127142e-13, -1.1270822525221733e-13, 3.8108713588528985e-16,
+    7.106125536922576e-15, -1.5230878975373303e-15, -9.464871412184462e-17, 1.218683266906788e-16,
+    -3.0494735135424696e-17,
+};
+
+static double or_erfccheb(double z)
+{
+    double t = 2.0 / (2.0 + z);
+    double ty = 4.0 * t - 2.0;
+    double d = 0.0, dd = 0.0;
+    for (int j = 27; j >= 1; --j) {
+        double tmp = d;
+        d = ty * d - dd + OR_ERFC_COF[j];
+        dd = tmp;
+    }
+    return t * exp(-z * z + (ty / 2.0) * d - dd + OR_ERFC_COF[0] / 2.0);
+}
+
+double or_erfc_dev(double x) { return x >= 0 ? or_erfccheb(x) : 2.0 - or_erfccheb(-x); }
+
+#define OR_PAIR_ERFC_XMAX 4.0
+static const double OR_PAIR_COF64[17] = {
+    -0.888894912038317, 0.4479385778277769, -0.00021608983881297707, -0.003410058708799526,
+    8.555152981629436e-5, 5.532830637302869e-5, -5.2098383667087075e-6, -8.380063246741613e-7,
+    1.9764068422519322e-7, 1.7838566371819855e-9, -5.26989861621132e-9, 5.378177355603348e-10,
+    7.748359588558286e-11, -2.3770397709020423e-11, 8.934868696420547e-13, 5.207576546902621e-13,
+    -9.42604840757663e-14,
+};
+static const float OR_PAIR_COF32[8] = {
+    -0.8888949f, 0.4479386f, -0.00021608984f, -0.0034100588f,
+    8.555145e-5f, 5.5327768e-5f, -5.2045684e-6f, -8.397902e-7f,
+};
+
+double or_pair_erfc_f64(double z)
+{
+    const double xlo = 2.0 * (2.0 / (2.0 + OR_PAIR_ERFC_XMAX)) - 1.0;
+    double t = 2.0 / (2.0 + z);
+    double xi = 2.0 * (2.0 * t - 1.0 - xlo) / (1.0 - xlo) - 1.0;
+    double d = 0.0, dd = 0.0;
+    for (int j = 16; j >= 1; --j) {
+        double tmp = d;
+        d = 2.0 * xi * d - dd + OR_PAIR_COF64[j];
+        dd = tmp;
+    }
+    return t * exp(xi * d - dd + OR_PAIR_COF64[0] / 2.0 - z * z);
+}
+
+/* Float32 arithmetic throughout (the reference's F32 method, src/ewald.jl:120-130). */
+float or_pair_erfc_f32(float z)
+{
+    const float xlo = (float)(2.0 * (2.0 / (2.0 + OR_PAIR_ERFC_XMAX)) - 1.0);
+    float t = 2.0f / (2.0f + z);
+    float xi = 2.0f * (2.0f * t - 1.0f - xlo) / (1.0f - xlo) - 1.0f;
+    float d = 0.0f, dd = 0.0f;
+    for (int j = 7; j >= 1; --j) {
+        float tmp = d;
+        d = 2.0f * xi * d - dd + OR_PAIR_COF32[j];
+        dd = tmp;
+    }
+    return t * expf(xi * d - dd + OR_PAIR_COF32[0] / 2.0f - z * z);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Ewald parameters and tables: src/ewald.jl:21-39,142-170; src/reference.jl:10-16.
+ * ---------------------------------------------------------------------------------------- */
+/* Bisection on erfc(z) = rc*prec/2 (src/ewald.jl:21-32). The reference uses
+ * SpecialFunctions.erfc; this uses C99 erfc. Returns -1 when the precision is unreachable. */
+double or_ewald_alpha(double cutoff, double precision)
+{
+    double target = cutoff * precision / 2.0;
+    double lo = 0.0, hi = 20.0;
+    if (!(erfc(hi) - target < 0)) return -1.0;
+    for (int it = 0; it < 200; ++it) {
+        double mid = (lo + hi) / 2.0;
+        if (erfc(mid) - target > 0) lo = mid; else hi = mid;
+        if (hi - lo < 1.0e-12) break;
+    }
+    return (lo + hi) / 2.0 / cutoff;
+}
+
+double or_ewald_kmax(double alpha, double precision) { return 2.0 * alpha * sqrt(-log(precision / 2.0)); }
+
+/* src/ewald.jl:36-39 */
+double or_pk(double k2, double alpha, double V) { return 2.0 * OR_PI / V * exp(-k2 / (4.0 * alpha * alpha)) / k2; }
+
+/* Half-space enumeration (src/ewald.jl:142-157): n1 in 0..n1max outer, n2 then n3 in
+ * -n..n, skip 0, keep |k| <= kmax, weight 1 if n1 == 0 else 2. Writes up to `cap` entries
+ * and returns the total count (call with cap = 0 to size). */
+int or_kvectors(const double A[9], double kmax, int cap, double *ks, double *w, int *coeffs)
+{
+    double invA[9], B[9];
+    or_inv3(A, invA);
+    /* B = 2pi * inv(A)', B[i + 3j] = 2pi * invA[j + 3i] */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[i + 3 * j] = OR_TWO_PI * invA[j + 3 * i];
+    int nmax[3];
+    for (int i = 0; i < 3; ++i) nmax[i] = (int)ceil(kmax * or_norm3(A + 3 * i) / OR_TWO_PI);
+    int count = 0;
+    for (int n1 = 0; n1 <= nmax[0]; ++n1)
+        for (int n2 = -nmax[1]; n2 <= nmax[1]; ++n2)
+            for (int n3 = -nmax[2]; n3 <= nmax[2]; ++n3) {
+                if (n1 == 0 && n2 == 0 && n3 == 0) continue;
+                double k[3];
+                for (int i = 0; i < 3; ++i) k[i] = B[i] * n1 + B[i + 3] * n2 + B[i + 6] * n3;
+                if (!(or_norm3(k) <= kmax)) continue;
+                if (count < cap) {
+                    ks[3 * count] = k[0]; ks[3 * count + 1] = k[1]; ks[3 * count + 2] = k[2];
+                    w[count] = n1 == 0 ? 1.0 : 2.0;
+                    coeffs[3 * count] = n1; coeffs[3 * count + 1] = n2; coeffs[3 * count + 2] = n3;
+                }
+                ++count;
+            }
+    return count;
+}
+
+static double or_dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* src/ewald.jl:159-170 */
+void or_structure_factor(int nk, const double *ks, int n, const double *pos, const double *q,
+                         double *Sre, double *Sim)
+{
+    for (int i = 0; i < nk; ++i) {
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double ph = or_dot3(ks + 3 * i, pos + 3 * j);
+            re += q[j] * cos(ph);
+            im += q[j] * sin(ph);
+        }
+        Sre[i] = re;
+        Sim[i] = im;
+    }
+}
+
+/* Full Ewald energy of one periodic system (src/ewald.jl:177-203): the oracle of the oracle,
+ * pinned by the NaCl Madelung constant and alpha independence. */
+double or_ewald_energy(const double A[9], int n, const double *pos, const double *q, const int *mol,
+                       double alpha, double cutoff, int nk, const double *ks, const double *w)
+{
+    double invA[9];
+    or_inv3(A, invA);
+    double V = fabs(or_det3(A));
+    double E_sr = 0.0, E_excl = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+            if (mol[i] == mol[j]) {
+                double r = or_norm3(d);
+                E_excl -= q[i] * q[j] * (1.0 - or_erfc_dev(alpha * r)) / r;
+            } else {
+                double m[3];
+                or_minimum_image(A, invA, d, m);
+                double r = or_norm3(m);
+                if (r < cutoff) E_sr += q[i] * q[j] * or_erfc_dev(alpha * r) / r;
+            }
+        }
+    double *Sre = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+    double *Sim = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+    or_structure_factor(nk, ks, n, pos, q, Sre, Sim);
+    double E_lr = 0.0;
+    for (int i = 0; i < nk; ++i)
+        E_lr += w[i] * or_pk(or_dot3(ks + 3 * i, ks + 3 * i), alpha, V) * (Sre[i] * Sre[i] + Sim[i] * Sim[i]);
+    free(Sre);
+    free(Sim);
+    double q2 = 0.0, Q = 0.0;
+    for (int i = 0; i < n; ++i) { q2 += q[i] * q[i]; Q += q[i]; }
+    double E_self = -alpha / sqrt(OR_PI) * q2;
+    double E_net = -OR_PI / (2.0 * V * alpha * alpha) * Q * Q;
+    return OR_KE * (E_sr + E_lr + E_self + E_excl + E_net);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Force-field tail correction: src/forcefield.jl:103-124. sigma/eps are ntypes x ntypes,
+ * column-major (index a + ntypes*b).
+ * ---------------------------------------------------------------------------------------- */
+static double or_tail_coefficient(int ntypes, const double *sigma, const double *eps, double cutoff, int i, int j)
+{
+    double e = eps[i + ntypes * j];
+    if (e == 0.0) return 0.0;
+    double s = sigma[i + ntypes * j];
+    double x = s / cutoff;
+    double x3 = x * x * x;
+    return e * (s * s * s) * (x3 * x3 * x3 / 3.0 - x3);
+}
+
+double or_tail_delta(int ntypes, const double *sigma, const double *eps, double cutoff, int tail,
+                     const int64_t *counts, const int64_t *gcounts, double V)
+{
+    if (!tail) return 0.0;
+    double acc = 0.0;
+    for (int a = 0; a < ntypes; ++a)
+        for (int b = 0; b < ntypes; ++b) {
+            double c = or_tail_coefficient(ntypes, sigma, eps, cutoff, a, b);
+            acc += 2.0 * (double)counts[a] * (double)gcounts[b] * c + (double)gcounts[a] * (double)gcounts[b] * c;
+        }
+    return (8.0 * OR_PI / 3.0) / V * acc;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * One framework system with the per-framework tables of FrameworkBatch
+ * (src/batch.jl:237-432): coupled-k filter, replication check, 64-orientation guest self
+ * term, constant_offset. Atoms stay in input (CIF/replicate) order; the reference's cell sort
+ * (src/batch.jl:332-335) changes only the summation order.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+    double A[9], invA[9], V;
+    int natoms;
+    double *pos, *q;
+    int *type;
+    int rep[3];
+    int ntypes;
+    double *sigma, *eps;
+    double rc_lj;
+    int tail;
+    int nsites;
+    double gsite[3 * OR_MAXSITES], gq[OR_MAXSITES];
+    int gtype[OR_MAXSITES];
+    int charged;
+    double rc_ew, prec, alpha, kmax;
+    int nkfull;
+    double *kfull, *kpref_full, *Sre_full, *Sim_full;
+    int *coef_full;
+    int nk;
+    int *kidx;
+    double self_mean, self_halfrange, constant_offset, tail_term, self_term, excl_term, net_term;
+    int status;
+    char err[512];
+} or_system;
+
+/* Self-term orientations: ChainRNG(0x5e1f, 0, o), o = 1..64, three F64 uniforms each.
+ * The reference draws its 64 orientations from Xoshiro(0x5e1f) (src/batch.jl:302-303),
+ * which cannot be reproduced here; energies then differ from the reference by at most
+ * 2*self_term_halfrange (the reference's own tolerance for this approximation). */
+void or_self_orientation(int o, double q[4])
+{
+    or_chain c = or_chain_new(0x5e1fULL, 0, (uint64_t)o);
+    double u1 = or_unit_f64(or_next_bits(&c));
+    double u2 = or_unit_f64(or_next_bits(&c));
+    double u3 = or_unit_f64(or_next_bits(&c));
+    or_shoemake(u1, u2, u3, q);
+}
+
+static void or_fail(or_system *s, const char *msg)
+{
+    s->status = -1;
+    snprintf(s->err, sizeof s->err, "%s", msg);
+}
+
+void or_system_free(or_system *s)
+{
+    if (!s) return;
+    free(s->pos); free(s->q); free(s->type); free(s->sigma); free(s->eps);
+    free(s->kfull); free(s->kpref_full); free(s->Sre_full); free(s->Sim_full); free(s->coef_full);
+    free(s->kidx);
+    free(s);
+}
+
+or_system *or_system_new(const double *A, int natoms, const double *frac, const int *type, const double *q,
+                         const int *rep, int ntypes, const double *sigma, const double *eps, double rc_lj,
+                         int tail, int nsites, const double *gsites, const int *gtypes, const double *gq,
+                         double rc_ew, double prec)
+{
+    or_system *s = calloc(1, sizeof(or_system));
+    if (nsites < 1 || nsites > OR_MAXSITES) { or_fail(s, "guest site count out of range"); return s; }
+    memcpy(s->A, A, sizeof s->A);
+    or_inv3(s->A, s->invA);
+    s->V = fabs(or_det3(s->A));
+    s->natoms = natoms;
+    s->pos = malloc(sizeof(double) * 3 * (size_t)(natoms > 0 ? natoms : 1));
+    s->q = malloc(sizeof(double) * (size_t)(natoms > 0 ? natoms : 1));
+    s->type = malloc(sizeof(int) * (size_t)(natoms > 0 ? natoms : 1));
+    for (int j = 0; j < natoms; ++j) {
+        or_matvec(s->A, frac + 3 * j, s->pos + 3 * j);
+        s->q[j] = q[j];
+        s->type[j] = type[j];
+    }
+    memcpy(s->rep, rep, sizeof s->rep);
+    s->ntypes = ntypes;
+    s->sigma = malloc(sizeof(double) * (size_t)ntypes * ntypes);
+    s->eps = malloc(sizeof(double) * (size_t)ntypes * ntypes);
+    memcpy(s->sigma, sigma, sizeof(double) * (size_t)ntypes * ntypes);
+    memcpy(s->eps, eps, sizeof(double) * (size_t)ntypes * ntypes);
+    s->rc_lj = rc_lj;
+    s->tail = tail;
+    s->nsites = nsites;
+    s->charged = 0;
+    double rguest = 0.0;
+    for (int a = 0; a < nsites; ++a) {
+        for (int i = 0; i < 3; ++i) s->gsite[3 * a + i] = gsites[3 * a + i];
+        s->gtype[a] = gtypes[a];
+        s->gq[a] = gq[a];
+        if (gq[a] != 0.0) s->charged = 1;
+        double r = or_norm3(gsites + 3 * a);
+        if (r > rguest) rguest = r;
+    }
+    s->rc_ew = rc_ew;
+    s->prec = prec;
+    (void)rguest;
+    /* src/batch.jl:289-297 */
+    s->alpha = or_ewald_alpha(rc_ew, prec);
+    if (s->alpha < 0) { or_fail(s, "cannot reach precision with cutoff"); return s; }
+    if (s->alpha * rc_ew > OR_PAIR_ERFC_XMAX) { or_fail(s, "alpha*ewald_cutoff exceeds PAIR_ERFC_XMAX"); return s; }
+    s->kmax = or_ewald_kmax(s->alpha, prec);
+    int need_k = s->charged || rep[0] != 1 || rep[1] != 1 || rep[2] != 1;
+    if (need_k) {
+        int nk = or_kvectors(s->A, s->kmax, 0, NULL, NULL, NULL);
+        s->nkfull = nk;
+        s->kfull = malloc(sizeof(double) * 3 * (size_t)(nk > 0 ? nk : 1));
+        double *w = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+        s->coef_full = malloc(sizeof(int) * 3 * (size_t)(nk > 0 ? nk : 1));
+        or_kvectors(s->A, s->kmax, nk, s->kfull, w, s->coef_full);
+        s->kpref_full = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+        for (int i = 0; i < nk; ++i) s->kpref_full[i] = w[i] * or_pk(or_dot3(s->kfull + 3 * i, s->kfull + 3 * i), s->alpha, s->V);
+        free(w);
+        s->Sre_full = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+        s->Sim_full = malloc(sizeof(double) * (size_t)(nk > 0 ? nk : 1));
+        or_structure_factor(nk, s->kfull, natoms, s->pos, s->q, s->Sre_full, s->Sim_full);
+        s->kidx = malloc(sizeof(int) * (size_t)(nk > 0 ? nk : 1));
+        s->nk = 0;
+        /* src/batch.jl:355-359: coupled iff every coefficient is divisible by the replication */
+        for (int i = 0; i < nk; ++i) {
+            int c = 1;
+            for (int d = 0; d < 3; ++d) if (s->coef_full[3 * i + d] % rep[d] != 0) c = 0;
+            if (c && s->charged) s->kidx[s->nk++] = i;
+        }
+        /* replication check, src/batch.jl:173-195 */
+        if (rep[0] != 1 || rep[1] != 1 || rep[2] != 1) {
+            int *unc = malloc(sizeof(int) * (size_t)(nk > 0 ? nk : 1));
+            int nu = 0;
+            for (int i = 0; i < nk; ++i) {
+                int c = 1;
+                for (int d = 0; d < 3; ++d) if (s->coef_full[3 * i + d] % rep[d] != 0) c = 0;
+                if (!c) unc[nu++] = i;
+            }
+            if (nu > 0) {
+                int stride = (nu + 31) / 32;
+                if (stride < 1) stride = 1;
+                double maxS = 0.0, sabs = 0.0;
+                int taken = 0;
+                for (int t = 0; t < nu && taken < 32; t += stride, ++taken) {
+                    int i = unc[t];
+                    double m = hypot(s->Sre_full[i], s->Sim_full[i]);
+                    if (m > maxS) maxS = m;
+                }
+                for (int j = 0; j < natoms; ++j) sabs += fabs(s->q[j]);
+                double thr = fmax(1.0e-8, 100.0 * 2.220446049250313e-16) * sabs;
+                if (maxS > thr) { free(unc); or_fail(s, "replication does not describe the atoms"); return s; }
+            }
+            free(unc);
+        }
+    }
+    /* 64-orientation self term, src/batch.jl:360-389 */
+    if (s->charged) {
+        double samples[64];
+        for (int o = 0; o < 64; ++o) {
+            double qo[4];
+            or_self_orientation(o + 1, qo);
+            double gs[3 * OR_MAXSITES];
+            for (int a = 0; a < nsites; ++a) or_rotate(qo, s->gsite + 3 * a, gs + 3 * a);
+            double acc = 0.0;
+            for (int i = 0; i < s->nkfull; ++i) {
+                double re = 0.0, im = 0.0;
+                for (int a = 0; a < nsites; ++a) {
+                    double ph = or_dot3(s->kfull + 3 * i, gs + 3 * a);
+                    re += s->gq[a] * cos(ph);
+                    im += s->gq[a] * sin(ph);
+                }
+                acc += s->kpref_full[i] * (re * re + im * im);
+            }
+            samples[o] = OR_KE * acc;
+        }
+        double sum = 0.0, mx = samples[0], mn = samples[0];
+        for (int o = 0; o < 64; ++o) {
+            sum += samples[o];
+            if (samples[o] > mx) mx = samples[o];
+            if (samples[o] < mn) mn = samples[o];
+        }
+        s->self_mean = sum / 64.0;
+        s->self_halfrange = (mx - mn) / 2.0;
+        const double KT_REF = OR_KB * 300.0;
+        if (2.0 * s->self_halfrange / KT_REF > 1.0e-3) { or_fail(s, "guest self-term half-range exceeds tolerance"); return s; }
+    }
+    /* constant_offset, src/batch.jl:396-407 */
+    int64_t *counts = calloc((size_t)ntypes, sizeof(int64_t));
+    int64_t *gcounts = calloc((size_t)ntypes, sizeof(int64_t));
+    for (int j = 0; j < natoms; ++j) counts[type[j]] += 1;
+    for (int a = 0; a < nsites; ++a) gcounts[gtypes[a]] += 1;
+    s->tail_term = or_tail_delta(ntypes, sigma, eps, rc_lj, tail, counts, gcounts, s->V);
+    free(counts);
+    free(gcounts);
+    double q2 = 0.0;
+    for (int a = 0; a < nsites; ++a) q2 += s->gq[a] * s->gq[a];
+    s->self_term = -s->alpha / sqrt(OR_PI) * q2;
+    double excl = 0.0;
+    for (int a = 0; a < nsites; ++a)
+        for (int c = a + 1; c < nsites; ++c) {
+            double d[3] = {s->gsite[3 * a] - s->gsite[3 * c], s->gsite[3 * a + 1] - s->gsite[3 * c + 1],
+                           s->gsite[3 * a + 2] - s->gsite[3 * c + 2]};
+            double r = or_norm3(d);
+            excl -= s->gq[a] * s->gq[c] * (1.0 - or_erfc_dev(s->alpha * r)) / r;
+        }
+    s->excl_term = excl;
+    double Qh = 0.0, Qg = 0.0;
+    for (int j = 0; j < natoms; ++j) Qh += s->q[j];
+    for (int a = 0; a < nsites; ++a) Qg += s->gq[a];
+    s->net_term = -OR_PI / (2.0 * s->V * s->alpha * s->alpha) * ((Qh + Qg) * (Qh + Qg) - Qh * Qh);
+    s->constant_offset = s->tail_term + OR_KE * (s->self_term + s->excl_term + s->net_term) + s->self_mean;
+    s->status = 0;
+    return s;
+}
+
+int or_system_status(const or_system *s, char *buf, int n)
+{
+    if (buf && n > 0) snprintf(buf, (size_t)n, "%s", s->err);
+    return s->status;
+}
+
+/* out: V, alpha, kmax, constant_offset, self_mean, self_halfrange, tail, self, excl, net,
+ * nk (coupled), nkfull */
+void or_system_scalars(const or_system *s, double out[12])
+{
+    out[0] = s->V; out[1] = s->alpha; out[2] = s->kmax; out[3] = s->constant_offset;
+    out[4] = s->self_mean; out[5] = s->self_halfrange; out[6] = s->tail_term; out[7] = s->self_term;
+    out[8] = s->excl_term; out[9] = s->net_term; out[10] = s->nk; out[11] = s->nkfull;
+}
+
+/* Copies the full (which = 1) or coupled (which = 0) k table. */
+void or_system_ktable(const or_system *s, int which, double *ks, double *kpref, double *Sre, double *Sim, int *coef)
+{
+    int n = which ? s->nkfull : s->nk;
+    for (int t = 0; t < n; ++t) {
+        int i = which ? t : s->kidx[t];
+        for (int d = 0; d < 3; ++d) { ks[3 * t + d] = s->kfull[3 * i + d]; coef[3 * t + d] = s->coef_full[3 * i + d]; }
+        kpref[t] = s->kpref_full[i];
+        Sre[t] = s->Sre_full[i];
+        Sim[t] = s->Sim_full[i];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Insertion energy. `or_insertion_energy` is the production contract of FrameworkBatch +
+ * insertion_energy (src/energy.jl:42-82): LJ + screened Coulomb with pair_erfc_dev, the
+ * reciprocal cross term over the coupled k subset only, the orientation-averaged self term
+ * left to constant_offset. Minimum image per (site, atom) pair, the oracle's own semantics
+ * (src/reference.jl:39-56), which equals the reference's single image per atom whenever its
+ * construction guard holds and stays exact down to perpendicular widths of 2*cutoff.
+ * Returns E without constant_offset; *npairs receives the in-cutoff (site, atom) count.
+ * ---------------------------------------------------------------------------------------- */
+double or_insertion_energy(const or_system *s, const double pos[3], const double q[4], int64_t *npairs)
+{
+    double rc_lj2 = s->rc_lj * s->rc_lj, rc_ew2 = s->rc_ew * s->rc_ew;
+    double gp[3 * OR_MAXSITES];
+    for (int a = 0; a < s->nsites; ++a) {
+        double r[3];
+        or_rotate(q, s->gsite + 3 * a, r);
+        for (int i = 0; i < 3; ++i) gp[3 * a + i] = pos[i] + r[i];
+    }
+    double E_lj = 0.0, E_sr = 0.0;
+    int64_t np = 0;
+    for (int j = 0; j < s->natoms; ++j) {
+        const double *hp = s->pos + 3 * j;
+        int ht = s->type[j];
+        double hq = s->q[j];
+        for (int a = 0; a < s->nsites; ++a) {
+            double d[3] = {gp[3 * a] - hp[0], gp[3 * a + 1] - hp[1], gp[3 * a + 2] - hp[2]};
+            double m[3];
+            or_minimum_image(s->A, s->invA, d, m);
+            double r2 = or_dot3(m, m);
+            if (!(r2 < rc_lj2 || r2 < rc_ew2)) continue;
+            ++np;
+            int gt = s->gtype[a];
+            if (r2 < rc_lj2) {
+                double sg = s->sigma[gt + s->ntypes * ht], ep = s->eps[gt + s->ntypes * ht];
+                double x = sg * sg / r2;
+                x = x * x * x;
+                E_lj += 4.0 * ep * (x * x - x);
+            }
+            if (r2 < rc_ew2 && s->charged) {
+                double r = sqrt(r2);
+                E_sr += s->gq[a] * hq * or_pair_erfc_f64(s->alpha * r) / r;
+            }
+        }
+    }
+    double E_lr = 0.0;
+    for (int t = 0; t < s->nk; ++t) {
+        int i = s->kidx[t];
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < s->nsites; ++a) {
+            double ph = or_dot3(s->kfull + 3 * i, gp + 3 * a);
+            re += s->gq[a] * cos(ph);
+            im += s->gq[a] * sin(ph);
+        }
+        E_lr += s->kpref_full[i] * 2.0 * (s->Sre_full[i] * re + s->Sim_full[i] * im);
+    }
+    if (npairs) *npairs = np;
+    return E_lj + OR_KE * (E_sr + E_lr);
+}
+
+/* insertion_energy_reference (src/reference.jl:31-67): per-site minimum image, erfc_dev,
+ * FULL k table with the cross AND the guest self term. Without constant_offset pieces. */
+double or_insertion_energy_reference(const or_system *s, const double pos[3], const double q[4])
+{
+    double rc_lj2 = s->rc_lj * s->rc_lj, rc_ew2 = s->rc_ew * s->rc_ew;
+    double gp[3 * OR_MAXSITES];
+    for (int a = 0; a < s->nsites; ++a) {
+        double r[3];
+        or_rotate(q, s->gsite + 3 * a, r);
+        for (int i = 0; i < 3; ++i) gp[3 * a + i] = pos[i] + r[i];
+    }
+    double E_lj = 0.0, E_sr = 0.0;
+    for (int a = 0; a < s->nsites; ++a) {
+        int gt = s->gtype[a];
+        for (int j = 0; j < s->natoms; ++j) {
+            const double *hp = s->pos + 3 * j;
+            double d[3] = {gp[3 * a] - hp[0], gp[3 * a + 1] - hp[1], gp[3 * a + 2] - hp[2]};
+            double m[3];
+            or_minimum_image(s->A, s->invA, d, m);
+            double r2 = or_dot3(m, m);
+            if (!(r2 < rc_lj2 || r2 < rc_ew2)) continue;
+            if (r2 < rc_lj2) {
+                int ht = s->type[j];
+                double sg = s->sigma[gt + s->ntypes * ht], ep = s->eps[gt + s->ntypes * ht];
+                double x = sg * sg / r2;
+                x = x * x * x;
+                E_lj += 4.0 * ep * (x * x - x);
+            }
+            if (r2 < rc_ew2) {
+                double r = sqrt(r2);
+                E_sr += s->gq[a] * s->q[j] * or_erfc_dev(s->alpha * r) / r;
+            }
+        }
+    }
+    double E_lr = 0.0;
+    for (int i = 0; i < s->nkfull; ++i) {
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < s->nsites; ++a) {
+            double ph = or_dot3(s->kfull + 3 * i, gp + 3 * a);
+            re += s->gq[a] * cos(ph);
+            im += s->gq[a] * sin(ph);
+        }
+        E_lr += s->kpref_full[i] * (2.0 * (s->Sre_full[i] * re + s->Sim_full[i] * im) + (re * re + im * im));
+    }
+    return E_lj + OR_KE * (E_sr + E_lr);
+}
+
+/* Per-insertion Delta U = insertion_energy + constant_offset for insertions j0..j0+n-1 of
+ * system `sysidx` under `seed` (the widom_kernel! contract, src/widom.jl:157-175). Optional
+ * outputs: poses (7 doubles each: pos, quat) and the in-cutoff pair count per insertion. */
+void or_energies(const or_system *s, uint64_t seed, int64_t sysidx, int64_t j0, int64_t n, double *dU,
+                 double *poses, int64_t *npairs, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < n; ++t) {
+        double pos[3], q[4];
+        or_pose(seed, sysidx, j0 + t, s->A, pos, q);
+        int64_t np = 0;
+        double e = or_insertion_energy(s, pos, q, &np);
+        dU[t] = e + s->constant_offset;
+        if (poses) {
+            for (int i = 0; i < 3; ++i) poses[7 * t + i] = pos[i];
+            for (int i = 0; i < 4; ++i) poses[7 * t + 3 + i] = q[i];
+        }
+        if (npairs) npairs[t] = np;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Widom bookkeeping: src/widom.jl:34-53 (system_counts, default_run), :80-84
+ * (boltzmann_weight), :394-441 (per-system blocks), :451-492 (_reduce).
+ * ---------------------------------------------------------------------------------------- */
+int64_t or_default_run(int64_t ninsert, int64_t nsys)
+{
+    int64_t r = (ninsert / nsys) / 4;
+    if (r < 1) r = 1;
+    if (r > 256) r = 256;
+    return r;
+}
+
+void or_system_counts(int64_t ninsert, int64_t nsys, int64_t run, int64_t *ns)
+{
+    int64_t q = ninsert / run, rem = ninsert % run;
+    int64_t full = q / nsys, hi = q % nsys;
+    for (int64_t s = 0; s < nsys; ++s) ns[s] = full * run;
+    for (int64_t r = 0; r < hi; ++r) ns[r] += run;
+    if (rem > 0) ns[q % nsys] += rem;
+}
+
+void or_boltzmann_weight(double dU, double kT, double *w, double *uw)
+{
+    *w = exp(-dU / kT);
+    *uw = (*w == 0.0) ? 0.0 : dU * *w;
+}
+
+/* Returns 0, or -1 (zero-sample block), -2 (W/UW overflow), -3 (mu_ex or q_st non-finite).
+ * out: mu_ex, mu_ex_err, K_H, K_H_err, q_st, q_st_err, nsamples, nblocks. */
+int or_reduce(int nb, const double *sW, const double *sUW, const int64_t *n, double kT, double V, double out[8])
+{
+    int64_t ntot = 0;
+    double tW = 0.0, tUW = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        if (n[b] <= 0) return -1;
+        ntot += n[b];
+        tW += sW[b];
+        tUW += sUW[b];
+    }
+    double W = tW / (double)ntot, UW = tUW / (double)ntot;
+    if (!(isfinite(W) && isfinite(UW))) return -2;
+    double varW = 0.0, varUW = 0.0, cov = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        double mW = sW[b] / (double)n[b], mUW = sUW[b] / (double)n[b];
+        varW += (mW - W) * (mW - W);
+        varUW += (mUW - UW) * (mUW - UW);
+        cov += (mW - W) * (mUW - UW);
+    }
+    varW /= (nb - 1);
+    varUW /= (nb - 1);
+    cov /= (nb - 1);
+    double semW = sqrt(varW / nb);
+    double ratio = UW / W;
+    double var_ratio = UW == 0.0 ? 0.0 : ratio * ratio * (varUW / (UW * UW) + varW / (W * W) - 2.0 * cov / (UW * W)) / nb;
+    double mu_ex = -kT * log(W);
+    double q_st = kT - ratio;
+    if (!isfinite(mu_ex) || !isfinite(q_st)) return -3;
+    out[0] = mu_ex;
+    out[1] = kT * semW / W;
+    out[2] = V * W / kT;
+    out[3] = V * semW / kT;
+    out[4] = q_st;
+    out[5] = sqrt(var_ratio > 0.0 ? var_ratio : 0.0);
+    out[6] = (double)ntot;
+    out[7] = (double)nb;
+    return 0;
+}
+
+/* Widom for one system of a batch: ns insertions keyed (seed, sysidx, j), blocks of
+ * ns/nblocks with the remainder in the last block, FP64 Boltzmann sums in sample order.
+ * block_sums: 3*nblocks (sW, sUW, n). Returns or_reduce's status. */
+int or_widom(const or_system *s, double T, int64_t ns, uint64_t seed, int64_t sysidx, int nblocks,
+             double *block_sums, double out[8], int nthreads)
+{
+    double kT = OR_KB * T;
+    double *dU = malloc(sizeof(double) * (size_t)(ns > 0 ? ns : 1));
+    or_energies(s, seed, sysidx, 0, ns, dU, NULL, NULL, nthreads);
+    double *sW = calloc((size_t)nblocks, sizeof(double));
+    double *sUW = calloc((size_t)nblocks, sizeof(double));
+    int64_t *n = calloc((size_t)nblocks, sizeof(int64_t));
+    int64_t block_len = ns / nblocks;
+    for (int64_t j = 0; j < ns; ++j) {
+        int64_t blk = block_len > 0 ? j / block_len : nblocks - 1;
+        if (blk > nblocks - 1) blk = nblocks - 1;
+        double w, uw;
+        or_boltzmann_weight(dU[j], kT, &w, &uw);
+        sW[blk] += w;
+        sUW[blk] += uw;
+        n[blk] += 1;
+    }
+    for (int b = 0; b < nblocks; ++b) {
+        block_sums[3 * b] = sW[b];
+        block_sums[3 * b + 1] = sUW[b];
+        block_sums[3 * b + 2] = (double)n[b];
+    }
+    int st = or_reduce(nblocks, sW, sUW, n, kT, s->V, out);
+    free(dU); free(sW); free(sUW); free(n);
+    return st;
+}

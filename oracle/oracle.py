@@ -1,0 +1,345 @@
+"""ctypes front end of the FP64 C oracle (``oracle.c``) -- TEST INFRASTRUCTURE ONLY.
+
+Each wrapper names the reference function it restates (file:line into PureAdsorb.jl).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+KB = 1.38064852e-23 / 1.6021766208e-19  # src/constants.jl:3
+KE = 14.399645351950548  # src/constants.jl:4
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    """Compile ``liboracle.so`` with the committed Makefile (gcc, no GPU code)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    L.or_splitmix64.restype = C.c_uint64
+    L.or_splitmix64.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_mulhi64.restype = C.c_uint64
+    L.or_mulhi64.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_unit_f64.restype = C.c_double
+    L.or_unit_f64.argtypes = [C.c_uint64]
+    L.or_unit_f32.restype = C.c_float
+    L.or_unit_f32.argtypes = [C.c_uint64]
+    L.or_chain_words.restype = None
+    L.or_chain_words.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, _up]
+    L.or_rand_range.restype = C.c_int64
+    L.or_rand_range.argtypes = [C.c_uint64, C.c_int64]
+    L.or_sincospi.restype = None
+    L.or_sincospi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_shoemake.restype = None
+    L.or_shoemake.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+    L.or_pose.restype = None
+    L.or_pose.argtypes = [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp, _dp]
+    L.or_cell_matrix.restype = C.c_int
+    L.or_cell_matrix.argtypes = [C.c_double] * 6 + [_dp]
+    L.or_inv3.restype = None
+    L.or_inv3.argtypes = [_dp, _dp]
+    L.or_det3.restype = C.c_double
+    L.or_det3.argtypes = [_dp]
+    L.or_perpendicular_lengths.restype = None
+    L.or_perpendicular_lengths.argtypes = [_dp, _dp]
+    L.or_minimum_image.restype = None
+    L.or_minimum_image.argtypes = [_dp, _dp, _dp, _dp]
+    L.or_rotate.restype = None
+    L.or_rotate.argtypes = [_dp, _dp, _dp]
+    L.or_erfc_dev.restype = C.c_double
+    L.or_erfc_dev.argtypes = [C.c_double]
+    L.or_pair_erfc_f64.restype = C.c_double
+    L.or_pair_erfc_f64.argtypes = [C.c_double]
+    L.or_pair_erfc_f32.restype = C.c_float
+    L.or_pair_erfc_f32.argtypes = [C.c_float]
+    L.or_ewald_alpha.restype = C.c_double
+    L.or_ewald_alpha.argtypes = [C.c_double, C.c_double]
+    L.or_ewald_kmax.restype = C.c_double
+    L.or_ewald_kmax.argtypes = [C.c_double, C.c_double]
+    L.or_pk.restype = C.c_double
+    L.or_pk.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.or_kvectors.restype = C.c_int
+    L.or_kvectors.argtypes = [_dp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_structure_factor.restype = None
+    L.or_structure_factor.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    L.or_ewald_energy.restype = C.c_double
+    L.or_ewald_energy.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.or_tail_delta.restype = C.c_double
+    L.or_tail_delta.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _lp, _lp, C.c_double]
+    L.or_self_orientation.restype = None
+    L.or_self_orientation.argtypes = [C.c_int, _dp]
+    L.or_system_new.restype = C.c_void_p
+    L.or_system_new.argtypes = [
+        _dp, C.c_int, _dp, _ip, _dp, _ip, C.c_int, _dp, _dp, C.c_double, C.c_int,
+        C.c_int, _dp, _ip, _dp, C.c_double, C.c_double,
+    ]
+    L.or_system_free.restype = None
+    L.or_system_free.argtypes = [C.c_void_p]
+    L.or_system_status.restype = C.c_int
+    L.or_system_status.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    L.or_system_scalars.restype = None
+    L.or_system_scalars.argtypes = [C.c_void_p, _dp]
+    L.or_system_ktable.restype = None
+    L.or_system_ktable.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _ip]
+    L.or_insertion_energy.restype = C.c_double
+    L.or_insertion_energy.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int64)]
+    L.or_insertion_energy_reference.restype = C.c_double
+    L.or_insertion_energy_reference.argtypes = [C.c_void_p, _dp, _dp]
+    L.or_energies.restype = None
+    L.or_energies.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_void_p, C.c_void_p, C.c_int]
+    L.or_default_run.restype = C.c_int64
+    L.or_default_run.argtypes = [C.c_int64, C.c_int64]
+    L.or_system_counts.restype = None
+    L.or_system_counts.argtypes = [C.c_int64, C.c_int64, C.c_int64, _lp]
+    L.or_reduce.restype = C.c_int
+    L.or_reduce.argtypes = [C.c_int, _dp, _dp, _lp, C.c_double, C.c_double, _dp]
+    L.or_widom.restype = C.c_int
+    L.or_widom.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, C.c_int]
+    _lib = L
+    return L
+
+
+# --- scalar helpers -------------------------------------------------------------------------
+def splitmix64(seed: int, index: int) -> int:
+    """src/state.jl:225-230"""
+    return lib().or_splitmix64(seed & (2**64 - 1), index & (2**64 - 1))
+
+
+def mulhi64(a: int, b: int) -> int:
+    """src/rng.jl:14-24"""
+    return lib().or_mulhi64(a, b)
+
+
+def chain_words(chain_seed: int, cycle: int, move: int, n: int) -> np.ndarray:
+    """Words 0..n-1 of ChainRNG(chain_seed, cycle, move) (src/rng.jl:43-61)."""
+    out = np.zeros(n, dtype=np.uint64)
+    lib().or_chain_words(chain_seed & (2**64 - 1), cycle & (2**64 - 1), move & (2**64 - 1), n, out)
+    return out
+
+
+def unit_f64(raw: int) -> float:
+    return lib().or_unit_f64(raw)
+
+
+def unit_f32(raw: int) -> float:
+    return lib().or_unit_f32(raw)
+
+
+def rand_range(raw: int, n: int) -> int:
+    return lib().or_rand_range(raw, n)
+
+
+def sincospi(x: float) -> tuple[float, float]:
+    s, c = C.c_double(), C.c_double()
+    lib().or_sincospi(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def shoemake(u1: float, u2: float, u3: float) -> np.ndarray:
+    """src/rng.jl:126-129"""
+    q = np.zeros(4)
+    lib().or_shoemake(u1, u2, u3, q)
+    return q
+
+
+def pose(seed: int, sys: int, j: int, A: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """propose_reinsertion keyed ChainRNG(splitmix64(seed, sys+1), j+1, 0) (src/moves.jl:154-161)."""
+    pos, q = np.zeros(3), np.zeros(4)
+    lib().or_pose(seed & (2**64 - 1), sys, j, _colmajor(A), pos, q)
+    return pos, q
+
+
+def _colmajor(A) -> np.ndarray:
+    A = np.asarray(A, dtype=np.float64)
+    return np.ascontiguousarray(A.reshape(3, 3).T.reshape(9)) if A.shape == (3, 3) else np.ascontiguousarray(A)
+
+
+def cell_matrix(a, b, c, alpha, beta, gamma) -> np.ndarray:
+    """src/cell.jl:2-12; returns the 3x3 matrix (lattice vectors as columns)."""
+    A = np.zeros(9)
+    st = lib().or_cell_matrix(a, b, c, alpha, beta, gamma, A)
+    if st != 0:
+        raise ValueError(f"degenerate cell (status {st})")
+    return A.reshape(3, 3, order="F")
+
+
+def erfc_dev(x: float) -> float:
+    return lib().or_erfc_dev(x)
+
+
+def pair_erfc_f64(x: float) -> float:
+    return lib().or_pair_erfc_f64(x)
+
+
+def pair_erfc_f32(x: float) -> float:
+    return lib().or_pair_erfc_f32(x)
+
+
+def ewald_alpha(cutoff: float, precision: float) -> float:
+    return lib().or_ewald_alpha(cutoff, precision)
+
+
+def ewald_kmax(alpha: float, precision: float) -> float:
+    return lib().or_ewald_kmax(alpha, precision)
+
+
+def kvectors(A, kmax: float):
+    """src/ewald.jl:142-157: (ks (n,3), weights (n,), coeffs (n,3))."""
+    Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.reshape(9))
+    n = lib().or_kvectors(Acm, kmax, 0, None, None, None)
+    ks = np.zeros(3 * max(n, 1))
+    w = np.zeros(max(n, 1))
+    co = np.zeros(3 * max(n, 1), dtype=np.int32)
+    lib().or_kvectors(Acm, kmax, n, ks.ctypes.data, w.ctypes.data, co.ctypes.data)
+    return ks[: 3 * n].reshape(n, 3), w[:n], co[: 3 * n].reshape(n, 3)
+
+
+def ewald_energy(A, pos, q, mol, alpha, cutoff, ks, w) -> float:
+    """src/ewald.jl:177-203"""
+    Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.reshape(9))
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    mol = np.ascontiguousarray(mol, dtype=np.int32)
+    ks = np.ascontiguousarray(ks, dtype=np.float64).reshape(-1)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    return lib().or_ewald_energy(Acm, len(q), pos, q, mol, alpha, cutoff, len(w), ks if len(ks) else np.zeros(1), w if len(w) else np.zeros(1))
+
+
+def self_orientation(o: int) -> np.ndarray:
+    q = np.zeros(4)
+    lib().or_self_orientation(o, q)
+    return q
+
+
+def default_run(ninsert: int, nsys: int) -> int:
+    return lib().or_default_run(ninsert, nsys)
+
+
+def system_counts(ninsert: int, nsys: int, run: int) -> np.ndarray:
+    ns = np.zeros(nsys, dtype=np.int64)
+    lib().or_system_counts(ninsert, nsys, run, ns)
+    return ns
+
+
+def reduce(sW, sUW, n, kT, V):
+    """src/widom.jl:451-492. Returns (status, 8 fields)."""
+    out = np.zeros(8)
+    sW = np.ascontiguousarray(sW, dtype=np.float64)
+    sUW = np.ascontiguousarray(sUW, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.int64)
+    st = lib().or_reduce(len(sW), sW, sUW, n, kT, V, out)
+    return st, out
+
+
+class OracleSystem:
+    """One framework system with FrameworkBatch's per-framework tables (src/batch.jl:319-408).
+
+    ``cell`` is the 3x3 cell matrix, ``frac`` (natoms, 3), ``types`` force-field type indices
+    (0-based) into the ``sigma``/``epsilon`` tables, ``guest_types`` likewise.
+    """
+
+    def __init__(self, cell, frac, types, charges, replication, sigma, epsilon, cutoff, tail,
+                 guest_sites, guest_types, guest_charges, ewald_cutoff, precision=1e-6):
+        L = lib()
+        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        Acm = np.ascontiguousarray(self.cell.T.reshape(9))
+        frac = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1)
+        self.natoms = len(frac) // 3
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        charges = np.ascontiguousarray(charges, dtype=np.float64)
+        rep = np.ascontiguousarray(replication, dtype=np.int32)
+        sigma = np.asarray(sigma, dtype=np.float64)
+        epsilon = np.asarray(epsilon, dtype=np.float64)
+        nt = sigma.shape[0]
+        sig = np.ascontiguousarray(sigma.T.reshape(-1))
+        eps = np.ascontiguousarray(epsilon.T.reshape(-1))
+        gs = np.ascontiguousarray(guest_sites, dtype=np.float64).reshape(-1)
+        gt = np.ascontiguousarray(guest_types, dtype=np.int32)
+        gq = np.ascontiguousarray(guest_charges, dtype=np.float64)
+        if self.natoms == 0:
+            frac = np.zeros(3)
+            types = np.zeros(1, dtype=np.int32)
+            charges = np.zeros(1)
+        self._h = L.or_system_new(Acm, self.natoms, frac, types, charges, rep, nt, sig, eps, cutoff, int(bool(tail)),
+                                  len(gq), gs, gt, gq, ewald_cutoff, precision)
+        buf = C.create_string_buffer(512)
+        st = L.or_system_status(self._h, buf, 512)
+        if st != 0:
+            msg = buf.value.decode()
+            L.or_system_free(self._h)
+            self._h = None
+            raise ValueError(msg)
+        sc = np.zeros(12)
+        L.or_system_scalars(self._h, sc)
+        (self.V, self.alpha, self.kmax, self.constant_offset, self.self_mean, self.self_halfrange,
+         self.tail_term, self.self_term, self.excl_term, self.net_term) = sc[:10]
+        self.nk = int(sc[10])
+        self.nkfull = int(sc[11])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_system_free(self._h)
+            self._h = None
+
+    def ktable(self, full=False):
+        n = self.nkfull if full else self.nk
+        ks = np.zeros(3 * max(n, 1)); kp = np.zeros(max(n, 1)); sr = np.zeros(max(n, 1)); si = np.zeros(max(n, 1))
+        co = np.zeros(3 * max(n, 1), dtype=np.int32)
+        lib().or_system_ktable(self._h, int(full), ks, kp, sr, si, co)
+        return ks[: 3 * n].reshape(n, 3), kp[:n], sr[:n] + 1j * si[:n], co[: 3 * n].reshape(n, 3)
+
+    def insertion_energy(self, pos, q):
+        """src/energy.jl:42-82 (per-site imaging); returns (E, in-cutoff pair count)."""
+        npair = C.c_int64()
+        e = lib().or_insertion_energy(self._h, np.ascontiguousarray(pos, dtype=np.float64),
+                                      np.ascontiguousarray(q, dtype=np.float64), C.byref(npair))
+        return e, npair.value
+
+    def insertion_energy_reference(self, pos, q):
+        """src/reference.jl:31-67"""
+        return lib().or_insertion_energy_reference(self._h, np.ascontiguousarray(pos, dtype=np.float64),
+                                                   np.ascontiguousarray(q, dtype=np.float64))
+
+    def energies(self, seed, sysidx, j0, n, want_poses=False, want_pairs=False, nthreads=0):
+        """Delta U (insertion_energy + constant_offset) for insertions j0..j0+n-1."""
+        dU = np.zeros(max(n, 1))
+        poses = np.zeros(7 * max(n, 1)) if want_poses else None
+        pairs = np.zeros(max(n, 1), dtype=np.int64) if want_pairs else None
+        lib().or_energies(self._h, seed & (2**64 - 1), sysidx, j0, n, dU,
+                          poses.ctypes.data if poses is not None else None,
+                          pairs.ctypes.data if pairs is not None else None, nthreads)
+        out = [dU[:n]]
+        if want_poses:
+            out.append(poses[: 7 * n].reshape(n, 7))
+        if want_pairs:
+            out.append(pairs[:n])
+        return out[0] if len(out) == 1 else tuple(out)
+
+    def widom(self, T, ns, seed, sysidx, nblocks, nthreads=0):
+        """One system's Widom result (src/widom.jl:290-492, FP64, no rejection).
+
+        Returns (status, fields[8], block_sums (nblocks, 3))."""
+        bs = np.zeros(3 * nblocks)
+        out = np.zeros(8)
+        st = lib().or_widom(self._h, T, ns, seed & (2**64 - 1), sysidx, nblocks, bs, out, nthreads)
+        return st, out, bs.reshape(nblocks, 3)

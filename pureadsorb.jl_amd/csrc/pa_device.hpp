@@ -1,0 +1,61 @@
+// pa_device.hpp -- device-side data layout shared by the host library and the kernels.
+#pragma once
+
+#include "pa_numeric.hpp"
+
+namespace pa {
+
+// Per unique framework (FrameworkBatch's deduplicated per-framework storage).
+struct FwDesc {
+    double A[9];       // cell, column-major
+    double invA[9];
+    float Af[9];       // A * 2^-32 (maps a 32-bit fixed-point fractional offset to Å)
+    double alpha;      // Ewald splitting parameter
+    double const_off;  // constant_offset (eV)
+    double volume;
+    float alphaf;
+    int atom_off, natoms;  // slice of the atom arrays
+    int k_off, nk;         // slice of the k tables
+    int upper;             // A is upper triangular (cell_matrix's form)
+};
+
+// The guest, with compact type indices.
+struct GuestDev {
+    int nsites;
+    int charged;
+    double site[MAX_SITES][3];
+    double q[MAX_SITES];
+    float qke[MAX_SITES];   // KE * q_s in FP32
+    int type[MAX_SITES];
+};
+
+// Everything one energy launch needs. The flattened insertion index g runs system-major:
+// system s owns g in [seg[s], seg[s+1]); insertion j of system s is g = seg[s] + j.
+struct WidomArgs {
+    const FwDesc *fw;
+    const int *fw_of;          // nsys
+    const int64_t *seg;        // nsys + 1
+    int nsys;
+    uint64_t seed;
+    int64_t g0, g1;            // this launch covers g in [g0, g1)
+    double *dU;                // indexed by g - out_base
+    int64_t out_base;
+    // FP64 plain path tables
+    const double4 *atom64;     // x, y, z (Å), q
+    const int *atype;          // compact type per atom
+    const double4 *k64;        // kx, ky, kz, kprefactor
+    const double2 *S64;        // host structure factor
+    // mixed path tables
+    const uint4 *atomfx;       // 32-bit fixed-point fractional x, y, z; q as float bits
+    const int4 *kn;            // integer k coefficients n1, n2, n3
+    const float2 *kc;          // 2 * kprefactor * (Re S, Im S); the guest charges carry KE
+    // pair tables, [site][type] with ntypes_c compact types
+    const double2 *pair64;     // sigma, epsilon
+    const float2 *pair32;      // sigma^2, 4 epsilon
+    int ntypes_c;
+    double rc_lj2, rc_ew2;
+    float rc_lj2f, rc_ew2f, rc_max2f;
+    GuestDev guest;
+};
+
+}  // namespace pa

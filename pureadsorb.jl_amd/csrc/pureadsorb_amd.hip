@@ -1,0 +1,953 @@
+// pureadsorb_amd.hip -- host side of the MI355X Widom backend: FrameworkBatch table builder
+// (FP64, multithreaded) and the C ABI declared in include/pureadsorb_amd.h.
+#include <pureadsorb_amd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pa_device.hpp"
+
+namespace pa {
+hipError_t launch_energy(const WidomArgs &a, int precision, bool upper, hipStream_t st);
+hipError_t launch_tiles(const double *dU, const int64_t *tile_g0, const int *tile_n, int ntiles, double kT, double4 *out,
+                        hipStream_t st);
+hipError_t launch_poses(const FwDesc *fw, const int *fw_of, int sys, uint64_t seed, int64_t j0, int64_t n, double *pos,
+                        double *quat, hipStream_t st);
+}  // namespace pa
+
+using namespace pa;
+
+// ------------------------------------------------------------------------------------------
+// Errors
+// ------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+static int ok()
+{
+    g_err.clear();
+    return PA_OK;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return fail(PA_ERR_DEVICE, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), \
+                                          __FILE__, __LINE__, #expr);                                    \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// Host numerics (FP64)
+// ------------------------------------------------------------------------------------------
+namespace {
+
+// erfc_dev: 28-term Chebyshev series of log(erfc(z)/t) + z^2, t = 2/(2+z)
+// (src/ewald.jl:47-74; Numerical Recipes 3rd ed. section 6.2.2). Used for the guest
+// intramolecular exclusion term exactly as the reference does (src/batch.jl:402).
+double erfc_dev(double x)
+{
+    static const double cof[28] = {
+        -1.3026537197817094, 6.419697923564902e-1, 1.9476473204185836e-2,
+        -9.561514786808632e-3, -9.465953444820369e-4, 3.6683949785276145e-4, 4.252332480690777e-5,
+        -2.0278578112534242e-5, -1.6242900046470256e-6, 1.3036558355805232e-6, 1.5626441722066142e-8,
+        -8.523809591492654e-8, 6.5290544390988515e-9, 5.0593434955514685e-9, -9.913641564930322e-10,
+        -2.2736512229318417e-10, 9.646791102014962e-11, 2.394038083065756e-12, -6.886027526536277e-12,
+        8.944879271730368e-13, 3.130921408127142e-13, -1.1270822525221733e-13, 3.8108713588528985e-16,
+        7.106125536922576e-15, -1.5230878975373303e-15, -9.464871412184462e-17, 1.218683266906788e-16,
+        -3.0494735135424696e-17};
+    auto cheb = [&](double z) {
+        double t = 2.0 / (2.0 + z), ty = 4.0 * t - 2.0, d = 0.0, dd = 0.0;
+        for (int j = 27; j >= 1; --j) {
+            double tmp = d;
+            d = ty * d - dd + cof[j];
+            dd = tmp;
+        }
+        return t * std::exp(-z * z + (ty / 2.0) * d - dd + cof[0] / 2.0);
+    };
+    return x >= 0 ? cheb(x) : 2.0 - cheb(-x);
+}
+
+double det3(const double A[9])
+{
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) + A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+void inv3(const double A[9], double inv[9])
+{
+    double d = det3(A);
+    inv[0] = (A[4] * A[8] - A[7] * A[5]) / d;
+    inv[1] = -(A[1] * A[8] - A[7] * A[2]) / d;
+    inv[2] = (A[1] * A[5] - A[4] * A[2]) / d;
+    inv[3] = -(A[3] * A[8] - A[6] * A[5]) / d;
+    inv[4] = (A[0] * A[8] - A[6] * A[2]) / d;
+    inv[5] = -(A[0] * A[5] - A[3] * A[2]) / d;
+    inv[6] = (A[3] * A[7] - A[6] * A[4]) / d;
+    inv[7] = -(A[0] * A[7] - A[6] * A[1]) / d;
+    inv[8] = (A[0] * A[4] - A[3] * A[1]) / d;
+}
+
+double norm3(const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+void cross3(const double *a, const double *b, double *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// src/cell.jl:14-20
+void perpendicular_lengths(const double A[9], double L[3])
+{
+    double V = std::fabs(det3(A)), x[3];
+    cross3(A + 3, A + 6, x); L[0] = V / norm3(x);
+    cross3(A + 0, A + 6, x); L[1] = V / norm3(x);
+    cross3(A + 0, A + 3, x); L[2] = V / norm3(x);
+}
+
+// ewald_alpha: bisection on erfc(z) = cutoff*precision/2 (src/ewald.jl:21-32).
+double ewald_alpha(double cutoff, double precision)
+{
+    double target = cutoff * precision / 2.0, lo = 0.0, hi = 20.0;
+    if (!(std::erfc(hi) - target < 0)) return -1.0;
+    for (int it = 0; it < 200; ++it) {
+        double mid = (lo + hi) / 2.0;
+        if (std::erfc(mid) - target > 0) lo = mid; else hi = mid;
+        if (hi - lo < 1.0e-12) break;
+    }
+    return (lo + hi) / 2.0 / cutoff;
+}
+
+template <class F>
+void parallel_for(int64_t n, F &&fn)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 64 || nt == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    int64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t b = (int64_t)t * per, e = std::min(n, b + per);
+        if (b >= e) break;
+        th.emplace_back([&fn, b, e] { fn(b, e); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// k-vector table of one stored cell (src/ewald.jl:142-157), host structure factor
+// (src/ewald.jl:159-170) and prefactors (src/ewald.jl:36-39).
+struct KTable {
+    std::vector<double> k;       // 3 per vector, Cartesian
+    std::vector<int> n;          // 3 per vector, reciprocal-basis coefficients
+    std::vector<double> kpref;   // w * pk(|k|^2, alpha, V)
+    std::vector<double> Sre, Sim;
+};
+
+void build_ktable(const double A[9], double kmax, double alpha, double V, KTable &kt)
+{
+    double invA[9], B[9];
+    inv3(A, invA);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[i + 3 * j] = TWO_PI * invA[j + 3 * i];
+    int nmax[3];
+    for (int i = 0; i < 3; ++i) nmax[i] = (int)std::ceil(kmax * norm3(A + 3 * i) / TWO_PI);
+    for (int n1 = 0; n1 <= nmax[0]; ++n1)
+        for (int n2 = -nmax[1]; n2 <= nmax[1]; ++n2)
+            for (int n3 = -nmax[2]; n3 <= nmax[2]; ++n3) {
+                if (n1 == 0 && n2 == 0 && n3 == 0) continue;
+                double k[3];
+                for (int i = 0; i < 3; ++i) k[i] = B[i] * n1 + B[i + 3] * n2 + B[i + 6] * n3;
+                if (!(norm3(k) <= kmax)) continue;
+                kt.k.insert(kt.k.end(), k, k + 3);
+                kt.n.push_back(n1); kt.n.push_back(n2); kt.n.push_back(n3);
+                double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                double w = n1 == 0 ? 1.0 : 2.0;
+                kt.kpref.push_back(w * (2.0 * PI / V * std::exp(-k2 / (4.0 * alpha * alpha)) / k2));
+            }
+}
+
+// S(k) = sum_j q_j exp(i k.r_j) over the framework's atoms. The phase of atom j at k = B n is
+// 2 pi n.f_j; exp(i 2 pi n_d f_jd) is tabulated per atom and axis, so each (k, atom) term is
+// two complex products instead of a sincos.
+void structure_factor(const KTable &kt, const std::vector<double> &frac, const std::vector<double> &q,
+                      std::vector<double> &Sre, std::vector<double> &Sim)
+{
+    const int64_t nk = (int64_t)kt.kpref.size(), na = (int64_t)q.size();
+    Sre.assign(nk, 0.0);
+    Sim.assign(nk, 0.0);
+    if (nk == 0 || na == 0) return;
+    int nmax[3] = {0, 0, 0};
+    for (int64_t i = 0; i < nk; ++i)
+        for (int d = 0; d < 3; ++d) nmax[d] = std::max(nmax[d], std::abs(kt.n[3 * i + d]));
+    const int w0 = nmax[0] + 1, w1 = 2 * nmax[1] + 1, w2 = 2 * nmax[2] + 1;
+    std::vector<double> c0((size_t)na * w0), s0((size_t)na * w0), c1((size_t)na * w1), s1((size_t)na * w1),
+        c2((size_t)na * w2), s2((size_t)na * w2);
+    parallel_for(na, [&](int64_t b, int64_t e) {
+        for (int64_t j = b; j < e; ++j) {
+            for (int m = 0; m < w0; ++m) {
+                double ph = TWO_PI * (m * frac[3 * j]);
+                c0[j * w0 + m] = std::cos(ph); s0[j * w0 + m] = std::sin(ph);
+            }
+            for (int m = 0; m < w1; ++m) {
+                double ph = TWO_PI * ((m - nmax[1]) * frac[3 * j + 1]);
+                c1[j * w1 + m] = std::cos(ph); s1[j * w1 + m] = std::sin(ph);
+            }
+            for (int m = 0; m < w2; ++m) {
+                double ph = TWO_PI * ((m - nmax[2]) * frac[3 * j + 2]);
+                c2[j * w2 + m] = std::cos(ph); s2[j * w2 + m] = std::sin(ph);
+            }
+        }
+    });
+    parallel_for(nk, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            const int m0 = kt.n[3 * i], m1 = kt.n[3 * i + 1] + nmax[1], m2 = kt.n[3 * i + 2] + nmax[2];
+            double re = 0.0, im = 0.0;
+            for (int64_t j = 0; j < na; ++j) {
+                double ar = c0[j * w0 + m0], ai = s0[j * w0 + m0];
+                double br = c1[j * w1 + m1], bi = s1[j * w1 + m1];
+                double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+                double cr = c2[j * w2 + m2], ci = s2[j * w2 + m2];
+                double rr = pr * cr - pi * ci, ri = pr * ci + pi * cr;
+                re += q[j] * rr;
+                im += q[j] * ri;
+            }
+            Sre[i] = re;
+            Sim[i] = im;
+        }
+    });
+}
+
+// 64 fixed guest orientations for the orientation-averaged self term: ChainRNG(0x5e1f, 0, o),
+// o = 1..64, three F64 uniforms each (this build's replacement for the reference's
+// Xoshiro(0x5e1f) draws, src/batch.jl:302-303).
+void self_orientation(int o, double q[4])
+{
+    Chain c(0x5e1fULL, 0, (uint64_t)o);
+    double u1 = unit_f64(c.next()), u2 = unit_f64(c.next()), u3 = unit_f64(c.next());
+    shoemake(u1, u2, u3, q);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Batch
+// ------------------------------------------------------------------------------------------
+struct DevAlloc {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevAlloc() { if (p) (void)hipFree(p); }
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, n ? n : 16);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct FwHost {
+    int natoms, nk, nk_full;
+    double volume, alpha, const_off, halfrange;
+};
+
+struct pa_batch {
+    int nsys = 0, nfw = 0;
+    std::vector<int> fw_of;
+    std::vector<FwHost> info;
+    std::vector<FwDesc> desc;
+    bool upper_all = true;
+    // guest exactly as given (for the same-guest check) and its device form
+    int nsites = 0;
+    std::vector<double> gsites, gq;
+    std::vector<int> gtypes;
+    GuestDev gdev{};
+    int ntypes_c = 0;
+    double rc_lj = 0, rc_ew = 0;
+    // device tables
+    DevAlloc d_fw, d_fw_of, d_atom64, d_atype, d_k64, d_S64, d_atomfx, d_kn, d_kc, d_pair64, d_pair32;
+    // per-call scratch
+    DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    pa_timing last{};
+    bool host_only = false;
+    ~pa_batch()
+    {
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int device_ready()
+{
+    static std::once_flag once;
+    static int status = PA_OK;
+    static std::string msg;
+    std::call_once(once, [] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n < 1) {
+            status = PA_ERR_DEVICE;
+            msg = std::string("no HIP device available: ") + hipGetErrorString(e);
+        }
+    });
+    if (status != PA_OK) return fail(status, "%s", msg.c_str());
+    return PA_OK;
+}
+
+extern "C" int pa_last_error(char *buf, size_t len)
+{
+    if (buf && len) {
+        std::snprintf(buf, len, "%s", g_err.c_str());
+    }
+    return (int)g_err.size();
+}
+
+extern "C" int pa_device_name(char *buf, size_t len)
+{
+    int st = device_ready();
+    if (st) return st;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (buf && len) std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return ok();
+}
+
+template <class T>
+static hipError_t upload(DevAlloc &d, const std::vector<T> &v)
+{
+    hipError_t e = d.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_forcefield *ff, const pa_guest *guest,
+                               const pa_ewald *ewald, uint32_t flags, pa_batch **out)
+{
+    if (!out) return fail(PA_ERR_ARG, "out must not be NULL");
+    *out = nullptr;
+    if (!fws || !ff || !guest || !ewald) return fail(PA_ERR_ARG, "NULL argument");
+    if (nsys < 1) return fail(PA_ERR_ARG, "nsys=%d must be >= 1", nsys);
+    if (guest->nsites < 1 || guest->nsites > MAX_SITES)
+        return fail(PA_ERR_ARG, "guest has %d sites; 1..%d are supported", guest->nsites, MAX_SITES);
+    if (ff->ntypes < 1 || !ff->sigma || !ff->epsilon) return fail(PA_ERR_ARG, "force field has no types");
+    const int NT = ff->ntypes, NS = guest->nsites;
+    for (int a = 0; a < NS; ++a)
+        if (guest->types[a] < 0 || guest->types[a] >= NT)
+            return fail(PA_ERR_ARG, "unknown LJ type index %d for guest site %d; force field defines %d types",
+                        guest->types[a], a, NT);
+    const bool fullk = (flags & PA_BATCH_FULLK) != 0;
+    const double rc_lj = ff->cutoff, rc_ew = ewald->cutoff;
+    const double rc = std::max(rc_lj, rc_ew);
+
+    std::unique_ptr<pa_batch> b(new pa_batch());
+    b->nsys = nsys;
+    b->nsites = NS;
+    b->gsites.assign(guest->sites, guest->sites + 3 * NS);
+    b->gq.assign(guest->charges, guest->charges + NS);
+    b->gtypes.assign(guest->types, guest->types + NS);
+    b->rc_lj = rc_lj;
+    b->rc_ew = rc_ew;
+    bool charged = false;
+    for (int a = 0; a < NS; ++a) charged = charged || guest->charges[a] != 0.0;
+
+    // ---- dedup by value: cell, replication, fractional coordinates, types, charges
+    // (src/batch.jl:246-265)
+    std::vector<int> uniq;
+    b->fw_of.resize(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        const pa_framework &F = fws[s];
+        if (F.natoms < 0) return fail(PA_ERR_DIM, "framework %d has natoms=%d", s, F.natoms);
+        if (F.natoms > 0 && (!F.frac || !F.types || !F.charges))
+            return fail(PA_ERR_ARG, "framework %d has NULL atom arrays", s);
+        for (int d = 0; d < 3; ++d)
+            if (F.replication[d] < 1) return fail(PA_ERR_ARG, "replication factors must be >= 1, got (%d, %d, %d)",
+                                                  F.replication[0], F.replication[1], F.replication[2]);
+        int found = -1;
+        for (size_t u = 0; u < uniq.size() && found < 0; ++u) {
+            const pa_framework &G = fws[uniq[u]];
+            if (G.natoms != F.natoms || std::memcmp(G.cell, F.cell, sizeof F.cell) ||
+                std::memcmp(G.replication, F.replication, sizeof F.replication))
+                continue;
+            size_t n = (size_t)F.natoms;
+            if (n && (std::memcmp(G.frac, F.frac, 3 * n * sizeof(double)) ||
+                      std::memcmp(G.types, F.types, n * sizeof(int32_t)) ||
+                      std::memcmp(G.charges, F.charges, n * sizeof(double))))
+                continue;
+            found = (int)u;
+        }
+        if (found < 0) {
+            uniq.push_back(s);
+            found = (int)uniq.size() - 1;
+        }
+        b->fw_of[s] = found;
+    }
+    b->nfw = (int)uniq.size();
+
+    // ---- compact LJ types: those present in any framework, union the guest's (src/batch.jl:305-317)
+    std::vector<int> present(NT, 0);
+    for (int u : uniq) {
+        const pa_framework &F = fws[u];
+        for (int j = 0; j < F.natoms; ++j) {
+            if (F.types[j] < 0 || F.types[j] >= NT)
+                return fail(PA_ERR_ARG, "unknown LJ type index %d (framework %d atom %d); force field defines %d types",
+                            F.types[j], u, j, NT);
+            present[F.types[j]] = 1;
+        }
+    }
+    for (int a = 0; a < NS; ++a) present[guest->types[a]] = 1;
+    std::vector<int> orig_to_c(NT, -1), c_to_orig;
+    for (int t = 0; t < NT; ++t)
+        if (present[t]) { orig_to_c[t] = (int)c_to_orig.size(); c_to_orig.push_back(t); }
+    const int NTC = (int)c_to_orig.size();
+    b->ntypes_c = NTC;
+    if (NTC > 64) return fail(PA_ERR_ARG, "%d distinct LJ types in the batch; at most 64 are supported", NTC);
+
+    // ---- Ewald parameters (src/batch.jl:289-297)
+    const double alpha = ewald_alpha(rc_ew, ewald->precision);
+    if (alpha < 0) return fail(PA_ERR_ARG, "cannot reach precision %g with cutoff %g", ewald->precision, rc_ew);
+    if (alpha * rc_ew > PAIR_ERFC_XMAX)
+        return fail(PA_ERR_ARG,
+                    "α·ewald_cutoff = %.17g exceeds PAIR_ERFC_XMAX = %.1f: insertion_energy's screened-Coulomb series is "
+                    "only fitted up to that bound; loosen ewald.precision or shorten ewald.cutoff to bring α·ewald_cutoff "
+                    "back under %.1f",
+                    alpha * rc_ew, PAIR_ERFC_XMAX, PAIR_ERFC_XMAX);
+    const double kmax = 2.0 * alpha * std::sqrt(-std::log(ewald->precision / 2.0));
+
+    std::vector<double> self_q(64 * 4);
+    for (int o = 0; o < 64; ++o) self_orientation(o + 1, &self_q[4 * o]);
+
+    std::vector<double4> atom64;
+    std::vector<int> atype;
+    std::vector<uint4> atomfx;
+    std::vector<double4> k64;
+    std::vector<double2> S64;
+    std::vector<int4> kn;
+    std::vector<float2> kc;
+    b->desc.resize(b->nfw);
+    b->info.resize(b->nfw);
+    for (int u = 0; u < b->nfw; ++u) {
+        const pa_framework &F = fws[uniq[u]];
+        const int n = F.natoms;
+        // per-pair minimum image is exact whenever every perpendicular width is >= 2*cutoff
+        double L[3];
+        perpendicular_lengths(F.cell, L);
+        int m[3];
+        for (int d = 0; d < 3; ++d) m[d] = (int)std::ceil(2.0 * rc / L[d]);
+        if (m[0] > 1 || m[1] > 1 || m[2] > 1)
+            return fail(PA_ERR_ARG,
+                        "framework %d is too small for cutoff %g (perpendicular widths %.4f, %.4f, %.4f Å must be >= %g); "
+                        "replicate it by (%d, %d, %d) first",
+                        uniq[u] + 1, rc, L[0], L[1], L[2], 2 * rc, m[0], m[1], m[2]);
+        FwDesc &D = b->desc[u];
+        std::memcpy(D.A, F.cell, sizeof D.A);
+        inv3(D.A, D.invA);
+        for (int i = 0; i < 9; ++i) D.Af[i] = (float)(D.A[i] * 2.3283064365386963e-10);
+        D.upper = (D.A[1] == 0.0 && D.A[2] == 0.0 && D.A[5] == 0.0) ? 1 : 0;
+        b->upper_all = b->upper_all && D.upper;
+        const double V = std::fabs(det3(D.A));
+        D.volume = V;
+        D.alpha = alpha;
+        D.alphaf = (float)alpha;
+        D.atom_off = (int)atom64.size();
+        D.natoms = n;
+        std::vector<double> frac(F.frac, F.frac + 3 * (size_t)n), q(F.charges, F.charges + n);
+        for (int j = 0; j < n; ++j) {
+            double f[3] = {frac[3 * j], frac[3 * j + 1], frac[3 * j + 2]}, p[3];
+            matvec(D.A, f, p);
+            atom64.push_back(make_double4(p[0], p[1], p[2], q[j]));
+            atype.push_back(orig_to_c[F.types[j]]);
+            float qf = (float)q[j];
+            uint32_t qb;
+            std::memcpy(&qb, &qf, 4);
+            atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
+        }
+        const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
+        KTable kt;
+        double self_mean = 0.0, halfrange = 0.0;
+        D.k_off = (int)k64.size();
+        if (charged || !rep1) {
+            build_ktable(D.A, kmax, alpha, V, kt);
+            structure_factor(kt, frac, q, kt.Sre, kt.Sim);
+        }
+        const int nkf = (int)kt.kpref.size();
+        auto coupled = [&](int i) {
+            for (int d = 0; d < 3; ++d)
+                if (kt.n[3 * i + d] % F.replication[d] != 0) return false;
+            return true;
+        };
+        // replication check on up to 32 uncoupled k-vectors (src/batch.jl:173-195)
+        if (!rep1) {
+            std::vector<int> unc;
+            for (int i = 0; i < nkf; ++i)
+                if (!coupled(i)) unc.push_back(i);
+            if (!unc.empty()) {
+                size_t stride = std::max<size_t>(1, (unc.size() + 31) / 32);
+                double maxS = 0.0, sabs = 0.0;
+                int arg = -1, taken = 0;
+                for (size_t t = 0; t < unc.size() && taken < 32; t += stride, ++taken) {
+                    double mS = std::hypot(kt.Sre[unc[t]], kt.Sim[unc[t]]);
+                    if (mS > maxS) { maxS = mS; arg = unc[t]; }
+                }
+                for (int j = 0; j < n; ++j) sabs += std::fabs(q[j]);
+                double thr = std::max(1.0e-8, 100.0 * 2.220446049250313e-16) * sabs;
+                if (maxS > thr)
+                    return fail(PA_ERR_ARG,
+                                "framework %d claims replication (%d, %d, %d), but its host structure factor at k=[%g, %g, "
+                                "%g] (not coupled to that replication) is %g, exceeding threshold %g = max(1e-8, "
+                                "100·eps(Float64))·Σ|q_host|; this replication does not describe the atoms",
+                                uniq[u] + 1, F.replication[0], F.replication[1], F.replication[2], kt.k[3 * arg],
+                                kt.k[3 * arg + 1], kt.k[3 * arg + 2], maxS, thr);
+            }
+        }
+        if (charged) {
+            for (int i = 0; i < nkf; ++i) {
+                if (!fullk && !coupled(i)) continue;
+                k64.push_back(make_double4(kt.k[3 * i], kt.k[3 * i + 1], kt.k[3 * i + 2], kt.kpref[i]));
+                S64.push_back(make_double2(kt.Sre[i], kt.Sim[i]));
+                kn.push_back(make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0));
+                kc.push_back(make_float2((float)(2.0 * kt.kpref[i] * kt.Sre[i]), (float)(2.0 * kt.kpref[i] * kt.Sim[i])));
+            }
+            // orientation-averaged guest self term over the FULL k table (src/batch.jl:360-389)
+            double samples[64];
+            for (int o = 0; o < 64; ++o) {
+                double gs[MAX_SITES][3];
+                for (int a = 0; a < NS; ++a) rotate(&self_q[4 * o], guest->sites + 3 * a, gs[a]);
+                double acc = 0.0;
+                for (int i = 0; i < nkf; ++i) {
+                    double re = 0.0, im = 0.0;
+                    for (int a = 0; a < NS; ++a) {
+                        double ph = kt.k[3 * i] * gs[a][0] + kt.k[3 * i + 1] * gs[a][1] + kt.k[3 * i + 2] * gs[a][2];
+                        re += guest->charges[a] * std::cos(ph);
+                        im += guest->charges[a] * std::sin(ph);
+                    }
+                    acc += kt.kpref[i] * (re * re + im * im);
+                }
+                samples[o] = KE * acc;
+            }
+            double sum = 0.0, mx = samples[0], mn = samples[0];
+            for (double x : samples) { sum += x; mx = std::max(mx, x); mn = std::min(mn, x); }
+            self_mean = sum / 64.0;
+            halfrange = (mx - mn) / 2.0;
+            const double KT_REF = KB * 300.0;
+            if (2.0 * halfrange / KT_REF > 1.0e-3)
+                return fail(PA_ERR_ARG,
+                            "guest self-term half-range estimate %g eV (guard factor 2) at framework %d: 2·%g exceeds "
+                            "0.001·KB·300K = %g eV; this guest/cell combination needs the per-insertion reciprocal sum",
+                            halfrange, uniq[u] + 1, halfrange, 1.0e-3 * KT_REF);
+        }
+        D.nk = (int)k64.size() - D.k_off;
+        // constant_offset (src/batch.jl:396-407; tail_delta src/forcefield.jl:103-124)
+        double tail = 0.0;
+        if (ff->tail) {
+            std::vector<double> counts(NT, 0.0), gcounts(NT, 0.0);
+            for (int j = 0; j < n; ++j) counts[F.types[j]] += 1.0;
+            for (int a = 0; a < NS; ++a) gcounts[guest->types[a]] += 1.0;
+            double acc = 0.0;
+            for (int x = 0; x < NT; ++x)
+                for (int y = 0; y < NT; ++y) {
+                    double e = ff->epsilon[x + NT * y];
+                    if (e == 0.0) continue;
+                    double sg = ff->sigma[x + NT * y], r = sg / rc_lj, r3 = r * r * r;
+                    double c = e * (sg * sg * sg) * (r3 * r3 * r3 / 3.0 - r3);
+                    acc += 2.0 * counts[x] * gcounts[y] * c + gcounts[x] * gcounts[y] * c;
+                }
+            tail = (8.0 * PI / 3.0) / V * acc;
+        }
+        double q2 = 0.0, Qh = 0.0, Qg = 0.0, excl = 0.0;
+        for (int a = 0; a < NS; ++a) { q2 += guest->charges[a] * guest->charges[a]; Qg += guest->charges[a]; }
+        for (int j = 0; j < n; ++j) Qh += q[j];
+        for (int a = 0; a < NS; ++a)
+            for (int c = a + 1; c < NS; ++c) {
+                const double *sa = guest->sites + 3 * a, *sc = guest->sites + 3 * c;
+                double d[3] = {sa[0] - sc[0], sa[1] - sc[1], sa[2] - sc[2]};
+                double r = norm3(d);
+                excl -= guest->charges[a] * guest->charges[c] * (1.0 - erfc_dev(alpha * r)) / r;
+            }
+        const double self = -alpha / std::sqrt(PI) * q2;
+        const double net = -PI / (2.0 * V * alpha * alpha) * ((Qh + Qg) * (Qh + Qg) - Qh * Qh);
+        D.const_off = tail + KE * (self + excl + net) + self_mean;
+        b->info[u] = FwHost{n, D.nk, nkf, V, alpha, D.const_off, halfrange};
+    }
+
+    // ---- guest and pair tables
+    GuestDev &G = b->gdev;
+    G.nsites = NS;
+    G.charged = charged ? 1 : 0;
+    for (int a = 0; a < NS; ++a) {
+        for (int d = 0; d < 3; ++d) G.site[a][d] = guest->sites[3 * a + d];
+        G.q[a] = guest->charges[a];
+        G.qke[a] = (float)(KE * guest->charges[a]);
+        G.type[a] = orig_to_c[guest->types[a]];
+    }
+    std::vector<double2> pair64((size_t)NS * NTC);
+    std::vector<float2> pair32((size_t)NS * NTC);
+    for (int a = 0; a < NS; ++a)
+        for (int t = 0; t < NTC; ++t) {
+            int go = guest->types[a], ho = c_to_orig[t];
+            double sg = ff->sigma[go + NT * ho], ep = ff->epsilon[go + NT * ho];
+            pair64[(size_t)a * NTC + t] = make_double2(sg, ep);
+            pair32[(size_t)a * NTC + t] = make_float2((float)(sg * sg), (float)(4.0 * ep));
+        }
+
+    if (flags & PA_BATCH_HOST_ONLY) {
+        b->host_only = true;
+        *out = b.release();
+        return ok();
+    }
+    // ---- device upload
+    int st = device_ready();
+    if (st) return st;
+    HIP_TRY(hipStreamCreate(&b->stream));
+    for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(upload(b->d_fw, b->desc));
+    HIP_TRY(upload(b->d_fw_of, b->fw_of));
+    HIP_TRY(upload(b->d_atom64, atom64));
+    HIP_TRY(upload(b->d_atype, atype));
+    HIP_TRY(upload(b->d_atomfx, atomfx));
+    HIP_TRY(upload(b->d_k64, k64));
+    HIP_TRY(upload(b->d_S64, S64));
+    HIP_TRY(upload(b->d_kn, kn));
+    HIP_TRY(upload(b->d_kc, kc));
+    HIP_TRY(upload(b->d_pair64, pair64));
+    HIP_TRY(upload(b->d_pair32, pair32));
+    *out = b.release();
+    return ok();
+}
+
+extern "C" void pa_batch_destroy(pa_batch *b) { delete b; }
+
+extern "C" int pa_batch_nframeworks(const pa_batch *b, int32_t *nfw, int32_t *nsys)
+{
+    if (!b) return fail(PA_ERR_ARG, "NULL batch");
+    if (nfw) *nfw = b->nfw;
+    if (nsys) *nsys = b->nsys;
+    return ok();
+}
+
+extern "C" int pa_batch_framework_of(const pa_batch *b, int32_t *framework_of)
+{
+    if (!b || !framework_of) return fail(PA_ERR_ARG, "NULL argument");
+    for (int s = 0; s < b->nsys; ++s) framework_of[s] = b->fw_of[s];
+    return ok();
+}
+
+extern "C" int pa_batch_framework_info(const pa_batch *b, int32_t fw, pa_framework_info *out)
+{
+    if (!b || !out) return fail(PA_ERR_ARG, "NULL argument");
+    if (fw < 0 || fw >= b->nfw) return fail(PA_ERR_ARG, "framework index %d out of range 0..%d", fw, b->nfw - 1);
+    const FwHost &h = b->info[fw];
+    out->natoms = h.natoms;
+    out->nk = h.nk;
+    out->nk_full = h.nk_full;
+    out->volume = h.volume;
+    out->alpha = h.alpha;
+    out->constant_offset = h.const_off;
+    out->self_term_halfrange = h.halfrange;
+    return ok();
+}
+
+// ------------------------------------------------------------------------------------------
+// Widom
+// ------------------------------------------------------------------------------------------
+static WidomArgs base_args(pa_batch *b)
+{
+    WidomArgs a{};
+    a.fw = b->d_fw.as<FwDesc>();
+    a.fw_of = b->d_fw_of.as<int>();
+    a.nsys = b->nsys;
+    a.atom64 = b->d_atom64.as<double4>();
+    a.atype = b->d_atype.as<int>();
+    a.k64 = b->d_k64.as<double4>();
+    a.S64 = b->d_S64.as<double2>();
+    a.atomfx = b->d_atomfx.as<uint4>();
+    a.kn = b->d_kn.as<int4>();
+    a.kc = b->d_kc.as<float2>();
+    a.pair64 = b->d_pair64.as<double2>();
+    a.pair32 = b->d_pair32.as<float2>();
+    a.ntypes_c = b->ntypes_c;
+    a.rc_lj2 = b->rc_lj * b->rc_lj;
+    a.rc_ew2 = b->rc_ew * b->rc_ew;
+    a.rc_lj2f = (float)a.rc_lj2;
+    a.rc_ew2f = b->gdev.charged ? (float)a.rc_ew2 : 0.0f;
+    a.rc_max2f = std::max(a.rc_lj2f, a.rc_ew2f);
+    a.guest = b->gdev;
+    return a;
+}
+
+static int check_precision(const pa_batch *b, uint32_t precision)
+{
+    if (b->host_only) return fail(PA_ERR_ARG, "batch was built with PA_BATCH_HOST_ONLY and has no device tables");
+    if (precision != PA_PREC_F64 && precision != PA_PREC_MIXED)
+        return fail(PA_ERR_ARG, "precision=%u: use PA_PREC_F64 (0) or PA_PREC_MIXED (1)", precision);
+    if (b->nsites > 6) return fail(PA_ERR_ARG, "guests with %d sites: the device kernels take 1..6 sites", b->nsites);
+    return PA_OK;
+}
+
+// Per-system sample counts under the run-length assignment (src/widom.jl:44-53).
+static void system_counts(int64_t ninsert, int64_t nsys, int64_t run, std::vector<int64_t> &ns)
+{
+    int64_t q = ninsert / run, rem = ninsert % run, full = q / nsys, hi = q % nsys;
+    ns.assign(nsys, full * run);
+    for (int64_t r = 0; r < hi; ++r) ns[r] += run;
+    if (rem > 0) ns[q % nsys] += rem;
+}
+
+struct LseH {
+    double m, w, u;
+};
+
+static LseH lse_merge_h(LseH x, LseH y)
+{
+    if (y.m == -INFINITY) return x;
+    if (x.m == -INFINITY) return y;
+    double m = std::max(x.m, y.m), ex = std::exp(x.m - m), ey = std::exp(y.m - m);
+    return LseH{m, x.w * ex + y.w * ey, x.u * ex + y.u * ey};
+}
+
+extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
+                        int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out)
+{
+    if (!b || !guest || !out) return fail(PA_ERR_ARG, "NULL argument");
+    // argument checks in the reference's order and wording (src/widom.jl:302-330)
+    if (nblocks < 2) return fail(PA_ERR_ARG, "nblocks=%d: at least two blocks are needed for a standard error", nblocks);
+    if (chunk < 0) return fail(PA_ERR_ARG, "chunk=%lld must be ≥ 1", (long long)chunk);
+    bool same = guest->nsites == b->nsites;
+    for (int a = 0; same && a < b->nsites; ++a) {
+        same = guest->types[a] == b->gtypes[a] && guest->charges[a] == b->gq[a];
+        for (int d = 0; d < 3; ++d) same = same && guest->sites[3 * a + d] == b->gsites[3 * a + d];
+    }
+    if (!same)
+        return fail(PA_ERR_ARG, "guest passed to widom does not match the guest FrameworkBatch was built with; build a new "
+                                "FrameworkBatch for a different guest");
+    if (!(T > 0)) return fail(PA_ERR_ARG, "T=%g must be a positive temperature in K", T);
+    if (ninsert < 1) return fail(PA_ERR_ARG, "ninsert=%lld must be ≥ 1", (long long)ninsert);
+    const int64_t nsys = b->nsys, per = ninsert / nsys;
+    int64_t runlen;
+    if (run == 0) {
+        runlen = std::min<int64_t>(256, std::max<int64_t>(1, (ninsert / nsys) / 4));
+    } else {
+        if (run < 1) return fail(PA_ERR_ARG, "run=%d must be ≥ 1", run);
+        if (run > per) return fail(PA_ERR_ARG, "run=%d must be ≤ per=%lld (= ninsert ÷ nsys)", run, (long long)per);
+        runlen = run;
+    }
+    std::vector<int64_t> ns;
+    system_counts(ninsert, nsys, runlen, ns);
+    int64_t nmin = ns[0], amin = 0;
+    for (int64_t s = 0; s < nsys; ++s)
+        if (ns[s] < nmin) { nmin = ns[s]; amin = s; }
+    if (nmin < 2 * (int64_t)nblocks)
+        return fail(PA_ERR_ARG,
+                    "ninsert=%lld, nsys=%lld, run=%lld: system %lld gets only %lld samples, need at least 2·nblocks = %d per "
+                    "system",
+                    (long long)ninsert, (long long)nsys, (long long)runlen, (long long)(amin + 1), (long long)nmin,
+                    2 * nblocks);
+    int st = check_precision(b, precision);
+    if (st) return st;
+    const double kT = KB * T;
+
+    std::vector<int64_t> seg(nsys + 1, 0);
+    for (int64_t s = 0; s < nsys; ++s) seg[s + 1] = seg[s] + ns[s];
+    const int64_t G = seg[nsys];
+    HIP_TRY(upload(b->d_seg, seg));
+    HIP_TRY(b->d_dU.ensure(sizeof(double) * G));
+
+    // tiles: each block of each system split into runs of at most TILE insertions
+    const int64_t TILE = 8192;
+    std::vector<int64_t> tile_g0;
+    std::vector<int> tile_n;
+    std::vector<int> tile_owner;  // s * nblocks + blk
+    for (int64_t s = 0; s < nsys; ++s) {
+        const int64_t blen = ns[s] / nblocks;
+        for (int blk = 0; blk < nblocks; ++blk) {
+            const int64_t j0 = blk * blen, j1 = (blk == nblocks - 1) ? ns[s] : (blk + 1) * blen;
+            for (int64_t t = j0; t < j1; t += TILE) {
+                tile_g0.push_back(seg[s] + t);
+                tile_n.push_back((int)std::min(TILE, j1 - t));
+                tile_owner.push_back((int)(s * nblocks + blk));
+            }
+        }
+    }
+    const int ntiles = (int)tile_g0.size();
+    HIP_TRY(upload(b->d_tile_g0, tile_g0));
+    HIP_TRY(upload(b->d_tile_n, tile_n));
+    HIP_TRY(b->d_tiles.ensure(sizeof(double4) * ntiles));
+
+    WidomArgs a = base_args(b);
+    a.seg = b->d_seg.as<int64_t>();
+    a.seed = seed;
+    a.dU = b->d_dU.as<double>();
+    a.out_base = 0;
+    const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    int64_t launches = 0;
+    for (int64_t g0 = 0; g0 < G; g0 += CH) {
+        a.g0 = g0;
+        a.g1 = std::min(G, g0 + CH);
+        HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
+        ++launches;
+    }
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(launch_tiles(b->d_dU.as<double>(), b->d_tile_g0.as<int64_t>(), b->d_tile_n.as<int>(), ntiles, kT,
+                         b->d_tiles.as<double4>(), b->stream));
+    HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    std::vector<double4> tiles(ntiles);
+    HIP_TRY(hipMemcpyAsync(tiles.data(), b->d_tiles.p, sizeof(double4) * ntiles, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float e01 = 0, e12 = 0, e02 = 0;
+    HIP_TRY(hipEventElapsedTime(&e01, b->ev[0], b->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&e12, b->ev[1], b->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&e02, b->ev[0], b->ev[2]));
+    b->last = pa_timing{e02, e01, launches, e12, G};
+
+    // combine tiles per (system, block) in order, then the reference's block statistics
+    std::vector<LseH> blk((size_t)nsys * nblocks, LseH{-INFINITY, 0.0, 0.0});
+    std::vector<double> nanc((size_t)nsys * nblocks, 0.0);
+    for (int t = 0; t < ntiles; ++t) {
+        const double4 &x = tiles[t];
+        blk[tile_owner[t]] = lse_merge_h(blk[tile_owner[t]], LseH{x.x, x.y, x.z});
+        nanc[tile_owner[t]] += x.w;
+    }
+    for (int64_t s = 0; s < nsys; ++s) {
+        const FwHost &h = b->info[b->fw_of[s]];
+        const int64_t blen = ns[s] / nblocks;
+        double M = -INFINITY, nans = 0.0;
+        for (int k = 0; k < nblocks; ++k) {
+            M = std::max(M, blk[s * nblocks + k].m);
+            nans += nanc[s * nblocks + k];
+        }
+        if (nans > 0 || M == INFINITY)
+            return fail(PA_ERR_ARG,
+                        "system %lld: Boltzmann accumulation overflowed Float64 (W=NaN, UW=NaN); an insertion energy "
+                        "below about -709·kT was sampled",
+                        (long long)(s + 1));
+        if (M == -INFINITY) M = 0.0;  // every weight is exactly zero
+        // block sums scaled by exp(-M)
+        std::vector<double> sW(nblocks), sUW(nblocks), nb(nblocks);
+        double tW = 0.0, tUW = 0.0, ntot = 0.0;
+        for (int k = 0; k < nblocks; ++k) {
+            const LseH &x = blk[s * nblocks + k];
+            const double sc = x.m == -INFINITY ? 0.0 : std::exp(x.m - M);
+            sW[k] = x.w * sc;
+            sUW[k] = x.u * sc;
+            nb[k] = (double)((k == nblocks - 1) ? ns[s] - k * blen : blen);
+            tW += sW[k];
+            tUW += sUW[k];
+            ntot += nb[k];
+        }
+        // _reduce (src/widom.jl:451-492) on the scaled sums: W = Ws * e^M
+        const double Ws = tW / ntot, UWs = tUW / ntot;
+        double varW = 0.0, varUW = 0.0, cov = 0.0;
+        for (int k = 0; k < nblocks; ++k) {
+            const double mW = sW[k] / nb[k], mUW = sUW[k] / nb[k];
+            varW += (mW - Ws) * (mW - Ws);
+            varUW += (mUW - UWs) * (mUW - UWs);
+            cov += (mW - Ws) * (mUW - UWs);
+        }
+        varW /= (nblocks - 1);
+        varUW /= (nblocks - 1);
+        cov /= (nblocks - 1);
+        const double semWs = std::sqrt(varW / nblocks);
+        const double ratio = UWs / Ws;
+        const double var_ratio =
+            UWs == 0.0 ? 0.0 : ratio * ratio * (varUW / (UWs * UWs) + varW / (Ws * Ws) - 2.0 * cov / (UWs * Ws)) / nblocks;
+        const double eM = std::exp(M);
+        const double W = Ws * eM, UW = UWs * eM;
+        if (!std::isfinite(W) || !std::isfinite(UW))
+            return fail(PA_ERR_ARG,
+                        "system %lld: Boltzmann accumulation overflowed Float64 (W=%g, UW=%g); an insertion energy below "
+                        "about -709·kT was sampled",
+                        (long long)(s + 1), W, UW);
+        const double lnW = std::log(Ws) + M;
+        const double mu_ex = -kT * lnW, q_st = kT - ratio;
+        if (!std::isfinite(mu_ex)) return fail(PA_ERR_ARG, "system %lld: mu_ex = %g is not finite", (long long)(s + 1), mu_ex);
+        if (!std::isfinite(q_st)) return fail(PA_ERR_ARG, "system %lld: q_st = %g is not finite", (long long)(s + 1), q_st);
+        pa_widom_result &r = out[s];
+        r.mu_ex = mu_ex;
+        r.mu_ex_err = kT * semWs / Ws;
+        r.K_H = h.volume * W / kT;
+        r.K_H_err = h.volume * semWs * eM / kT;
+        r.q_st = q_st;
+        r.q_st_err = std::sqrt(std::max(var_ratio, 0.0));
+        r.nsamples = (int64_t)ntot;
+        r.nblocks = nblocks;
+        r.ln_K_H = std::log(h.volume) + lnW - std::log(kT);
+    }
+    return ok();
+}
+
+extern "C" int pa_widom_energies(pa_batch *b, int32_t sys, uint64_t seed, int64_t j0, int64_t n, uint32_t precision,
+                                 double *dU)
+{
+    if (!b || !dU) return fail(PA_ERR_ARG, "NULL argument");
+    int st = check_precision(b, precision);
+    if (st) return st;
+    if (sys < 0 || sys >= b->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, b->nsys - 1);
+    if (j0 < 0 || n < 0) return fail(PA_ERR_ARG, "insertion range j0=%lld, n=%lld is invalid", (long long)j0, (long long)n);
+    if (n == 0) return ok();
+    // system `sys` owns g in [0, j0 + n); the launch covers g in [j0, j0 + n)
+    std::vector<int64_t> seg(b->nsys + 1, 0);
+    for (int s = sys + 1; s <= b->nsys; ++s) seg[s] = j0 + n;
+    HIP_TRY(upload(b->d_seg, seg));
+    HIP_TRY(b->d_dU.ensure(sizeof(double) * n));
+    WidomArgs a = base_args(b);
+    a.seg = b->d_seg.as<int64_t>();
+    a.seed = seed;
+    a.dU = b->d_dU.as<double>();
+    a.out_base = j0;
+    a.g0 = j0;
+    a.g1 = j0 + n;
+    HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
+    HIP_TRY(hipMemcpyAsync(dU, b->d_dU.p, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return ok();
+}
+
+extern "C" int pa_widom_poses(pa_batch *b, int32_t sys, uint64_t seed, int64_t j0, int64_t n, double *pos, double *quat)
+{
+    if (!b || !pos || !quat) return fail(PA_ERR_ARG, "NULL argument");
+    if (b->host_only) return fail(PA_ERR_ARG, "batch was built with PA_BATCH_HOST_ONLY and has no device tables");
+    if (sys < 0 || sys >= b->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, b->nsys - 1);
+    if (j0 < 0 || n < 0) return fail(PA_ERR_ARG, "insertion range j0=%lld, n=%lld is invalid", (long long)j0, (long long)n);
+    if (n == 0) return ok();
+    HIP_TRY(b->d_pos.ensure(sizeof(double) * 3 * n));
+    HIP_TRY(b->d_quat.ensure(sizeof(double) * 4 * n));
+    HIP_TRY(launch_poses(b->d_fw.as<FwDesc>(), b->d_fw_of.as<int>(), sys, seed, j0, n, b->d_pos.as<double>(),
+                         b->d_quat.as<double>(), b->stream));
+    HIP_TRY(hipMemcpyAsync(pos, b->d_pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(quat, b->d_quat.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return ok();
+}
+
+extern "C" int pa_last_timing(const pa_batch *b, pa_timing *out)
+{
+    if (!b || !out) return fail(PA_ERR_ARG, "NULL argument");
+    *out = b->last;
+    return ok();
+}

@@ -1,0 +1,22 @@
+"""MI355X (gfx950) backend of PureAdsorb.jl's Widom insertion path.
+
+The public names mirror PureAdsorb.jl's API (src/PureAdsorb.jl:27-37): `Framework`,
+`read_cif`, `replicate`, `ForceField`, `read_forcefield`, `Guest`, `read_guest`,
+`EwaldParams`, `FrameworkBatch`, `widom`, `WidomResult`. The compute runs in
+libpureadsorb_amd.so (HIP kernels for gfx950) through its C ABI; there is no CPU fallback.
+"""
+from ._lib import ArgumentError, DeviceError, DimensionMismatch, device_name
+from .batch import FrameworkBatch, nframeworks
+from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
+from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
+                        reciprocal_basis, replicate, total_charge, volume)
+from .widom import (KB, KE, PASCAL, WidomResult, default_run, last_timing, sys_of_index, system_counts, widom,
+                    widom_energies, widom_poses)
+
+__all__ = [
+    "ArgumentError", "DeviceError", "DimensionMismatch", "device_name", "FrameworkBatch", "nframeworks", "EwaldParams",
+    "ForceField", "Guest", "read_forcefield", "read_guest", "tail_coefficient", "tail_delta", "typeindex", "Framework",
+    "cartesian", "cell_matrix", "min_multiplicity", "natoms", "perpendicular_lengths", "read_cif", "reciprocal_basis",
+    "replicate", "total_charge", "volume", "KB", "KE", "PASCAL", "WidomResult", "default_run", "last_timing",
+    "sys_of_index", "system_counts", "widom", "widom_energies", "widom_poses",
+]

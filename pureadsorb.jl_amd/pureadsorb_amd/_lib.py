@@ -1,0 +1,170 @@
+"""ctypes binding of libpureadsorb_amd.so (the C ABI in include/pureadsorb_amd.h).
+
+The library is loaded from this directory. A missing library is an error: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpureadsorb_amd.so")
+
+PA_OK = 0
+PA_ERR_ARG = 1
+PA_ERR_DIM = 2
+PA_ERR_DEVICE = 3
+PA_ERR_INTERNAL = 4
+
+PA_PREC_F64 = 0
+PA_PREC_MIXED = 1
+
+PA_BATCH_FULLK = 1
+PA_BATCH_HOST_ONLY = 2
+
+# Every symbol include/pureadsorb_amd.h declares.
+EXPORTS = (
+    "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
+    "pa_batch_framework_info", "pa_widom", "pa_widom_energies", "pa_widom_poses", "pa_last_timing",
+    "pa_last_error", "pa_device_name",
+)
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError (a bad argument, caught before any launch)."""
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure."""
+
+
+class PaFramework(C.Structure):
+    _fields_ = [
+        ("cell", C.c_double * 9),
+        ("natoms", C.c_int32),
+        ("frac", C.c_void_p),
+        ("types", C.c_void_p),
+        ("charges", C.c_void_p),
+        ("replication", C.c_int32 * 3),
+    ]
+
+
+class PaForcefield(C.Structure):
+    _fields_ = [
+        ("ntypes", C.c_int32),
+        ("sigma", C.c_void_p),
+        ("epsilon", C.c_void_p),
+        ("cutoff", C.c_double),
+        ("tail", C.c_int32),
+    ]
+
+
+class PaGuest(C.Structure):
+    _fields_ = [
+        ("nsites", C.c_int32),
+        ("sites", C.c_void_p),
+        ("types", C.c_void_p),
+        ("charges", C.c_void_p),
+    ]
+
+
+class PaEwald(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("precision", C.c_double)]
+
+
+class PaWidomResult(C.Structure):
+    _fields_ = [
+        ("mu_ex", C.c_double), ("mu_ex_err", C.c_double),
+        ("K_H", C.c_double), ("K_H_err", C.c_double),
+        ("q_st", C.c_double), ("q_st_err", C.c_double),
+        ("nsamples", C.c_int64), ("nblocks", C.c_int64),
+        ("ln_K_H", C.c_double),
+    ]
+
+
+class PaFrameworkInfo(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int32), ("nk", C.c_int32), ("nk_full", C.c_int32),
+        ("volume", C.c_double), ("alpha", C.c_double),
+        ("constant_offset", C.c_double), ("self_term_halfrange", C.c_double),
+    ]
+
+
+class PaTiming(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
+        ("reduce_ms", C.c_double), ("insertions", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python pureadsorb.jl_amd/build.py` "
+            "(there is no non-HIP fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.pa_batch_create.restype = C.c_int
+    L.pa_batch_create.argtypes = [C.POINTER(PaFramework), C.c_int32, C.POINTER(PaForcefield), C.POINTER(PaGuest),
+                                  C.POINTER(PaEwald), C.c_uint32, C.POINTER(vp)]
+    L.pa_batch_destroy.restype = None
+    L.pa_batch_destroy.argtypes = [vp]
+    L.pa_batch_nframeworks.restype = C.c_int
+    L.pa_batch_nframeworks.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.pa_batch_framework_of.restype = C.c_int
+    L.pa_batch_framework_of.argtypes = [vp, vp]
+    L.pa_batch_framework_info.restype = C.c_int
+    L.pa_batch_framework_info.argtypes = [vp, C.c_int32, C.POINTER(PaFrameworkInfo)]
+    L.pa_widom.restype = C.c_int
+    L.pa_widom.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_int64,
+                           C.c_uint32, vp]
+    L.pa_widom_energies.restype = C.c_int
+    L.pa_widom_energies.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int64, C.c_int64, C.c_uint32, vp]
+    L.pa_widom_poses.restype = C.c_int
+    L.pa_widom_poses.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int64, C.c_int64, vp, vp]
+    L.pa_last_timing.restype = C.c_int
+    L.pa_last_timing.argtypes = [vp, C.POINTER(PaTiming)]
+    L.pa_last_error.restype = C.c_int
+    L.pa_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    L.pa_device_name.restype = C.c_int
+    L.pa_device_name.argtypes = [C.c_char_p, C.c_size_t]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(4096)
+    lib().pa_last_error(buf, len(buf))
+    return buf.value.decode("utf-8", "replace")
+
+
+def check(status: int) -> None:
+    """Raises the exception class matching a C-ABI status."""
+    if status == PA_OK:
+        return
+    msg = last_error()
+    if status == PA_ERR_ARG:
+        raise ArgumentError(msg)
+    if status == PA_ERR_DIM:
+        raise DimensionMismatch(msg)
+    if status == PA_ERR_DEVICE:
+        raise DeviceError(msg)
+    raise RuntimeError(f"status {status}: {msg}")
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().pa_device_name(buf, len(buf)))
+    return buf.value.decode()

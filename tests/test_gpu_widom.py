@@ -1,0 +1,301 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the FP64 oracle on the same
+insertion streams.
+
+Bars: poses bit-exact; FP64 plain-path energies within 1e-11 relative (IEEE-exact
+arithmetic except the device's libm exp/sincos, 1-2 ulp apart from the host's); mixed-path
+Widom results within |d ln K_H| <= 1e-4 and |d q_st| <= 0.01 kJ/mol = 1.036e-4 eV of the
+oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pureadsorb_amd as pa
+from conftest import oracle_for, oracle_threads, rel_close
+from pureadsorb_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+QST_TOL = 0.01 / 96.48533212  # 0.01 kJ/mol in eV
+LNKH_TOL = 1.0e-4
+
+
+@pytest.fixture(scope="module")
+def c1(config1):
+    fw, ff, g, ew = config1
+    print(f"config 1: {fw.natoms} atoms", flush=True)
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    return fw, ff, g, ew, b, oracle_for(fw, ff, g, ew)
+
+
+@pytest.fixture(scope="module")
+def c0(config0):
+    fw, ff, g, ew = config0
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    return fw, ff, g, ew, b, oracle_for(fw, ff, g, ew)
+
+
+def test_device_is_gfx950():
+    name = pa.device_name()
+    print(name, flush=True)
+    assert "gfx950" in name
+
+
+def test_poses_bit_exact(c1):
+    fw, ff, g, ew, b, o = c1
+    pos, quat = pa.widom_poses(b, 0, 42, 0, 4096)
+    _, poses = o.energies(42, 0, 0, 64, want_poses=True)
+    assert np.array_equal(pos[:64], poses[:, :3])
+    assert np.array_equal(quat[:64], poses[:, 3:])
+    # the rest through the scalar oracle pose function
+    for j in (100, 1000, 4095):
+        p, q = O.pose(42, 0, j, fw.cell)
+        assert np.array_equal(pos[j], p) and np.array_equal(quat[j], q)
+
+
+def test_f64_energies_match_oracle(c1):
+    fw, ff, g, ew, b, o = c1
+    n = 512
+    dU = pa.widom_energies(b, 0, 42, 0, n, precision="f64")
+    ref = o.energies(42, 0, 0, n, nthreads=oracle_threads())
+    ok, mask = rel_close(dU, ref, 1e-11)
+    assert ok, f"{np.count_nonzero(~mask)} of {n} differ; worst rel {np.max(np.abs(dU - ref) / np.maximum(np.abs(ref), 1))}"
+
+
+def test_f64_energies_offset_range(c1):
+    """A range starting mid-stream gives the same energies as the same insertions of a longer range."""
+    fw, ff, g, ew, b, o = c1
+    a = pa.widom_energies(b, 0, 7, 0, 300, precision="f64")
+    c = pa.widom_energies(b, 0, 7, 100, 200, precision="f64")
+    assert np.array_equal(a[100:], c)
+
+
+def test_mixed_energies_close_to_oracle(c1):
+    """Config 1 as specified leaves no accessible insertion (every Delta U is tens to millions of
+    eV), so the mixed path is checked relative to the oracle here."""
+    fw, ff, g, ew, b, o = c1
+    n = 2048
+    dU = pa.widom_energies(b, 0, 42, 0, n, precision="mixed")
+    ref = o.energies(42, 0, 0, n, nthreads=oracle_threads())
+    assert np.all(np.isfinite(dU))
+    rel = np.abs(dU - ref) / np.maximum(np.abs(ref), 1.0)
+    print(f"config 1 mixed vs oracle: max rel {rel.max():.2e}, median {np.median(rel):.2e}", flush=True)
+    assert rel.max() < 1e-5
+
+
+@pytest.fixture(scope="module")
+def c1p(config1):
+    """Config 1's cell, replication, guest and Ewald settings at a porous density: 80 atoms per
+    hexagonal cell (2,160 atoms), where about 1% of CO2 insertions bind."""
+    _, ff, g, ew = config1
+    fw = pa.replicate(synth.synthetic_framework(13.0, 13.0, 15.0, 90, 90, 120, 80, 42, charge_sd=0.5), (3, 3, 3))
+    return fw, ff, g, ew, pa.FrameworkBatch([fw], ff, g, ew), oracle_for(fw, ff, g, ew)
+
+
+def test_mixed_energies_weighted_poses(c1p):
+    """On binding poses (Delta U < 10 kT) the FP32 pair and k-space sums stay within 3e-6 eV."""
+    fw, ff, g, ew, b, o = c1p
+    n = 8192
+    dU = pa.widom_energies(b, 0, 42, 0, n, precision="mixed")
+    ref = o.energies(42, 0, 0, n, nthreads=oracle_threads())
+    kT = pa.KB * 298.15
+    w = ref < 10 * kT
+    err = np.abs(dU[w] - ref[w])
+    print(f"porous config 1 mixed on {w.sum()} binding poses: max {err.max():.2e} eV, rms "
+          f"{np.sqrt(np.mean(err**2)):.2e} eV", flush=True)
+    assert w.sum() > 20
+    assert err.max() < 3e-6
+
+
+def _results_close(r, ref, rtol):
+    for k in ("mu_ex", "K_H", "q_st", "mu_ex_err", "K_H_err", "q_st_err"):
+        a, b = getattr(r, k), ref[k]
+        assert abs(a - b) <= rtol * max(abs(b), 1e-300), (k, a, b)
+
+
+def _oracle_result(o, T, ns, seed, sys, nblocks):
+    st, out, _ = o.widom(T, ns, seed, sys, nblocks, nthreads=oracle_threads())
+    assert st == 0
+    keys = ("mu_ex", "mu_ex_err", "K_H", "K_H_err", "q_st", "q_st_err", "nsamples", "nblocks")
+    return dict(zip(keys, out))
+
+
+def test_widom_f64_matches_oracle(c1p):
+    fw, ff, g, ew, b, o = c1p
+    r = pa.widom(b, g, T=298.15, ninsert=4000, seed=42, nblocks=10, precision="f64")[0]
+    ref = _oracle_result(o, 298.15, 4000, 42, 0, 10)
+    assert r.nsamples == 4000 and r.nblocks == 10
+    _results_close(r, ref, 1e-9)
+
+
+def _ln_kh_oracle(o, T, ns, seed, sys):
+    """ln K_H from the oracle's per-insertion energies by log-sum-exp (finite even when every
+    Boltzmann weight underflows Float64)."""
+    from scipy.special import logsumexp
+    dU = o.energies(seed, sys, 0, ns, nthreads=oracle_threads())
+    kT = pa.KB * T
+    return np.log(o.V) - np.log(kT) + logsumexp(-dU / kT) - np.log(ns)
+
+
+def test_widom_config1_literal_lse(c1):
+    """Config 1 as specified: the weights all underflow, the log-sum-exp accumulation keeps
+    ln K_H finite. FP64 plain path within 1e-12 relative of the oracle; mixed within 1e-6
+    relative (FP32 on 1e2-1e6 eV repulsive energies)."""
+    fw, ff, g, ew, b, o = c1
+    ns = 20000
+    ref = _ln_kh_oracle(o, 298.15, ns, 42, 0)
+    r64 = pa.widom(b, g, T=298.15, ninsert=ns, seed=42, precision="f64")[0]
+    rmx = pa.widom(b, g, T=298.15, ninsert=ns, seed=42, precision="mixed")[0]
+    print(f"config 1 ln K_H: oracle {ref:.10f}, f64 {r64.ln_K_H:.10f}, mixed {rmx.ln_K_H:.10f}", flush=True)
+    assert r64.K_H == 0.0 and np.isfinite(r64.mu_ex)
+    assert abs(r64.ln_K_H - ref) <= 1e-12 * abs(ref)
+    assert abs(rmx.ln_K_H - ref) <= 1e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("case", ["porous_config1", "rubtak"])
+def test_widom_mixed_parity(case, c1p, rubtak, trappe, co2):
+    """The north-star gate: mixed vs the FP64 oracle on identical streams,
+    |d ln K_H| <= 1e-4 and |d q_st| <= 0.01 kJ/mol."""
+    if case == "porous_config1":
+        fw, ff, g, ew, b, o = c1p
+        seed, ns = 42, 20000
+    else:
+        ff, g, ew = trappe, co2, pa.EwaldParams(12.0, 1e-6)
+        fw = pa.replicate(rubtak, (3, 3, 3))
+        b, o = pa.FrameworkBatch([fw], ff, g, ew), oracle_for(fw, ff, g, ew)
+        seed, ns = 3, 20000
+    r = pa.widom(b, g, T=298.15, ninsert=ns, seed=seed, nblocks=10, precision="mixed")[0]
+    r64 = pa.widom(b, g, T=298.15, ninsert=ns, seed=seed, nblocks=10, precision="f64")[0]
+    ref = _oracle_result(o, 298.15, ns, seed, 0, 10)
+    _results_close(r64, ref, 1e-9)
+    dlnkh = abs(np.log(r.K_H) - np.log(ref["K_H"]))
+    dqst = abs(r.q_st - ref["q_st"])
+    print(f"{case} mixed: K_H {r.K_H:.6e} vs {ref['K_H']:.6e} (d ln {dlnkh:.2e}); q_st {r.q_st:.6f} vs "
+          f"{ref['q_st']:.6f} eV (d {dqst:.2e})", flush=True)
+    assert dlnkh <= LNKH_TOL
+    assert dqst <= QST_TOL
+
+
+def test_widom_config0_ch4_f64_and_mixed(c0):
+    fw, ff, g, ew, b, o = c0
+    ns = 20000
+    ref = _oracle_result(o, 298.15, ns, 42, 0, 10)
+    r64 = pa.widom(b, g, T=298.15, ninsert=ns, seed=42, precision="f64")[0]
+    _results_close(r64, ref, 1e-9)
+    rm = pa.widom(b, g, T=298.15, ninsert=ns, seed=42, precision="mixed")[0]
+    assert abs(np.log(rm.K_H) - np.log(ref["K_H"])) <= LNKH_TOL
+    assert abs(rm.q_st - ref["q_st"]) <= QST_TOL
+
+
+def test_chunk_and_run_invariance(c1):
+    """Results depend only on the per-system sample streams: any chunk gives identical bits."""
+    fw, ff, g, ew, b, o = c1
+    ref = pa.widom(b, g, T=298.15, ninsert=3001, seed=9, nblocks=3, chunk=64)[0]
+    for chunk in (100, 1000, 4096, None):
+        r = pa.widom(b, g, T=298.15, ninsert=3001, seed=9, nblocks=3, chunk=chunk)[0]
+        assert r == ref
+
+
+def test_deterministic_repeat(c1):
+    fw, ff, g, ew, b, o = c1
+    r1 = pa.widom(b, g, T=298.15, ninsert=5000, seed=5)
+    r2 = pa.widom(b, g, T=298.15, ninsert=5000, seed=5)
+    assert r1 == r2
+
+
+def test_rubtak_triclinic_energies(rubtak, trappe, co2):
+    sc = pa.replicate(rubtak, (3, 3, 3))
+    ew = pa.EwaldParams(12.0, 1e-6)
+    b = pa.FrameworkBatch([sc], trappe, co2, ew)
+    o = oracle_for(sc, trappe, co2, ew)
+    assert b.nk[0] == 190
+    ref = o.energies(3, 0, 0, 256, nthreads=oracle_threads())
+    d64 = pa.widom_energies(b, 0, 3, 0, 256, precision="f64")
+    assert rel_close(d64, ref, 1e-11)[0]
+    dm = pa.widom_energies(b, 0, 3, 0, 256, precision="mixed")
+    kT = pa.KB * 298.15
+    w = ref < 30 * kT
+    assert w.sum() > 10
+    assert np.max(np.abs(dm[w] - ref[w])) < 2e-4
+
+
+def test_general_cell_orientation(c1p):
+    """A cell that is not upper triangular (a rotated copy) runs the general minimum-image
+    variant; energies are rotation invariant, so both agree with the oracle."""
+    fw, ff, g, ew = c1p[:4]
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    fwr = pa.Framework(R @ fw.cell, fw.frac, fw.labels, fw.symbols, fw.charges, fw.replication)
+    b = pa.FrameworkBatch([fwr], ff, g, ew)
+    o = oracle_for(fwr, ff, g, ew)
+    ref = o.energies(11, 0, 0, 256, nthreads=oracle_threads())
+    assert rel_close(pa.widom_energies(b, 0, 11, 0, 256, precision="f64"), ref, 1e-11)[0]
+    dm = pa.widom_energies(b, 0, 11, 0, 256, precision="mixed")
+    w = ref < 30 * pa.KB * 298.15
+    assert w.sum() > 5
+    assert np.max(np.abs(dm[w] - ref[w])) < 2e-4
+
+
+def test_multi_system_batch(c1p):
+    """Three systems over two distinct frameworks (dedup), uneven sizes: each system's result
+    equals the oracle on its own stream."""
+    fw1, ff, g, ew = c1p[:4]
+    fw_small = pa.replicate(synth.synthetic_framework(13.0, 13.0, 15.0, 90, 90, 120, 60, 7, charge_sd=0.5), (3, 3, 2))
+    b = pa.FrameworkBatch([fw1, fw_small, fw1], ff, g, ew)
+    assert b.nframeworks == 2 and list(b.framework_of) == [0, 1, 0]
+    rs = pa.widom(b, g, T=298.15, ninsert=6000, seed=13, nblocks=4, precision="f64")
+    ns = pa.system_counts(6000, 3, pa.default_run(6000, 3))
+    for s, fw in enumerate([fw1, fw_small, fw1]):
+        o = oracle_for(fw, ff, g, ew)
+        ref = _oracle_result(o, 298.15, int(ns[s]), 13, s, 4)
+        assert rs[s].nsamples == ns[s]
+        _results_close(rs[s], ref, 1e-9)
+
+
+def test_empty_box_ideal_gas():
+    """Empty box: mu_ex = 0, K_H = V/kT, q_st = kT exactly (test/widom_tests.jl:43-71)."""
+    ff = pa.ForceField.from_types(["X_"], [3.0], [0.001], cutoff=12.0, tail=False)
+    g = pa.Guest([[0.0, 0.0, 0.0]], [0], [0.0])
+    fw = pa.Framework(np.diag([30.0, 30.0, 30.0]), np.zeros((0, 3)), [], [], [])
+    b = pa.FrameworkBatch([fw], ff, g, pa.EwaldParams(12.0))
+    kT = pa.KB * 300.0
+    for prec in ("f64", "mixed"):
+        r = pa.widom(b, g, T=300.0, ninsert=10000, seed=1, precision=prec)[0]
+        assert abs(r.mu_ex) < 1e-12
+        assert r.K_H == pytest.approx(30.0**3 / kT, rel=1e-14)
+        assert r.q_st == pytest.approx(kT, rel=1e-14)
+        assert r.nsamples == 10000
+
+
+def test_single_lj_atom_radial_integral():
+    """One LJ atom in a 40 Å box against the radial quadrature of 1 - exp(-beta u)
+    (test/widom_tests.jl:97-114), within 4 standard errors."""
+    from scipy.integrate import quad
+    L, sig, eps, rc, T = 40.0, 3.4, 0.0103, 12.0, 300.0
+    ff = pa.ForceField.from_types(["X_"], [sig], [eps], cutoff=rc, tail=False)
+    g = pa.Guest([[0.0, 0.0, 0.0]], [0], [0.0])
+    fw = pa.Framework(np.diag([L, L, L]), [[0.5, 0.5, 0.5]], ["X"], ["X"], [0.0])
+    b = pa.FrameworkBatch([fw], ff, g, pa.EwaldParams(rc))
+    beta = 1 / (pa.KB * T)
+    u = lambda x: 4 * eps * ((sig / x) ** 12 - (sig / x) ** 6)  # noqa: E731
+    integral, _ = quad(lambda x: (1 - np.exp(-beta * u(x))) * x * x, 1e-3, rc, epsrel=1e-10, limit=200)
+    expected = 1 - 4 * np.pi * integral / L**3
+    for prec in ("f64", "mixed"):
+        r = pa.widom(b, g, T=T, ninsert=4_000_000, seed=2, nblocks=20, precision=prec)[0]
+        meanW = r.K_H * pa.KB * T / L**3
+        errW = r.K_H_err * pa.KB * T / L**3
+        assert abs(meanW - expected) < 4 * errW
+
+
+def test_remainder_blocks_and_validation(config1):
+    fw, ff, g, ew = config1
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    r = pa.widom(b, g, T=298.15, ninsert=81, nblocks=10, seed=4)[0]
+    assert r.nsamples == 81 and np.isfinite(r.mu_ex_err)
+    with pytest.raises(pa.ArgumentError, match="ninsert"):
+        pa.widom(b, g, T=298.15, ninsert=5, nblocks=10)
+    other = pa.Guest(g.sites, g.types, g.charges * 2)
+    with pytest.raises(pa.ArgumentError, match="does not match the guest"):
+        pa.widom(b, other, T=298.15, ninsert=100)
