@@ -1,0 +1,210 @@
+"""Widom insertion throughput on one MI355X.
+
+Workload (one step): CO2 Widom insertion with LJ + Ewald into the synthetic hexagonal
+framework of the benchmark's CO2 case -- a = b = 13 Å, c = 15 Å, gamma = 120 deg, 300 random
+atoms (C/H/O/Zn, 1.5 Å separation, Gaussian charges sd 0.5 e shifted to neutral), replicated
+3x3x3 (8,100 atoms); real-space and LJ cutoff 12 Å, Ewald precision 1e-6 (184 coupled
+k-vectors), T = 298.15 K, 1e6 insertions per step, seed 42. Inputs are resident on the device
+before timing starts; every insertion's energy is evaluated (no early rejection).
+
+Prints ONE JSON line. `value` is the mixed-precision path's insertions/s; the FP64 plain
+path is timed in the same run on the same insertion streams. `roofline` prices the energy
+kernel's algorithmic FLOPs (61 per in-cutoff charged pair, 11 per (k-vector, site), 4 per
+k-vector) against the FP32 vector peak, from HIP-event kernel durations. `cpu_baseline` times
+the FP64 CPU oracle (a port of the reference's CPU path) on the host's cores on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "pureadsorb.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+import pureadsorb_amd as pa  # noqa: E402
+from pureadsorb_amd import synth  # noqa: E402
+
+METRIC = "Widom insertions/s on 1 GPU, mixed vs FP64 at |d lnK_H|<=1e-4; grad cost"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+T_K = 298.15
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload():
+    fw = synth.config1_framework(42)
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    return fw, ff, g, ew
+
+
+def porous_workload():
+    """Config 1's cell/replication/guest/Ewald at 80 atoms per cell: about 1% of insertions
+    bind, so ln K_H carries information (the specified 300-atom cell admits no insertion)."""
+    _, ff, g, ew = workload()
+    fw = pa.replicate(synth.synthetic_framework(13.0, 13.0, 15.0, 90, 90, 120, 80, 42, charge_sd=0.5), (3, 3, 3))
+    return fw, ff, g, ew
+
+
+def time_steps(b, g, ninsert, steps, warmup, precision, seed=42):
+    for i in range(warmup):
+        pa.widom(b, g, T=T_K, ninsert=ninsert, seed=seed + i, precision=precision)
+    pa.synchronize()
+    energy_ms, launches = 0.0, 0
+    res = None
+    t0 = time.perf_counter()
+    for i in range(steps):
+        res = pa.widom(b, g, T=T_K, ninsert=ninsert, seed=seed, precision=precision)
+        t = pa.last_timing(b)
+        energy_ms += t["energy_ms"]
+        launches += t["energy_launches"]
+    pa.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, energy_ms, launches, res[0]
+
+
+def cpu_baseline(fw, ff, g, ew, sample):
+    """The FP64 CPU oracle on `sample` insertions of the workload, all host threads (<= 16).
+    Also returns the exact in-cutoff pair count per insertion on those poses."""
+    import oracle as O
+
+    types = [pa.typeindex(ff, s + "_") for s in fw.symbols]
+    o = O.OracleSystem(fw.cell, fw.frac, types, fw.charges, fw.replication, ff.sigma, ff.epsilon, ff.cutoff, ff.tail,
+                       g.sites, g.types, g.charges, ew.cutoff, ew.precision)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o.energies(42, 0, 0, 64, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    _, pairs = o.energies(42, 0, 0, sample, want_pairs=True, nthreads=threads)
+    wall = time.perf_counter() - t0
+    return sample / wall, threads, float(np.mean(pairs)), o.nk
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ninsert", type=int, default=1_000_000)
+    ap.add_argument("--f64-steps", type=int, default=2, help="timed steps of the FP64 plain path (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=4000, help="insertions timed on the CPU oracle (0: skip)")
+    ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one GPU, and --gpus {args.gpus} is not run. "
+              "Widom insertions shard by (system, insertion block) with one small reduction of per-block "
+              "partials, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy_mib} MiB three times")
+        return
+
+    log("device:", pa.device_name())
+    fw, ff, g, ew = workload()
+    t0 = time.perf_counter()
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    log(f"batch: {fw.natoms} atoms, {b.nk[0]} coupled k-vectors, built in {time.perf_counter() - t0:.2f} s")
+
+    wall, ems, nl, r_mx = time_steps(b, g, args.ninsert, args.steps, args.warmup, "mixed")
+    ins_s = args.steps * args.ninsert / wall
+    avg_launch_ms = ems / max(nl, 1)
+    ins_per_launch = args.steps * args.ninsert / max(nl, 1)
+    log(f"mixed: {ins_s:.4e} ins/s, {wall / args.steps * 1e3:.3f} ms/step, energy kernel {avg_launch_ms:.3f} ms/launch")
+
+    out = {
+        "metric": METRIC,
+        "value": ins_s,
+        "unit": "insertions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "CO2 Widom LJ+Ewald, hex cell 13/13/15 A gamma=120, 300 atoms x 3x3x3 = 8100 atoms, "
+                        "rc 12 A, Ewald 1e-6 (184 coupled k), T 298.15 K, 1e6 insertions/step",
+            "ninsert_per_step": args.ninsert,
+            "precision": "FP32 pair + k-space terms, FP64 accumulation, tables and Boltzmann sums",
+            "parallelism": "1 GPU",
+        },
+    }
+
+    f64 = None
+    if args.f64_steps > 0:
+        w64, e64, n64, r64 = time_steps(b, g, args.ninsert, args.f64_steps, 1, "f64")
+        f64 = args.f64_steps * args.ninsert / w64
+        log(f"f64 plain: {f64:.4e} ins/s, energy kernel {e64 / max(n64, 1):.3f} ms/launch")
+        rel = abs(r_mx.ln_K_H - r64.ln_K_H) / abs(r64.ln_K_H)
+        out["f64_plain"] = {"value": f64, "unit": "insertions/s", "ms_per_step": w64 / args.f64_steps * 1e3,
+                            "energy_kernel_ms_per_launch": e64 / max(n64, 1)}
+        out["speedup_mixed_vs_f64_plain"] = ins_s / f64
+        out["parity_workload"] = {
+            "note": "no insertion binds in this framework (min dU ~ +111 eV): ln K_H is finite only through the "
+                    "log-sum-exp sums and is set by repulsive poses",
+            "ln_K_H_mixed": r_mx.ln_K_H, "ln_K_H_f64": r64.ln_K_H, "rel_d_ln_K_H": rel,
+        }
+
+    if not args.no_parity:
+        pfw, pff, pg, pew = porous_workload()
+        pb = pa.FrameworkBatch([pfw], pff, pg, pew)
+        pm = pa.widom(pb, pg, T=T_K, ninsert=args.ninsert, seed=42, precision="mixed")[0]
+        p64 = pa.widom(pb, pg, T=T_K, ninsert=args.ninsert, seed=42, precision="f64")[0]
+        out["parity_porous_variant"] = {
+            "workload": "same cell, replication, guest and Ewald settings, 80 atoms per cell (2160 atoms)",
+            "ninsert": args.ninsert,
+            "d_ln_K_H": abs(pm.ln_K_H - p64.ln_K_H),
+            "d_q_st_eV": abs(pm.q_st - p64.q_st),
+            "K_H_mixed": pm.K_H, "K_H_f64": p64.K_H, "q_st_mixed_eV": pm.q_st, "q_st_f64_eV": p64.q_st,
+        }
+        log(f"porous parity: d ln K_H {abs(pm.ln_K_H - p64.ln_K_H):.2e}, d q_st {abs(pm.q_st - p64.q_st):.2e} eV")
+
+    pin = None
+    if args.cpu_sample > 0:
+        cpu_rate, threads, pin, nk = cpu_baseline(fw, ff, g, ew, args.cpu_sample)
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "insertions/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_sample} insertions of the same workload (seed 42, system 0), FP64 oracle "
+                      f"(C restatement of the reference CPU path), OpenMP over {threads} threads",
+        }
+        log(f"cpu oracle: {cpu_rate:.4e} ins/s on {threads} threads; mean in-cutoff pairs {pin:.1f}")
+    if pin is not None:
+        N, K = g.nsites, int(b.nk[0])
+        f_ins = 61.0 * pin + 11.0 * N * K + 4.0 * K
+        achieved = f_ins * ins_per_launch / (avg_launch_ms * 1e-3) / 1e12
+        out["roofline"] = {
+            "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "kernel": "widom_mixed_kernel<3,true,true>",
+            "flops_per_insertion": f_ins, "in_cutoff_pairs_per_insertion": pin,
+            "kernel_ms_per_launch": avg_launch_ms, "insertions_per_launch": ins_per_launch,
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,10 @@ int pa_last_error(char *buf, size_t len);
 /* Name of the HIP device the library runs on (0 devices is an error). */
 int pa_device_name(char *buf, size_t len);
 
+/* Waits for all work on the device (the reference's KernelAbstractions.synchronize(backend)
+ * after each launch, src/widom.jl:408,427). */
+int pa_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

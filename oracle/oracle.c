@@ -307,6 +307,16 @@ static const float OR_PAIR_COF32[8] = {
     8.555145e-5f, 5.5327768e-5f, -5.2045684e-6f, -8.397902e-7f,
 };
 
+/* Coefficient literals for the regeneration test: which = 0 (erfc_dev, 28), 1 (pair F64, 17),
+ * 2 (pair F32, 8). */
+double or_erfc_coef(int which, int j)
+{
+    if (which == 0 && j >= 0 && j < 28) return OR_ERFC_COF[j];
+    if (which == 1 && j >= 0 && j < 17) return OR_PAIR_COF64[j];
+    if (which == 2 && j >= 0 && j < 8) return (double)OR_PAIR_COF32[j];
+    return NAN;
+}
+
 double or_pair_erfc_f64(double z)
 {
     const double xlo = 2.0 * (2.0 / (2.0 + OR_PAIR_ERFC_XMAX)) - 1.0;

@@ -68,6 +68,8 @@ def lib():
     L.or_rotate.argtypes = [_dp, _dp, _dp]
     L.or_erfc_dev.restype = C.c_double
     L.or_erfc_dev.argtypes = [C.c_double]
+    L.or_erfc_coef.restype = C.c_double
+    L.or_erfc_coef.argtypes = [C.c_int, C.c_int]
     L.or_pair_erfc_f64.restype = C.c_double
     L.or_pair_erfc_f64.argtypes = [C.c_double]
     L.or_pair_erfc_f32.restype = C.c_float
@@ -111,6 +113,8 @@ def lib():
     L.or_default_run.argtypes = [C.c_int64, C.c_int64]
     L.or_system_counts.restype = None
     L.or_system_counts.argtypes = [C.c_int64, C.c_int64, C.c_int64, _lp]
+    L.or_boltzmann_weight.restype = None
+    L.or_boltzmann_weight.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.or_reduce.restype = C.c_int
     L.or_reduce.argtypes = [C.c_int, _dp, _dp, _lp, C.c_double, C.c_double, _dp]
     L.or_widom.restype = C.c_int
@@ -187,6 +191,12 @@ def erfc_dev(x: float) -> float:
     return lib().or_erfc_dev(x)
 
 
+def erfc_coefficients(which: str) -> np.ndarray:
+    """The oracle's coefficient literals: 'erfc_dev' (28), 'pair64' (17), 'pair32' (8)."""
+    w, n = {"erfc_dev": (0, 28), "pair64": (1, 17), "pair32": (2, 8)}[which]
+    return np.array([lib().or_erfc_coef(w, j) for j in range(n)])
+
+
 def pair_erfc_f64(x: float) -> float:
     return lib().or_pair_erfc_f64(x)
 
@@ -239,6 +249,13 @@ def system_counts(ninsert: int, nsys: int, run: int) -> np.ndarray:
     ns = np.zeros(nsys, dtype=np.int64)
     lib().or_system_counts(ninsert, nsys, run, ns)
     return ns
+
+
+def boltzmann_weight(dU: float, kT: float) -> tuple[float, float]:
+    """src/widom.jl:80-84: (exp(-dU/kT), dU*w) with the product forced to 0 when w == 0."""
+    w, uw = C.c_double(), C.c_double()
+    lib().or_boltzmann_weight(dU, kT, C.byref(w), C.byref(uw))
+    return w.value, uw.value
 
 
 def reduce(sW, sUW, n, kT, V):

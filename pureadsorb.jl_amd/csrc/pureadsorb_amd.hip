@@ -337,6 +337,14 @@ extern "C" int pa_device_name(char *buf, size_t len)
     return ok();
 }
 
+extern "C" int pa_synchronize(void)
+{
+    int st = device_ready();
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    return ok();
+}
+
 template <class T>
 static hipError_t upload(DevAlloc &d, const std::vector<T> &v)
 {

@@ -5,7 +5,7 @@ The public names mirror PureAdsorb.jl's API (src/PureAdsorb.jl:27-37): `Framewor
 `EwaldParams`, `FrameworkBatch`, `widom`, `WidomResult`. The compute runs in
 libpureadsorb_amd.so (HIP kernels for gfx950) through its C ABI; there is no CPU fallback.
 """
-from ._lib import ArgumentError, DeviceError, DimensionMismatch, device_name
+from ._lib import ArgumentError, DeviceError, DimensionMismatch, device_name, synchronize
 from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
 from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
@@ -14,7 +14,7 @@ from .widom import (KB, KE, PASCAL, WidomResult, default_run, last_timing, sys_o
                     widom_energies, widom_poses)
 
 __all__ = [
-    "ArgumentError", "DeviceError", "DimensionMismatch", "device_name", "FrameworkBatch", "nframeworks", "EwaldParams",
+    "ArgumentError", "DeviceError", "DimensionMismatch", "device_name", "synchronize", "FrameworkBatch", "nframeworks", "EwaldParams",
     "ForceField", "Guest", "read_forcefield", "read_guest", "tail_coefficient", "tail_delta", "typeindex", "Framework",
     "cartesian", "cell_matrix", "min_multiplicity", "natoms", "perpendicular_lengths", "read_cif", "reciprocal_basis",
     "replicate", "total_charge", "volume", "KB", "KE", "PASCAL", "WidomResult", "default_run", "last_timing",

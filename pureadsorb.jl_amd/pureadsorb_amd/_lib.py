@@ -27,7 +27,7 @@ PA_BATCH_HOST_ONLY = 2
 EXPORTS = (
     "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
     "pa_batch_framework_info", "pa_widom", "pa_widom_energies", "pa_widom_poses", "pa_last_timing",
-    "pa_last_error", "pa_device_name",
+    "pa_last_error", "pa_device_name", "pa_synchronize",
 )
 
 
@@ -140,6 +140,8 @@ def lib():
     L.pa_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.pa_device_name.restype = C.c_int
     L.pa_device_name.argtypes = [C.c_char_p, C.c_size_t]
+    L.pa_synchronize.restype = C.c_int
+    L.pa_synchronize.argtypes = []
     _lib = L
     return L
 
@@ -162,6 +164,11 @@ def check(status: int) -> None:
     if status == PA_ERR_DEVICE:
         raise DeviceError(msg)
     raise RuntimeError(f"status {status}: {msg}")
+
+
+def synchronize() -> None:
+    """Waits for all device work (KernelAbstractions.synchronize(backend) in the reference)."""
+    check(lib().pa_synchronize())
 
 
 def device_name() -> str:

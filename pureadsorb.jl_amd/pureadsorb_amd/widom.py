@@ -74,6 +74,9 @@ def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str 
         raise _lib.ArgumentError(f"nblocks={nblocks}: at least two blocks are needed for a standard error")
     if chunk is not None and chunk < 1:
         raise _lib.ArgumentError(f"chunk={chunk} must be ≥ 1")
+    if run is not None and run < 1:
+        # the C ABI reads run = 0 as "default"; an explicit run must be >= 1 (src/widom.jl:320)
+        raise _lib.ArgumentError(f"run={run} must be ≥ 1")
     T = float(getattr(T, "magnitude", T))
     g = _GuestArgs(guest)
     out = (_lib.PaWidomResult * batch.nsys)()
