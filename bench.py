@@ -199,7 +199,7 @@ def main():
         out["roofline"] = {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-            "kernel": "widom_mixed_kernel<3,true,true>",
+            "kernel": "widom_binned_kernel<3,true,true>",
             "flops_per_insertion": f_ins, "in_cutoff_pairs_per_insertion": pin,
             "kernel_ms_per_launch": avg_launch_ms, "insertions_per_launch": ins_per_launch,
         }

@@ -101,6 +101,7 @@ typedef struct pa_timing {
     int64_t energy_launches;
     double reduce_ms;             /* Boltzmann-sum reduction kernels */
     int64_t insertions;
+    double binning_ms;            /* mixed path: pose, bin count, scan and scatter launches */
 } pa_timing;
 
 typedef struct pa_batch pa_batch;

@@ -58,4 +58,45 @@ struct WidomArgs {
     GuestDev guest;
 };
 
+// Spatial bins of one system's insertions (mixed path): the system's cell is cut into
+// nb[0] x nb[1] x nb[2] parallelepipeds in fractional coordinates; every insertion is grouped
+// with the others whose reference point lies in the same bin, and a bin's neighbour list holds
+// every atom image within sqrt(rlist2) of the bin centre.
+struct BinSys {
+    int nb[3];
+    int bin_base;   // first global bin id of the system
+    float rlist2;   // (max cutoff + guest radius + bin circumradius + margin)^2, Å^2
+    float alt[3];   // an atom with |fractional offset| >= alt[i] on axis i also gets the image
+                    // across the cell boundary on that axis (alt >= 0.5: never)
+};
+
+struct BinnedArgs {
+    const FwDesc *fw;
+    const int *fw_of;
+    const int64_t *seg;        // nsys + 1 flattened insertion offsets
+    const BinSys *bins;        // nsys
+    const int *bin_base;       // nsys + 1
+    int nsys;
+    int nbins;
+    uint64_t seed;
+    int64_t g0, g1;            // this chunk covers g in [g0, g1)
+    uint32_t *sitefx;          // (g - g0) * 3 * NS fixed-point site coordinates
+    int *key;                  // (g - g0) global bin id
+    int *bin_count;            // nbins + 1
+    int *bin_start;            // nbins + 1
+    int *cursor;               // nbins
+    int *order;                // chunk-relative insertion indices grouped by bin
+    double *dU;                // indexed by g - out_base
+    int64_t out_base;
+    const uint4 *atomfx;
+    const int *atype;
+    const int4 *kn;
+    const float2 *kc;
+    const float2 *pair32;
+    int ntypes_c;
+    float rc_lj2f, rc_ew2f, rc_max2f;
+    int listcap;               // neighbour-list capacity (entries) of one pass
+    GuestDev guest;
+};
+
 }  // namespace pa

@@ -22,6 +22,10 @@ hipError_t launch_tiles(const double *dU, const int64_t *tile_g0, const int *til
                         hipStream_t st);
 hipError_t launch_poses(const FwDesc *fw, const int *fw_of, int sys, uint64_t seed, int64_t j0, int64_t n, double *pos,
                         double *quat, hipStream_t st);
+hipError_t binned_scan_bytes(int nbins, size_t *bytes);
+hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
+hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
+size_t binned_lds_bytes_host(int cap, int ns);
 }  // namespace pa
 
 using namespace pa;
@@ -270,6 +274,7 @@ struct DevAlloc {
 struct FwHost {
     int natoms, nk, nk_full;
     double volume, alpha, const_off, halfrange;
+    double perp[3];  // perpendicular widths, Å
 };
 
 struct pa_batch {
@@ -285,17 +290,22 @@ struct pa_batch {
     GuestDev gdev{};
     int ntypes_c = 0;
     double rc_lj = 0, rc_ew = 0;
+    double r_guest = 0;  // largest site distance from the guest's reference point
     // device tables
     DevAlloc d_fw, d_fw_of, d_atom64, d_atype, d_k64, d_S64, d_atomfx, d_kn, d_kc, d_pair64, d_pair32;
     // per-call scratch
     DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
+    // mixed-path binning scratch
+    DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_cursor, d_order, d_scan_tmp;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch
     pa_timing last{};
     bool host_only = false;
     ~pa_batch()
     {
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto &e : kev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -382,7 +392,10 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     b->rc_lj = rc_lj;
     b->rc_ew = rc_ew;
     bool charged = false;
-    for (int a = 0; a < NS; ++a) charged = charged || guest->charges[a] != 0.0;
+    for (int a = 0; a < NS; ++a) {
+        charged = charged || guest->charges[a] != 0.0;
+        b->r_guest = std::max(b->r_guest, norm3(guest->sites + 3 * a));
+    }
 
     // ---- dedup by value: cell, replication, fractional coordinates, types, charges
     // (src/batch.jl:246-265)
@@ -600,7 +613,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         const double self = -alpha / std::sqrt(PI) * q2;
         const double net = -PI / (2.0 * V * alpha * alpha) * ((Qh + Qg) * (Qh + Qg) - Qh * Qh);
         D.const_off = tail + KE * (self + excl + net) + self_mean;
-        b->info[u] = FwHost{n, D.nk, nkf, V, alpha, D.const_off, halfrange};
+        b->info[u] = FwHost{n, D.nk, nkf, V, alpha, D.const_off, halfrange, {L[0], L[1], L[2]}};
     }
 
     // ---- guest and pair tables
@@ -738,6 +751,135 @@ static LseH lse_merge_h(LseH x, LseH y)
     return LseH{m, x.w * ex + y.w * ey, x.u * ex + y.u * ey};
 }
 
+// Smallest bin edge (Å) of the mixed path, and the insertions a bin aims to hold when a system
+// has few insertions for its volume.
+constexpr double BIN_EDGE_MIN = 2.0;
+constexpr double BIN_TARGET_INSERTIONS = 64.0;
+constexpr int LISTCAP_MAX = 3072;
+
+// Delta U of flattened insertions g in [gb, ge) (system s owns g in [seg[s], seg[s+1])) into
+// b->d_dU[g - out_base], in chunks of at most `chunk` insertions. Records a HIP event pair around
+// every energy-kernel launch (b->kev) and returns the launch count in *launches.
+static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb, int64_t ge, int64_t out_base,
+                        uint64_t seed, uint32_t precision, int64_t chunk, int64_t *launches)
+{
+    const int nsys = b->nsys;
+    HIP_TRY(upload(b->d_seg, seg));
+    const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
+    const int64_t nchunks = (ge - gb + CH - 1) / CH;
+    while ((int64_t)b->kev.size() < 2 * nchunks) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        b->kev.push_back(e);
+    }
+    *launches = 0;
+    if (precision == PA_PREC_F64) {
+        WidomArgs a = base_args(b);
+        a.seg = b->d_seg.as<int64_t>();
+        a.seed = seed;
+        a.dU = b->d_dU.as<double>();
+        a.out_base = out_base;
+        for (int64_t g0 = gb; g0 < ge; g0 += CH) {
+            a.g0 = g0;
+            a.g1 = std::min(ge, g0 + CH);
+            HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
+            HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
+            HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
+            ++*launches;
+        }
+        return PA_OK;
+    }
+    // ---- mixed path: bin grid of every system, sized by its insertions in [gb, ge)
+    const double rcmax = std::max(b->rc_lj, b->gdev.charged ? b->rc_ew : 0.0);
+    std::vector<BinSys> bins(nsys);
+    std::vector<int> base(nsys + 1, 0);
+    double cap_est = 256.0;
+    for (int s = 0; s < nsys; ++s) {
+        const FwHost &h = b->info[b->fw_of[s]];
+        const FwDesc &D = b->desc[b->fw_of[s]];
+        const int64_t lo = std::max(gb, seg[s]), hi = std::min(ge, seg[s + 1]);
+        const double nins = (double)std::max<int64_t>(hi - lo, 1);
+        const double edge = std::max(BIN_EDGE_MIN, std::cbrt(BIN_TARGET_INSERTIONS * h.volume / nins));
+        BinSys &B = bins[s];
+        int64_t nb = 1;
+        for (int d = 0; d < 3; ++d) {
+            B.nb[d] = std::max(1, std::min(1024, (int)std::floor(h.perp[d] / edge)));
+            nb *= B.nb[d];
+        }
+        if (nb > (1 << 22)) return fail(PA_ERR_INTERNAL, "system %d would need %lld bins", s + 1, (long long)nb);
+        // circumradius of one bin: half its longest body diagonal
+        double rb = 0.0;
+        for (int sy = -1; sy <= 1; sy += 2)
+            for (int sz = -1; sz <= 1; sz += 2) {
+                const double f[3] = {0.5 / B.nb[0], 0.5 * sy / B.nb[1], 0.5 * sz / B.nb[2]};
+                double v[3];
+                matvec(D.A, f, v);
+                rb = std::max(rb, norm3(v));
+            }
+        const double R = rcmax + b->r_guest + rb + 0.02;
+        B.rlist2 = (float)(R * R);
+        for (int d = 0; d < 3; ++d) B.alt[d] = (float)std::max(0.0, 1.0 - R / h.perp[d]);
+        B.bin_base = base[s];
+        base[s + 1] = base[s] + (int)nb;
+        cap_est = std::max(cap_est, 1.3 * h.natoms / h.volume * (4.0 / 3.0) * PI * R * R * R + 256.0);
+    }
+    const int nbins = base[nsys];
+    const int listcap = (int)std::min<double>(LISTCAP_MAX, std::ceil(cap_est / 64.0) * 64.0);
+    HIP_TRY(upload(b->d_bins, bins));
+    HIP_TRY(upload(b->d_bin_base, base));
+    HIP_TRY(b->d_bin_count.ensure(sizeof(int) * (nbins + 1)));
+    HIP_TRY(b->d_bin_start.ensure(sizeof(int) * (nbins + 1)));
+    HIP_TRY(b->d_cursor.ensure(sizeof(int) * nbins));
+    size_t scan_bytes = 0;
+    HIP_TRY(binned_scan_bytes(nbins, &scan_bytes));
+    HIP_TRY(b->d_scan_tmp.ensure(scan_bytes));
+    const int64_t chunk_n = std::min<int64_t>(CH, ge - gb);
+    HIP_TRY(b->d_sitefx.ensure(sizeof(uint32_t) * 3 * b->nsites * chunk_n));
+    HIP_TRY(b->d_key.ensure(sizeof(int) * chunk_n));
+    HIP_TRY(b->d_order.ensure(sizeof(int) * chunk_n));
+
+    BinnedArgs A{};
+    A.fw = b->d_fw.as<FwDesc>();
+    A.fw_of = b->d_fw_of.as<int>();
+    A.seg = b->d_seg.as<int64_t>();
+    A.bins = b->d_bins.as<BinSys>();
+    A.bin_base = b->d_bin_base.as<int>();
+    A.nsys = nsys;
+    A.nbins = nbins;
+    A.seed = seed;
+    A.sitefx = b->d_sitefx.as<uint32_t>();
+    A.key = b->d_key.as<int>();
+    A.bin_count = b->d_bin_count.as<int>();
+    A.bin_start = b->d_bin_start.as<int>();
+    A.cursor = b->d_cursor.as<int>();
+    A.order = b->d_order.as<int>();
+    A.dU = b->d_dU.as<double>();
+    A.out_base = out_base;
+    A.atomfx = b->d_atomfx.as<uint4>();
+    A.atype = b->d_atype.as<int>();
+    A.kn = b->d_kn.as<int4>();
+    A.kc = b->d_kc.as<float2>();
+    A.pair32 = b->d_pair32.as<float2>();
+    A.ntypes_c = b->ntypes_c;
+    A.rc_lj2f = (float)(b->rc_lj * b->rc_lj);
+    A.rc_ew2f = b->gdev.charged ? (float)(b->rc_ew * b->rc_ew) : 0.0f;
+    A.rc_max2f = std::max(A.rc_lj2f, A.rc_ew2f);
+    A.listcap = listcap;
+    A.guest = b->gdev;
+    for (int64_t g0 = gb; g0 < ge; g0 += CH) {
+        A.g0 = g0;
+        A.g1 = std::min(ge, g0 + CH);
+        HIP_TRY(hipMemsetAsync(A.bin_count, 0, sizeof(int) * (nbins + 1), b->stream));
+        HIP_TRY(hipMemsetAsync(A.cursor, 0, sizeof(int) * nbins, b->stream));
+        HIP_TRY(launch_binning(A, b->d_scan_tmp.p, scan_bytes, b->stream));
+        HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
+        HIP_TRY(launch_binned_energy(A, b->upper_all, b->stream));
+        HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
+        ++*launches;
+    }
+    return PA_OK;
+}
+
 extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
                         int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out)
 {
@@ -782,7 +924,6 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
     std::vector<int64_t> seg(nsys + 1, 0);
     for (int64_t s = 0; s < nsys; ++s) seg[s + 1] = seg[s] + ns[s];
     const int64_t G = seg[nsys];
-    HIP_TRY(upload(b->d_seg, seg));
     HIP_TRY(b->d_dU.ensure(sizeof(double) * G));
 
     // tiles: each block of each system split into runs of at most TILE insertions
@@ -806,20 +947,10 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
     HIP_TRY(upload(b->d_tile_n, tile_n));
     HIP_TRY(b->d_tiles.ensure(sizeof(double4) * ntiles));
 
-    WidomArgs a = base_args(b);
-    a.seg = b->d_seg.as<int64_t>();
-    a.seed = seed;
-    a.dU = b->d_dU.as<double>();
-    a.out_base = 0;
-    const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     int64_t launches = 0;
-    for (int64_t g0 = 0; g0 < G; g0 += CH) {
-        a.g0 = g0;
-        a.g1 = std::min(G, g0 + CH);
-        HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
-        ++launches;
-    }
+    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches);
+    if (st) return st;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(launch_tiles(b->d_dU.as<double>(), b->d_tile_g0.as<int64_t>(), b->d_tile_n.as<int>(), ntiles, kT,
                          b->d_tiles.as<double4>(), b->stream));
@@ -831,7 +962,13 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
     HIP_TRY(hipEventElapsedTime(&e01, b->ev[0], b->ev[1]));
     HIP_TRY(hipEventElapsedTime(&e12, b->ev[1], b->ev[2]));
     HIP_TRY(hipEventElapsedTime(&e02, b->ev[0], b->ev[2]));
-    b->last = pa_timing{e02, e01, launches, e12, G};
+    double kernel_ms = 0.0;
+    for (size_t i = 0; i + 1 < b->kev.size() && (int64_t)(i / 2) < launches; i += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, b->kev[i], b->kev[i + 1]));
+        kernel_ms += ms;
+    }
+    b->last = pa_timing{e02, kernel_ms, launches, e12, G, e01 - kernel_ms};
 
     // combine tiles per (system, block) in order, then the reference's block statistics
     std::vector<LseH> blk((size_t)nsys * nblocks, LseH{-INFINITY, 0.0, 0.0});
@@ -918,19 +1055,13 @@ extern "C" int pa_widom_energies(pa_batch *b, int32_t sys, uint64_t seed, int64_
     if (sys < 0 || sys >= b->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, b->nsys - 1);
     if (j0 < 0 || n < 0) return fail(PA_ERR_ARG, "insertion range j0=%lld, n=%lld is invalid", (long long)j0, (long long)n);
     if (n == 0) return ok();
-    // system `sys` owns g in [0, j0 + n); the launch covers g in [j0, j0 + n)
+    // system `sys` owns g in [0, j0 + n); the launches cover g in [j0, j0 + n)
     std::vector<int64_t> seg(b->nsys + 1, 0);
     for (int s = sys + 1; s <= b->nsys; ++s) seg[s] = j0 + n;
-    HIP_TRY(upload(b->d_seg, seg));
     HIP_TRY(b->d_dU.ensure(sizeof(double) * n));
-    WidomArgs a = base_args(b);
-    a.seg = b->d_seg.as<int64_t>();
-    a.seed = seed;
-    a.dU = b->d_dU.as<double>();
-    a.out_base = j0;
-    a.g0 = j0;
-    a.g1 = j0 + n;
-    HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
+    int64_t launches = 0;
+    st = run_energies(b, seg, j0, j0 + n, j0, seed, precision, 0, &launches);
+    if (st) return st;
     HIP_TRY(hipMemcpyAsync(dU, b->d_dU.p, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     return ok();

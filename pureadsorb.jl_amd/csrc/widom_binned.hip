@@ -1,0 +1,433 @@
+// widom_binned.hip -- the mixed-precision Widom pipeline for gfx950.
+//
+// One chunk of insertions g in [g0, g1) runs four launches:
+//  1. bin_pose_kernel       draws each pose (FP64, bit-identical to the oracle's), stores its sites
+//                           as 32-bit fixed-point fractional coordinates, and counts insertions
+//                           per spatial bin of the system's cell;
+//  2. exclusive scan of the bin counts (hipCUB);
+//  3. bin_scatter_kernel    groups the chunk's insertions by bin;
+//  4. widom_binned_kernel   one workgroup per bin: builds the bin's neighbour list in LDS (every
+//                           atom image within rc + r_guest + r_bin of the bin centre, FP32
+//                           coordinates relative to that centre, computed in FP64), then each
+//                           wavefront takes one insertion at a time, sweeps the list with lanes
+//                           over atoms, compacts in-cutoff (site, atom) pairs into a per-wave LDS
+//                           ring, and evaluates the FP32 LJ + screened-Coulomb terms on full
+//                           wavefronts into FP64 accumulators; the reciprocal cross term uses
+//                           exact fixed-point phases n.f mod 1 and the hardware sin/cos.
+// Every insertion's energy depends only on its own pose and its bin's list, so results do not
+// depend on chunking or on the order insertions are grouped in.
+#include <hipcub/hipcub.hpp>
+
+#include "pa_device.hpp"
+
+namespace pa {
+
+constexpr int BIN_WG = 256;
+constexpr int BIN_NW = BIN_WG / 64;
+constexpr int BRING = 128;  // per-wave ring; drained after every site sweep (< 64 + 64 pending)
+constexpr int BIN_MAX_TYPES = 64;
+
+__device__ inline int find_sys64(const int64_t *seg, int nsys, int64_t g)
+{
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int find_sys32(const int *base, int nsys, int b)
+{
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Orders this wave's LDS ring accesses (written and read by different lanes of one wave).
+__device__ inline void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
+{
+#pragma clang fp contract(off)
+    const int64_t g = a.g0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.g1) return;
+    const int s = find_sys64(a.seg, a.nsys, g);
+    const int64_t j = g - a.seg[s];
+    const FwDesc &F = a.fw[a.fw_of[s]];
+    double f[3], q[4];
+    pose_uniforms(a.seed, s, j, f, q);
+    uint32_t *out = a.sitefx + (size_t)(g - a.g0) * 3 * NS;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        double r[3], df[3];
+        rotate(q, a.guest.site[t], r);
+        matvec(F.invA, r, df);
+        out[3 * t + 0] = to_fixed(f[0] + df[0]);
+        out[3 * t + 1] = to_fixed(f[1] + df[1]);
+        out[3 * t + 2] = to_fixed(f[2] + df[2]);
+    }
+    const BinSys B = a.bins[s];
+    const int bx = min(B.nb[0] - 1, (int)(f[0] * B.nb[0]));
+    const int by = min(B.nb[1] - 1, (int)(f[1] * B.nb[1]));
+    const int bz = min(B.nb[2] - 1, (int)(f[2] * B.nb[2]));
+    const int key = B.bin_base + bx + B.nb[0] * (by + B.nb[1] * bz);
+    a.key[g - a.g0] = key;
+    atomicAdd(&a.bin_count[key], 1);
+}
+
+__global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.g1 - a.g0) return;
+    const int k = a.key[i];
+    a.order[a.bin_start[k] + atomicAdd(&a.cursor[k], 1)] = (int)i;
+}
+
+template <bool CHARGED>
+__device__ inline float pair_term(float r2, float qj, int info, const float2 *pair, int ntc, const float *qke,
+                                  float rc_lj2, float rc_ew2, float alpha, float alpha2)
+{
+    const int site = info >> 16, t = info & 0xffff;
+    r2 = fmaxf(r2, 1.0e-30f);
+    float e = 0.0f;
+    if (r2 < rc_lj2) {
+        const float2 p = pair[site * ntc + t];
+        const float x = p.x * __builtin_amdgcn_rcpf(r2);
+        const float x3 = x * x * x;
+        e = p.y * x3 * (x3 - 1.0f);
+    }
+    if (CHARGED && r2 < rc_ew2) {
+        // pair_erfc_dev in Float32 (src/ewald.jl:242-254): the reference's 8-term series
+        constexpr float XLO = (float)PAIR_ERFC_XLO;
+        constexpr float XA = 4.0f / (1.0f - XLO);
+        constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float z = alpha * (r2 * rinv);
+        const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
+        const float xi = XA * tt + XB;
+        const float xi2 = 2.0f * xi;
+        float d = 0.0f, dd = 0.0f;
+#pragma unroll
+        for (int jj = 7; jj >= 1; --jj) {
+            const float tmp = d;
+            d = xi2 * d - dd + pair_erfc_c32(jj);
+            dd = tmp;
+        }
+        const float arg = xi * d - dd + 0.5f * pair_erfc_c32(0) - alpha2 * r2;
+        e += qke[site] * qj * (tt * __builtin_amdgcn_exp2f(arg * 1.4426950408889634f)) * rinv;
+    }
+    return e;
+}
+
+__device__ inline double wave_sum64(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Dynamic LDS layout of widom_binned_kernel for a list capacity `cap`.
+__host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
+{
+    size_t off = (size_t)cap * 16;                      // float4 entries
+    off += ((size_t)cap * 2 + 15) & ~(size_t)15;        // uint16 types
+    off += (size_t)BIN_NW * BRING * 12;                 // rings
+    off += (size_t)ns * BIN_MAX_TYPES * 8;              // pair table
+    off += 16 * 4;                                      // KE * q per site
+    off += (BIN_NW + 1) * 4;                            // wave counts
+    return off;
+}
+
+template <int NS, bool CHARGED, bool UPPER>
+__global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.x;
+    const int i0 = a.bin_start[b];
+    const int nins = a.bin_start[b + 1] - i0;
+    if (nins == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int cap = a.listcap;
+    const int ntc = a.ntypes_c;
+
+    float4 *ent = (float4 *)smem;
+    unsigned short *etype = (unsigned short *)(smem + (size_t)cap * 16);
+    size_t off = (size_t)cap * 16 + (((size_t)cap * 2 + 15) & ~(size_t)15);
+    float *ring_r2 = (float *)(smem + off) + wave * BRING * 3;
+    float *ring_q = ring_r2 + BRING;
+    int *ring_info = (int *)(ring_q + BRING);
+    off += (size_t)BIN_NW * BRING * 12;
+    float2 *pair_lds = (float2 *)(smem + off);
+    off += (size_t)NS * BIN_MAX_TYPES * 8;
+    float *qke_lds = (float *)(smem + off);
+    off += 16 * 4;
+    int *wcount = (int *)(smem + off);
+
+    const int s = find_sys32(a.bin_base, a.nsys, b);
+    const BinSys B = a.bins[s];
+    const FwDesc &F = a.fw[a.fw_of[s]];
+    const int local = b - B.bin_base;
+    const int bx = local % B.nb[0], by = (local / B.nb[0]) % B.nb[1], bz = local / (B.nb[0] * B.nb[1]);
+    const uint32_t cx = to_fixed((bx + 0.5) / B.nb[0]);
+    const uint32_t cy = to_fixed((by + 0.5) / B.nb[1]);
+    const uint32_t cz = to_fixed((bz + 0.5) / B.nb[2]);
+    for (int i = tid; i < NS * ntc; i += BIN_WG) pair_lds[(i / ntc) * ntc + (i % ntc)] = a.pair32[i];
+    if (tid < NS) qke_lds[tid] = a.guest.qke[tid];
+
+    const double A0 = F.A[0], A1 = F.A[1], A2 = F.A[2], A3 = F.A[3], A4 = F.A[4], A5 = F.A[5], A6 = F.A[6],
+                 A7 = F.A[7], A8 = F.A[8];
+    const uint4 *atoms = a.atomfx + F.atom_off;
+    const int *types = a.atype + F.atom_off;
+    const int natoms = F.natoms;
+    const double rl2 = (double)B.rlist2;
+    const float rc_lj2 = a.rc_lj2f, rc_ew2 = a.rc_ew2f, rc_max2 = a.rc_max2f;
+    const float alpha = F.alphaf, alpha2 = F.alphaf * F.alphaf;
+    constexpr double FIX = 2.3283064365386963e-10;  // 2^-32
+    __syncthreads();
+
+    int combo = 0, p = 0, pass = 0;
+    for (;;) {
+        // ---- build (a slice of) the bin's neighbour list
+        int cnt = 0;
+        bool full = false;
+        while (combo < 8 && !full) {
+            bool enabled = true;
+            for (int d = 0; d < 3; ++d)
+                if (((combo >> d) & 1) && !(B.alt[d] < 0.5f)) enabled = false;
+            if (!enabled) {
+                ++combo;
+                p = 0;
+                continue;
+            }
+            while (p < natoms) {
+                const int n = p + tid;
+                bool inc = false;
+                float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                int et = 0;
+                if (n < natoms) {
+                    const uint4 at = atoms[n];
+                    double f0 = (double)(int32_t)(at.x - cx) * FIX;
+                    double f1 = (double)(int32_t)(at.y - cy) * FIX;
+                    double f2 = (double)(int32_t)(at.z - cz) * FIX;
+                    bool ok = true;
+                    if (combo & 1) { if (fabs(f0) >= B.alt[0]) f0 -= (f0 > 0 ? 1.0 : -1.0); else ok = false; }
+                    if (combo & 2) { if (fabs(f1) >= B.alt[1]) f1 -= (f1 > 0 ? 1.0 : -1.0); else ok = false; }
+                    if (combo & 4) { if (fabs(f2) >= B.alt[2]) f2 -= (f2 > 0 ? 1.0 : -1.0); else ok = false; }
+                    if (ok) {
+                        const double X = A0 * f0 + A3 * f1 + A6 * f2;
+                        const double Y = UPPER ? A4 * f1 + A7 * f2 : A1 * f0 + A4 * f1 + A7 * f2;
+                        const double Z = UPPER ? A8 * f2 : A2 * f0 + A5 * f1 + A8 * f2;
+                        if (X * X + Y * Y + Z * Z <= rl2) {
+                            inc = true;
+                            e = make_float4((float)X, (float)Y, (float)Z, __uint_as_float(at.w));
+                            et = types[n];
+                        }
+                    }
+                }
+                const uint64_t m = __ballot(inc);
+                if (lane == 0) wcount[wave] = __popcll(m);
+                __syncthreads();
+                int base = 0, total = 0;
+                for (int w = 0; w < BIN_NW; ++w) {
+                    const int c = wcount[w];
+                    base += (w < wave) ? c : 0;
+                    total += c;
+                }
+                __syncthreads();
+                if (cnt + total > cap) {
+                    full = true;
+                    break;
+                }
+                if (inc) {
+                    const int slot = cnt + base + __popcll(m & below);
+                    ent[slot] = e;
+                    etype[slot] = (unsigned short)et;
+                }
+                cnt += total;
+                p += BIN_WG;
+            }
+            if (!full) {
+                ++combo;
+                p = 0;
+            }
+        }
+        __syncthreads();
+
+        // ---- evaluate the bin's insertions against the list
+        for (int ii = wave; ii < nins; ii += BIN_NW) {
+            const int grel = a.order[i0 + ii];
+            const uint32_t *sf = a.sitefx + (size_t)grel * 3 * NS;
+            uint32_t sfx[NS][3];
+            float sxr[NS], syr[NS], szr[NS];
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                for (int d = 0; d < 3; ++d) sfx[t][d] = __builtin_amdgcn_readfirstlane(sf[3 * t + d]);
+                const double f0 = (double)(int32_t)(sfx[t][0] - cx) * FIX;
+                const double f1 = (double)(int32_t)(sfx[t][1] - cy) * FIX;
+                const double f2 = (double)(int32_t)(sfx[t][2] - cz) * FIX;
+                sxr[t] = (float)(A0 * f0 + A3 * f1 + A6 * f2);
+                syr[t] = (float)(UPPER ? A4 * f1 + A7 * f2 : A1 * f0 + A4 * f1 + A7 * f2);
+                szr[t] = (float)(UPPER ? A8 * f2 : A2 * f0 + A5 * f1 + A8 * f2);
+            }
+            double acc = 0.0;
+            uint32_t head = 0, tail = 0;
+            for (int e0 = 0; e0 < cnt; e0 += 64) {
+                const int e = e0 + lane;
+                const bool valid = e < cnt;
+                float4 E = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+                int ht = 0;
+                if (valid) {
+                    E = ent[e];
+                    ht = etype[e];
+                }
+#pragma unroll
+                for (int t = 0; t < NS; ++t) {
+                    const float dx = sxr[t] - E.x, dy = syr[t] - E.y, dz = szr[t] - E.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const bool in = valid && (r2 < rc_max2);
+                    const uint64_t mask = __ballot(in);
+                    if (in) {
+                        const uint32_t slot = (tail + (uint32_t)__popcll(mask & below)) & (BRING - 1);
+                        ring_r2[slot] = r2;
+                        ring_q[slot] = E.w;
+                        ring_info[slot] = (t << 16) | ht;
+                    }
+                    tail += (uint32_t)__popcll(mask);
+                    if (tail - head >= 64u) {
+                        wave_lds_fence();
+                        const uint32_t slot = (head + lane) & (BRING - 1);
+                        acc += (double)pair_term<CHARGED>(ring_r2[slot], ring_q[slot], ring_info[slot], pair_lds, ntc,
+                                                          qke_lds, rc_lj2, rc_ew2, alpha, alpha2);
+                        head += 64u;
+                        wave_lds_fence();
+                    }
+                }
+            }
+            wave_lds_fence();
+            if ((uint32_t)lane < tail - head) {
+                const uint32_t slot = (head + lane) & (BRING - 1);
+                acc += (double)pair_term<CHARGED>(ring_r2[slot], ring_q[slot], ring_info[slot], pair_lds, ntc, qke_lds,
+                                                  rc_lj2, rc_ew2, alpha, alpha2);
+            }
+            wave_lds_fence();
+            if (CHARGED && pass == 0) {
+                const int4 *kn = a.kn + F.k_off;
+                const float2 *kc = a.kc + F.k_off;
+                for (int kb = 0; kb < F.nk; kb += 64) {
+                    const int k = kb + lane;
+                    if (k < F.nk) {
+                        const int4 nn = kn[k];
+                        const float2 c = kc[k];
+                        float re = 0.0f, im = 0.0f;
+#pragma unroll
+                        for (int t = 0; t < NS; ++t) {
+                            const uint32_t ph =
+                                (uint32_t)nn.x * sfx[t][0] + (uint32_t)nn.y * sfx[t][1] + (uint32_t)nn.z * sfx[t][2];
+                            const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
+                            const float qs = qke_lds[t];
+                            re += qs * __builtin_amdgcn_cosf(rev);
+                            im += qs * __builtin_amdgcn_sinf(rev);
+                        }
+                        acc += (double)(c.x * re + c.y * im);
+                    }
+                }
+            }
+            const double Etot = wave_sum64(acc);
+            if (lane == 0) {
+                double *o = a.dU + (a.g0 + grel - a.out_base);
+                if (pass == 0) *o = Etot + F.const_off;
+                else *o += Etot;
+            }
+        }
+        if (combo >= 8) break;
+        __syncthreads();
+        ++pass;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side launcher of one chunk
+// ------------------------------------------------------------------------------------------
+template <int NS>
+static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t st)
+{
+    const int64_t n = a.g1 - a.g0;
+    const dim3 g256((unsigned)((n + 255) / 256)), b256(256);
+    hipLaunchKernelGGL((bin_pose_kernel<NS>), g256, b256, 0, st, a);
+    return hipGetLastError();
+}
+
+template <int NS>
+static hipError_t launch_binned_energy_ns(const BinnedArgs &a, bool upper, hipStream_t st)
+{
+    const size_t lds = binned_lds_bytes(a.listcap, NS);
+    const dim3 grid((unsigned)a.nbins), block(BIN_WG);
+    if (a.guest.charged) {
+        if (upper) hipLaunchKernelGGL((widom_binned_kernel<NS, true, true>), grid, block, lds, st, a);
+        else hipLaunchKernelGGL((widom_binned_kernel<NS, true, false>), grid, block, lds, st, a);
+    } else {
+        if (upper) hipLaunchKernelGGL((widom_binned_kernel<NS, false, true>), grid, block, lds, st, a);
+        else hipLaunchKernelGGL((widom_binned_kernel<NS, false, false>), grid, block, lds, st, a);
+    }
+    return hipGetLastError();
+}
+
+size_t binned_lds_bytes_host(int cap, int ns) { return binned_lds_bytes(cap, ns); }
+
+// Scan temporary storage size for `nbins + 1` counts.
+hipError_t binned_scan_bytes(int nbins, size_t *bytes)
+{
+    *bytes = 0;
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (int *)nullptr, (int *)nullptr, nbins + 1);
+}
+
+// Steps 1-3 of the pipeline (pose + bin counts, scan, scatter). The caller has zeroed
+// bin_count[0 .. nbins] and cursor[0 .. nbins-1].
+hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st)
+{
+    const int64_t n = a.g1 - a.g0;
+    if (n <= 0) return hipSuccess;
+    hipError_t e;
+    switch (a.guest.nsites) {
+    case 1: e = launch_binned_ns<1>(a, true, st); break;
+    case 2: e = launch_binned_ns<2>(a, true, st); break;
+    case 3: e = launch_binned_ns<3>(a, true, st); break;
+    case 4: e = launch_binned_ns<4>(a, true, st); break;
+    case 5: e = launch_binned_ns<5>(a, true, st); break;
+    case 6: e = launch_binned_ns<6>(a, true, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    size_t bytes = scan_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, a.bin_count, a.bin_start, a.nbins + 1, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// Step 4: the binned energy kernel.
+hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st)
+{
+    if (a.g1 <= a.g0) return hipSuccess;
+    switch (a.guest.nsites) {
+    case 1: return launch_binned_energy_ns<1>(a, upper, st);
+    case 2: return launch_binned_energy_ns<2>(a, upper, st);
+    case 3: return launch_binned_energy_ns<3>(a, upper, st);
+    case 4: return launch_binned_energy_ns<4>(a, upper, st);
+    case 5: return launch_binned_energy_ns<5>(a, upper, st);
+    case 6: return launch_binned_energy_ns<6>(a, upper, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pa

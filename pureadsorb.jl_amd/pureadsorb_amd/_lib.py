@@ -98,7 +98,7 @@ class PaFrameworkInfo(C.Structure):
 class PaTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
-        ("reduce_ms", C.c_double), ("insertions", C.c_int64),
+        ("reduce_ms", C.c_double), ("insertions", C.c_int64), ("binning_ms", C.c_double),
     ]
 
 

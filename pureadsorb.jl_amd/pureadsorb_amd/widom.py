@@ -109,4 +109,4 @@ def last_timing(batch: FrameworkBatch) -> dict:
     t = _lib.PaTiming()
     _lib.check(_lib.lib().pa_last_timing(batch.handle, C.byref(t)))
     return {"total_ms": t.total_ms, "energy_ms": t.energy_ms, "energy_launches": int(t.energy_launches),
-            "reduce_ms": t.reduce_ms, "insertions": int(t.insertions)}
+            "reduce_ms": t.reduce_ms, "insertions": int(t.insertions), "binning_ms": t.binning_ms}
