@@ -59,6 +59,22 @@ def porous_workload():
     return fw, ff, g, ew
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "widom_binned_config1_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per 1e6-insertion launch of the binned kernel on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a calibration copy)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_steps(b, g, ninsert, steps, warmup, precision, seed=42):
     for i in range(warmup):
         pa.widom(b, g, T=T_K, ninsert=ninsert, seed=seed + i, precision=precision)
@@ -101,13 +117,15 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=2, help="timed steps of the FP64 plain path (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=4000, help="insertions timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
+    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
+                    help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU, and --gpus {args.gpus} is not run. "
               "Widom insertions shard by (system, insertion block) with one small reduction of per-block "
-              "partials, see DESIGN.md.", file=sys.stderr)
+              "log-sum-exp partials.", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy_mib:
         import torch
@@ -121,6 +139,8 @@ def main():
         return
 
     log("device:", pa.device_name())
+    if args.workload == "config2":
+        return bench_config2(args)
     fw, ff, g, ew = workload()
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
@@ -196,13 +216,52 @@ def main():
         N, K = g.nsites, int(b.nk[0])
         f_ins = 61.0 * pin + 11.0 * N * K + 4.0 * K
         achieved = f_ins * ins_per_launch / (avg_launch_ms * 1e-3) / 1e12
+        traffic = measured_traffic() if args.ninsert == 1_000_000 else None
         out["roofline"] = {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
             "kernel": "widom_binned_kernel<3,true,true>",
             "flops_per_insertion": f_ins, "in_cutoff_pairs_per_insertion": pin,
             "kernel_ms_per_launch": avg_launch_ms, "insertions_per_launch": ins_per_launch,
         }
+    print(json.dumps(out), flush=True)
+
+
+def bench_config2(args):
+    """256 synthetic frameworks (200-6000 atoms per cell, log-uniform; cells 10-40 Å, angles
+    70-110 deg; replicated to widths >= 24 Å), CO2 LJ + Ewald, 1e5 insertions each in one call."""
+    fws = synth.config2_frameworks(256)
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    t0 = time.perf_counter()
+    b = pa.FrameworkBatch(fws, ff, g, ew)
+    build_s = time.perf_counter() - t0
+    ninsert = 256 * 100_000
+    log(f"config2 batch: {sum(f.natoms for f in fws)} atoms in 256 frameworks, built in {build_s:.1f} s")
+    wall, ems, nl, _ = time_steps(b, g, ninsert, args.steps, args.warmup, "mixed")
+    rm = pa.widom(b, g, T=T_K, ninsert=ninsert, seed=42, precision="mixed")
+    t64 = time.perf_counter()
+    r64 = pa.widom(b, g, T=T_K, ninsert=ninsert, seed=42, precision="f64")
+    w64 = time.perf_counter() - t64
+    ins_s = args.steps * ninsert / wall
+    dln = np.array([abs(a.ln_K_H - c.ln_K_H) for a, c in zip(rm, r64)])
+    rel = dln / np.maximum(1.0, np.abs([c.ln_K_H for c in r64]))
+    out = {
+        "metric": METRIC + " (256-framework batch)", "value": ins_s, "unit": "insertions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "256 frameworks, 200-6000 atoms/cell log-uniform, cells 10-40 A, angles 70-110, "
+                               "widths >= 24 A, CO2 LJ+Ewald rc 12 A, 1e5 insertions each", "ninsert_per_step": ninsert},
+        "batch_build_s": build_s,
+        "f64_plain": {"value": ninsert / w64, "unit": "insertions/s"},
+        "speedup_mixed_vs_f64_plain": ins_s / (ninsert / w64),
+        "energy_kernel_ms_per_step": ems / args.steps,
+        "K_H_mixed": [r.K_H for r in rm], "K_H_f64": [r.K_H for r in r64],
+        "ln_K_H_f64": [r.ln_K_H for r in r64],
+        "max_abs_d_ln_K_H": float(dln.max()), "max_rel_d_ln_K_H": float(rel.max()),
+    }
+    log(f"config2 mixed: {ins_s:.4e} ins/s; f64 {ninsert / w64:.4e}; max |d ln K_H| {dln.max():.2e}")
     print(json.dumps(out), flush=True)
 
 

@@ -30,7 +30,22 @@ FLAGS = [
 ]
 
 
+SYNTH_OUT = os.path.join(HERE, "pureadsorb_amd", "libpa_synth.so")
+
+
+def build_synth(verbose: bool = False) -> str:
+    """Host-only helper of the synthetic-input generator (gcc, no GPU code)."""
+    tmp = SYNTH_OUT + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", tmp, os.path.join(CSRC, "synth.c"), "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SYNTH_OUT)
+    return SYNTH_OUT
+
+
 def build(verbose: bool = False) -> str:
+    build_synth(verbose)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]

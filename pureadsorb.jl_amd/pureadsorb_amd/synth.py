@@ -16,7 +16,9 @@ jamming density of 1.5 Å random sequential addition) is capped to that value.
 """
 from __future__ import annotations
 
+import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -59,39 +61,36 @@ def _mulhi64(a: int, b: int) -> int:
     return (a * b) >> 64
 
 
+_synth_lib = None
+
+
+def _positions(A, n, seed, min_sep, max_attempts):
+    """Random sequential addition in C (csrc/synth.c), the rule in the module docstring."""
+    global _synth_lib
+    if _synth_lib is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpa_synth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `python pureadsorb.jl_amd/build.py`")
+        L = ctypes.CDLL(path)
+        L.pa_synth_positions.restype = ctypes.c_int
+        L.pa_synth_positions.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_void_p]
+        _synth_lib = L
+    Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.reshape(9))
+    frac = np.zeros((max(n, 1), 3))
+    st = _synth_lib.pa_synth_positions(Acm.ctypes.data, n, seed, float(min_sep), int(max_attempts), frac.ctypes.data)
+    if st != n:
+        raise RuntimeError(f"synthetic_framework: atom {-st - 1} not placed after {max_attempts} attempts")
+    return frac[:n]
+
+
 def synthetic_framework(a, b, c, alpha, beta, gamma, natoms, seed, *, min_sep=1.5, charge_sd=0.0,
                         types=("C", "H", "O", "Zn"), max_density=0.15, max_attempts=100_000) -> Framework:
     """One unit cell of `natoms` random atoms (see the module docstring for the exact rule)."""
     A = cell_matrix(a, b, c, alpha, beta, gamma)
-    invA = np.linalg.inv(A)
     n = int(min(natoms, math.floor(max_density * volume(A))))
     seed = int(seed) & MASK64
-    frac = np.zeros((n, 3))
-    placed = 0
-    batch = 16
-    min2 = float(min_sep) ** 2
-    for i in range(n):
-        k0 = 1
-        done = False
-        while not done:
-            if k0 > max_attempts:
-                raise RuntimeError(f"synthetic_framework: atom {i} not placed after {max_attempts} attempts")
-            ks = np.arange(k0, k0 + batch, dtype=np.uint64)
-            cand = unit_f64(chain_words(np.uint64(seed), np.uint64(i + 1), ks, 3))  # (batch, 3)
-            if placed == 0:
-                frac[i] = cand[0]
-                done = True
-                break
-            d = cand[:, None, :] - frac[None, :placed, :]
-            d -= np.rint(d)
-            cart = d @ A.T
-            r2 = np.einsum("bpk,bpk->bp", cart, cart)
-            ok = np.all(r2 >= min2, axis=1)
-            if ok.any():
-                frac[i] = cand[int(np.argmax(ok))]
-                done = True
-            k0 += batch
-        placed += 1
+    frac = _positions(A, n, seed, min_sep, max_attempts)
     tw = chain_words(np.uint64(seed), np.arange(1, n + 1, dtype=np.uint64), np.uint64(0), 1)[:, 0]
     tidx = [_mulhi64(int(w), len(types)) for w in tw]
     symbols = [types[t] for t in tidx]
@@ -103,7 +102,6 @@ def synthetic_framework(a, b, c, alpha, beta, gamma, natoms, seed, *, min_sep=1.
         q = q - q.mean()
     else:
         q = np.zeros(n)
-    del invA
     return Framework(A, frac, [f"{s}{i + 1}" for i, s in enumerate(symbols)], symbols, q)
 
 

@@ -167,7 +167,7 @@ def test_widom_config1_literal_lse(c1):
 
 @pytest.mark.parametrize("case", ["porous_config1", "rubtak"])
 def test_widom_mixed_parity(case, c1p, rubtak, trappe, co2):
-    """The north-star gate: mixed vs the FP64 oracle on identical streams,
+    """The precision gate: mixed vs the FP64 oracle on identical streams,
     |d ln K_H| <= 1e-4 and |d q_st| <= 0.01 kJ/mol."""
     if case == "porous_config1":
         fw, ff, g, ew, b, o = c1p
@@ -309,6 +309,37 @@ def test_dense_cluster_overflows_the_list():
     w = ref < 10 * pa.KB * 298.15
     assert w.sum() > 10
     assert np.max(np.abs(dm[w] - ref[w])) < 3e-6
+
+
+def test_config2_batch_subset_against_oracle():
+    """Eight of the 256 batched frameworks (uneven sizes, triclinic, widths >= 24 Å) in one batch:
+    FP64 path per system within 1e-9 of the oracle; mixed within the ln K_H / q_st gate where
+    the framework admits binding insertions, and within 1e-6 of |ln K_H| where it does not."""
+    from scipy.special import logsumexp
+    fws = synth.config2_frameworks(256)
+    pick = [0, 17, 42, 77, 128, 180, 211, 255]
+    sub = [fws[i] for i in pick]
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    b = pa.FrameworkBatch(sub, ff, g, ew)
+    ninsert = 8 * 4000
+    rm = pa.widom(b, g, T=298.15, ninsert=ninsert, seed=7, nblocks=4, precision="mixed")
+    r64 = pa.widom(b, g, T=298.15, ninsert=ninsert, seed=7, nblocks=4, precision="f64")
+    ns = pa.system_counts(ninsert, len(sub), pa.default_run(ninsert, len(sub)))
+    kT = pa.KB * 298.15
+    for s, fw in enumerate(sub):
+        o = oracle_for(fw, ff, g, ew)
+        dU = o.energies(7, s, 0, int(ns[s]), nthreads=oracle_threads())
+        ref = np.log(o.V) - np.log(kT) + logsumexp(-dU / kT) - np.log(ns[s])
+        print(f"fw {pick[s]}: {fw.natoms} atoms, ln K_H oracle {ref:.6f} f64 {r64[s].ln_K_H:.6f} mixed {rm[s].ln_K_H:.6f}",
+              flush=True)
+        assert abs(r64[s].ln_K_H - ref) <= 1e-9 * max(1.0, abs(ref))
+        if dU.min() < 10 * kT:
+            assert abs(rm[s].ln_K_H - ref) <= LNKH_TOL
+            assert abs(rm[s].q_st - r64[s].q_st) <= QST_TOL
+        else:
+            assert abs(rm[s].ln_K_H - ref) <= 1e-6 * abs(ref)
 
 
 def test_empty_box_ideal_gas():
