@@ -117,8 +117,9 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=2, help="timed steps of the FP64 plain path (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=4000, help="insertions timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
-    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
-                    help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions")
+    ap.add_argument("--workload", choices=("config1", "config2", "config3"), default="config1",
+                    help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions; "
+                         "config3: gradient cost on 64 frameworks x 2e5 insertions")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -141,6 +142,8 @@ def main():
     log("device:", pa.device_name())
     if args.workload == "config2":
         return bench_config2(args)
+    if args.workload == "config3":
+        return bench_config3(args)
     fw, ff, g, ew = workload()
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
@@ -262,6 +265,50 @@ def bench_config2(args):
         "max_abs_d_ln_K_H": float(dln.max()), "max_rel_d_ln_K_H": float(rel.max()),
     }
     log(f"config2 mixed: {ins_s:.4e} ins/s; f64 {ninsert / w64:.4e}; max |d ln K_H| {dln.max():.2e}")
+    print(json.dumps(out), flush=True)
+
+
+def bench_config3(args):
+    """Gradient cost: the first 64 frameworks of the 256-framework draw, CO2, 2e5 insertions each;
+    wall time of widom_grad (weights, gradient kernel, host assembly) over widom's."""
+    fws = synth.config2_frameworks(256)[:64]
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    b = pa.FrameworkBatch(fws, ff, g, ew)
+    ninsert = 64 * 200_000
+    log(f"config3 batch: {sum(f.natoms for f in fws)} atoms in 64 frameworks")
+    for _ in range(args.warmup):
+        pa.widom(b, g, T=T_K, ninsert=ninsert, seed=42, precision="mixed")
+        pa.widom_grad(b, g, T=T_K, ninsert=ninsert, seed=42, precision="mixed")
+    pa.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pa.widom(b, g, T=T_K, ninsert=ninsert, seed=42, precision="mixed")
+    pa.synchronize()
+    tw = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    gms = 0.0
+    for _ in range(args.steps):
+        gr = pa.widom_grad(b, g, T=T_K, ninsert=ninsert, seed=42, precision="mixed")
+        gms += pa.last_timing(b)["grad_ms"]
+    pa.synchronize()
+    tg = (time.perf_counter() - t0) / args.steps
+    nweighted = pa.last_timing(b)["grad_insertions"]
+    out = {
+        "metric": METRIC + " (gradient cost, 64-framework batch)", "value": ninsert / tg,
+        "unit": "insertions/s with gradients", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tg * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 frameworks (first 64 of the 256-framework draw), CO2 LJ+Ewald rc 12 A, 2e5 "
+                               "insertions each; d ln K_H and d q_st wrt per-type eps, sigma and per-atom charges",
+                   "ninsert_per_step": ninsert},
+        "widom_ms": tw * 1e3, "widom_grad_ms": tg * 1e3, "grad_cost": tg / tw,
+        "grad_kernel_ms": gms / args.steps, "weighted_insertions": nweighted,
+        "charge_entries": int(sum(len(x.dlnkh_dq) for x in gr)),
+    }
+    log(f"config3: widom {tw * 1e3:.1f} ms, widom_grad {tg * 1e3:.1f} ms (cost x{tg / tw:.2f}), grad kernel "
+        f"{gms / args.steps:.1f} ms over {nweighted} weighted insertions")
     print(json.dumps(out), flush=True)
 
 

@@ -102,6 +102,8 @@ typedef struct pa_timing {
     double reduce_ms;             /* Boltzmann-sum reduction kernels */
     int64_t insertions;
     double binning_ms;            /* mixed path: pose, bin count, scan and scatter launches */
+    double grad_ms;               /* pa_widom_grad: accumulator clears and the gradient kernel */
+    int64_t grad_insertions;      /* pa_widom_grad: insertions with a weight above e^-40 of the best */
 } pa_timing;
 
 typedef struct pa_batch pa_batch;
@@ -128,6 +130,25 @@ int pa_batch_framework_info(const pa_batch *batch, int32_t fw, pa_framework_info
 int pa_widom(pa_batch *batch, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
              int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out);
 
+/* Gradients of ln K_H and q_st for every system of the batch, on the same insertion streams as
+ * pa_widom. No counterpart in the reference (it has no derivative path); the quantities are the
+ * reweighting estimators
+ *   d ln K_H / dθ = -β <w ∂U/∂θ> / <w>,
+ *   d q_st / dθ   = -(<w ∂U/∂θ> - β <w U ∂U/∂θ>) / <w> - β (<w U>/<w>) <w ∂U/∂θ> / <w>,
+ * over the force field's LJ parameters (per type epsilon_t and sigma_t, Lorentz-Berthelot
+ * mixing; entries of types absent from the frameworks are 0) and the framework charges.
+ * Charge entries follow the framework atoms as passed; the derivative of atom j moves every
+ * replication image of j together (its `replication` claim stays true), which needs atoms in
+ * `replicate` order (image-major, atoms inner) when the replication is not (1, 1, 1).
+ * Outputs (nullable):
+ *   out[nsys]                                      the pa_widom result of the same run,
+ *   dlnkh_deps, dlnkh_dsig, dqst_deps, dqst_dsig   nsys x ntypes (force-field types), row per system,
+ *   dlnkh_dq, dqst_dq                              sum over systems of their framework's natoms,
+ *                                                  system-major. */
+int pa_widom_grad(pa_batch *batch, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
+                  int32_t run, uint32_t precision, pa_widom_result *out, double *dlnkh_deps, double *dlnkh_dsig,
+                  double *dqst_deps, double *dqst_dsig, double *dlnkh_dq, double *dqst_dq);
+
 /* Delta U = insertion_energy + constant_offset (eV) of insertions j0 .. j0+n-1 of system `sys`,
  * computed on the device (the widom_kernel! contract, src/widom.jl:157-175). */
 int pa_widom_energies(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, int64_t n, uint32_t precision,
@@ -136,7 +157,7 @@ int pa_widom_energies(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, i
 /* Device-drawn poses of the same insertions: pos (n x 3, Å) and quaternions (n x 4, x y z w). */
 int pa_widom_poses(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, int64_t n, double *pos, double *quat);
 
-/* Timing of the last pa_widom call on `batch`. */
+/* Timing of the last pa_widom or pa_widom_grad call on `batch`. */
 int pa_last_timing(const pa_batch *batch, pa_timing *out);
 
 /* Copies the thread-local message of the last failing call (empty after success). Returns the

@@ -953,3 +953,178 @@ int or_widom(const or_system *s, double T, int64_t ns, uint64_t seed, int64_t sy
     free(dU); free(sW); free(sUW); free(n);
     return st;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Reweighting gradients of ln K_H and q_st of one system (no counterpart in the reference,
+ * which has no derivative path). Over insertions 0..ns-1 with w_i = exp(-beta dU_i - m):
+ *   d ln K_H / d theta = -beta G / W,
+ *   d q_st / d theta   = -(G - beta H) / W - beta (R / W) (G / W),
+ * W = sum w, R = sum w dU, G = sum w d dU/d theta, H = sum w dU d dU/d theta.
+ * theta runs over
+ *  - the per-type LJ parameters eps_t, sigma_t under Lorentz-Berthelot mixing
+ *    (eps_xy = sqrt(eps_x eps_y), sigma_xy = (sigma_x + sigma_y)/2, src/forcefield.jl:30-45),
+ *    through every pair term and the tail correction (src/forcefield.jl:103-124);
+ *  - each framework charge q_j: real-space pairs, the reciprocal cross term over the FULL k
+ *    table, and the net-charge term of constant_offset. With a replication claim the entry of
+ *    atom j is the mean over its images (atoms in replicate order, image-major), i.e. the
+ *    derivative along "all images of j change together" divided by the image count.
+ * Per-pair derivatives: d u_LJ/d eps_xy = 4 (x^12 - x^6), d u_LJ/d sigma_xy =
+ * 4 eps (12 x^12 - 6 x^6) / sigma with x = sigma/r; d E/d q_j = KE q_a erfc(alpha r)/r (pair
+ * series) + KE sum_k kpref 2 (cos(k.r_j) Re S_g + sin(k.r_j) Im S_g).
+ * Pinned by central finite differences of or_widom's ln K_H and q_st (tests).
+ * out: ln K_H, q_st. Returns 0, or -1 when no insertion has a finite energy.
+ * ---------------------------------------------------------------------------------------- */
+int or_widom_grad(const or_system *s, double T, int64_t ns, uint64_t seed, int64_t sysidx, int nthreads,
+                  double out[2], double *lnkh_eps, double *lnkh_sig, double *qst_eps, double *qst_sig,
+                  double *lnkh_q, double *qst_q)
+{
+    const double kT = OR_KB * T, beta = 1.0 / kT;
+    const int NT = s->ntypes, na = s->natoms, nkf = s->nkfull;
+    double *dU = malloc(sizeof(double) * (size_t)(ns > 0 ? ns : 1));
+    or_energies(s, seed, sysidx, 0, ns, dU, NULL, NULL, nthreads);
+    double m = -INFINITY;
+    for (int64_t i = 0; i < ns; ++i) if (-beta * dU[i] > m) m = -beta * dU[i];
+    if (!(m > -INFINITY)) { free(dU); return -1; }
+    const size_t nlj = (size_t)NT * NT;
+    /* accumulators: [0..3] per mixed pair (G_eps, H_eps, G_sig, H_sig), then Gq, Hq, then
+     * A_re, A_im, B_re, B_im per full k, then W, R */
+    const size_t nacc = 4 * nlj + 2 * (size_t)na + 4 * (size_t)nkf + 2;
+    double *acc = calloc(nacc, sizeof(double));
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        double *loc = calloc(nacc, sizeof(double));
+        double *Ge = loc, *He = loc + nlj, *Gs = loc + 2 * nlj, *Hs = loc + 3 * nlj;
+        double *Gq = loc + 4 * nlj, *Hq = Gq + na, *Ak = Hq + na, *Wl = Ak + 4 * (size_t)nkf;
+        const double rc_lj2 = s->rc_lj * s->rc_lj, rc_ew2 = s->rc_ew * s->rc_ew;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < ns; ++i) {
+            const double w = exp(-beta * dU[i] - m);
+            if (w == 0.0) continue;
+            const double wu = w * dU[i];
+            Wl[0] += w;
+            Wl[1] += wu;
+            double pos[3], q[4], gp[3 * OR_MAXSITES];
+            or_pose(seed, sysidx, i, s->A, pos, q);
+            for (int a = 0; a < s->nsites; ++a) {
+                double r[3];
+                or_rotate(q, s->gsite + 3 * a, r);
+                for (int d = 0; d < 3; ++d) gp[3 * a + d] = pos[d] + r[d];
+            }
+            for (int j = 0; j < na; ++j) {
+                const double *hp = s->pos + 3 * j;
+                const int ht = s->type[j];
+                for (int a = 0; a < s->nsites; ++a) {
+                    double d[3] = {gp[3 * a] - hp[0], gp[3 * a + 1] - hp[1], gp[3 * a + 2] - hp[2]}, mi[3];
+                    or_minimum_image(s->A, s->invA, d, mi);
+                    const double r2 = or_dot3(mi, mi);
+                    const int gt = s->gtype[a];
+                    const size_t p = (size_t)gt + (size_t)NT * ht;
+                    if (r2 < rc_lj2) {
+                        const double sg = s->sigma[p], ep = s->eps[p];
+                        double x6 = sg * sg / r2;
+                        x6 = x6 * x6 * x6;
+                        const double de = 4.0 * (x6 * x6 - x6);
+                        const double ds = sg > 0.0 ? 4.0 * ep * (12.0 * x6 * x6 - 6.0 * x6) / sg : 0.0;
+                        Ge[p] += w * de; He[p] += wu * de;
+                        Gs[p] += w * ds; Hs[p] += wu * ds;
+                    }
+                    if (s->charged && r2 < rc_ew2) {
+                        const double r = sqrt(r2);
+                        const double f = OR_KE * s->gq[a] * or_pair_erfc_f64(s->alpha * r) / r;
+                        Gq[j] += w * f;
+                        Hq[j] += wu * f;
+                    }
+                }
+            }
+            if (s->charged)
+                for (int k = 0; k < nkf; ++k) {
+                    double re = 0.0, im = 0.0;
+                    for (int a = 0; a < s->nsites; ++a) {
+                        const double ph = or_dot3(s->kfull + 3 * k, gp + 3 * a);
+                        re += s->gq[a] * cos(ph);
+                        im += s->gq[a] * sin(ph);
+                    }
+                    Ak[4 * k] += w * re; Ak[4 * k + 1] += w * im;
+                    Ak[4 * k + 2] += wu * re; Ak[4 * k + 3] += wu * im;
+                }
+        }
+#pragma omp critical
+        for (size_t t = 0; t < nacc; ++t) acc[t] += loc[t];
+        free(loc);
+    }
+    double *Ge = acc, *He = acc + nlj, *Gs = acc + 2 * nlj, *Hs = acc + 3 * nlj;
+    double *Gq = acc + 4 * nlj, *Hq = Gq + na, *Ak = Hq + na;
+    const double W = Ak[4 * (size_t)nkf], R = Ak[4 * (size_t)nkf + 1];
+    /* tail correction: every insertion carries d tail / d (eps_xy, sigma_xy) */
+    if (s->tail) {
+        int64_t *counts = calloc((size_t)NT, sizeof(int64_t)), *gcounts = calloc((size_t)NT, sizeof(int64_t));
+        for (int j = 0; j < na; ++j) counts[s->type[j]] += 1;
+        for (int a = 0; a < s->nsites; ++a) gcounts[s->gtype[a]] += 1;
+        const double pre = (8.0 * OR_PI / 3.0) / s->V, rc = s->rc_lj;
+        for (int x = 0; x < NT; ++x)
+            for (int y = 0; y < NT; ++y) {
+                const double mult = 2.0 * (double)counts[x] * (double)gcounts[y] + (double)gcounts[x] * (double)gcounts[y];
+                const size_t p = (size_t)x + (size_t)NT * y;
+                const double sg = s->sigma[p], ep = s->eps[p], u = sg / rc, u3 = u * u * u;
+                const double dce = sg * sg * sg * (u3 * u3 * u3 / 3.0 - u3);
+                const double dcs = ep * (12.0 * pow(sg, 11) / (3.0 * pow(rc, 9)) - 6.0 * pow(sg, 5) / pow(rc, 3));
+                Ge[p] += W * pre * mult * dce; He[p] += R * pre * mult * dce;
+                Gs[p] += W * pre * mult * dcs; Hs[p] += R * pre * mult * dcs;
+            }
+        free(counts); free(gcounts);
+    }
+    /* Lorentz-Berthelot chain rule */
+    for (int t = 0; t < NT; ++t) {
+        double ge = 0.0, he = 0.0, gs = 0.0, hs = 0.0;
+        const double et = s->eps[t + (size_t)NT * t];
+        for (int x = 0; x < NT; ++x)
+            for (int y = 0; y < NT; ++y) {
+                const int c = (x == t) + (y == t);
+                if (!c) continue;
+                const size_t p = (size_t)x + (size_t)NT * y;
+                const double fe = s->eps[p] / (2.0 * et) * c, fs = 0.5 * c;
+                if (Ge[p] != 0.0 || He[p] != 0.0) { ge += fe * Ge[p]; he += fe * He[p]; }
+                gs += fs * Gs[p]; hs += fs * Hs[p];
+            }
+        lnkh_eps[t] = -beta * ge / W;
+        lnkh_sig[t] = -beta * gs / W;
+        qst_eps[t] = -(ge - beta * he) / W - beta * (R / W) * (ge / W);
+        qst_sig[t] = -(gs - beta * hs) / W - beta * (R / W) * (gs / W);
+    }
+    /* charges: full-k reciprocal per atom, net-charge term, mean over replication images */
+    double Qg = 0.0;
+    for (int a = 0; a < s->nsites; ++a) Qg += s->gq[a];
+    const double cnet = s->charged ? -OR_KE * OR_PI * Qg / (s->V * s->alpha * s->alpha) : 0.0;
+    for (int j = 0; j < na; ++j) {
+        double gw = 0.0, gu = 0.0;
+        if (s->charged)
+            for (int k = 0; k < nkf; ++k) {
+                const double ph = or_dot3(s->kfull + 3 * k, s->pos + 3 * j), c = cos(ph), sn = sin(ph);
+                gw += 2.0 * s->kpref_full[k] * (c * Ak[4 * k] + sn * Ak[4 * k + 1]);
+                gu += 2.0 * s->kpref_full[k] * (c * Ak[4 * k + 2] + sn * Ak[4 * k + 3]);
+            }
+        Gq[j] += OR_KE * gw + W * cnet;
+        Hq[j] += OR_KE * gu + R * cnet;
+    }
+    const int nimg = s->rep[0] * s->rep[1] * s->rep[2];
+    if (nimg > 1 && na % nimg == 0) {
+        const int nu = na / nimg;
+        for (int a = 0; a < nu; ++a) {
+            double sg = 0.0, sh = 0.0;
+            for (int im = 0; im < nimg; ++im) { sg += Gq[im * nu + a]; sh += Hq[im * nu + a]; }
+            for (int im = 0; im < nimg; ++im) { Gq[im * nu + a] = sg / nimg; Hq[im * nu + a] = sh / nimg; }
+        }
+    }
+    for (int j = 0; j < na; ++j) {
+        lnkh_q[j] = -beta * Gq[j] / W;
+        qst_q[j] = -(Gq[j] - beta * Hq[j]) / W - beta * (R / W) * (Gq[j] / W);
+    }
+    out[0] = log(s->V) - log(kT) + log(W / (double)ns) + m;
+    out[1] = kT - R / W;
+    free(acc);
+    free(dU);
+    return 0;
+}

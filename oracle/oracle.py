@@ -119,6 +119,9 @@ def lib():
     L.or_reduce.argtypes = [C.c_int, _dp, _dp, _lp, C.c_double, C.c_double, _dp]
     L.or_widom.restype = C.c_int
     L.or_widom.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, C.c_int]
+    L.or_widom_grad.restype = C.c_int
+    L.or_widom_grad.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp,
+                                _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -288,6 +291,7 @@ class OracleSystem:
         sigma = np.asarray(sigma, dtype=np.float64)
         epsilon = np.asarray(epsilon, dtype=np.float64)
         nt = sigma.shape[0]
+        self.ntypes = nt
         sig = np.ascontiguousarray(sigma.T.reshape(-1))
         eps = np.ascontiguousarray(epsilon.T.reshape(-1))
         gs = np.ascontiguousarray(guest_sites, dtype=np.float64).reshape(-1)
@@ -360,3 +364,18 @@ class OracleSystem:
         out = np.zeros(8)
         st = lib().or_widom(self._h, T, ns, seed & (2**64 - 1), sysidx, nblocks, bs, out, nthreads)
         return st, out, bs.reshape(nblocks, 3)
+
+    def widom_grad(self, T, ns, seed, sysidx, nthreads=0):
+        """FP64 reweighting gradients of ln K_H and q_st over insertions 0..ns-1 (or_widom_grad).
+
+        Returns a dict: ln_K_H, q_st, dlnkh_deps, dlnkh_dsig, dqst_deps, dqst_dsig (per force-field
+        type) and dlnkh_dq, dqst_dq (per framework atom, mean over replication images)."""
+        nt, na = self.ntypes, self.natoms
+        out = np.zeros(2)
+        lj = [np.zeros(max(nt, 1)) for _ in range(4)]
+        dq = [np.zeros(max(na, 1)) for _ in range(2)]
+        st = lib().or_widom_grad(self._h, T, ns, seed & (2**64 - 1), sysidx, nthreads, out, *lj, *dq)
+        if st != 0:
+            raise RuntimeError(f"or_widom_grad: status {st}")
+        return {"ln_K_H": out[0], "q_st": out[1], "dlnkh_deps": lj[0][:nt], "dlnkh_dsig": lj[1][:nt],
+                "dqst_deps": lj[2][:nt], "dqst_dsig": lj[3][:nt], "dlnkh_dq": dq[0][:na], "dqst_dq": dq[1][:na]}

@@ -99,4 +99,29 @@ struct BinnedArgs {
     GuestDev guest;
 };
 
+// Gradient launch: one weighted insertion per wavefront (widom_grad.hip).
+struct GradArgs {
+    const FwDesc *fw;
+    const int *fw_of;
+    const int64_t *seg;
+    int nsys;
+    uint64_t seed;
+    const int64_t *list_g;      // flattened insertion index of each weighted insertion
+    const double *list_w;       // its weight w
+    const double *list_wu;      // w * Delta U
+    int64_t nlist;
+    const uint4 *atomfx;
+    const int *atype;
+    const int4 *kn;
+    const float4 *pairg;        // [site][type]: sigma^2, 4 epsilon, 1/sigma, 0
+    int ntypes_c;
+    float rc_lj2f, rc_ew2f, rc_max2f;
+    GuestDev guest;
+    double *acc_lj;             // nsys x NS x ntc x 4: sum w dU/deps, sum wU dU/deps, sum w dU/dsig, sum wU dU/dsig
+    double *acc_q;              // per system slice of 2 x natoms: sum w dU/dq_j, sum wU dU/dq_j
+    const int64_t *qoff;        // nsys: first atom slot of the system in acc_q
+    double *acc_k;              // per system slice of 4 x nk: w Re S_g, w Im S_g, wU Re S_g, wU Im S_g
+    const int64_t *koff;        // nsys: first k slot of the system in acc_k
+};
+
 }  // namespace pa

@@ -26,6 +26,7 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
 size_t binned_lds_bytes_host(int cap, int ns);
+hipError_t launch_grad(const GradArgs &a, bool upper, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -277,6 +278,16 @@ struct FwHost {
     double perp[3];  // perpendicular widths, Å
 };
 
+// Host copies of one framework's tables that the gradient entry point reads back.
+struct FwGradHost {
+    std::vector<double> frac;    // natoms x 3
+    std::vector<int> kn;         // 3 per kept k-vector: reciprocal-basis coefficients
+    std::vector<double> kpref;   // per kept k-vector
+    std::vector<double> counts;  // atoms per force-field type
+    int rep[3] = {1, 1, 1};
+    bool image_order = true;     // atoms are `replicate` images of the first natoms/nimg (image-major)
+};
+
 struct pa_batch {
     int nsys = 0, nfw = 0;
     std::vector<int> fw_of;
@@ -291,12 +302,20 @@ struct pa_batch {
     int ntypes_c = 0;
     double rc_lj = 0, rc_ew = 0;
     double r_guest = 0;  // largest site distance from the guest's reference point
+    // force field as given, compact type map and per-framework host tables (gradients)
+    int nt_ff = 0, ff_tail = 0;
+    std::vector<double> ff_sigma, ff_eps;
+    std::vector<int> c_to_orig;
+    std::vector<double2> pair64_h;
+    std::vector<FwGradHost> fgh;
     // device tables
     DevAlloc d_fw, d_fw_of, d_atom64, d_atype, d_k64, d_S64, d_atomfx, d_kn, d_kc, d_pair64, d_pair32;
     // per-call scratch
     DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
     // mixed-path binning scratch
     DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_cursor, d_order, d_scan_tmp;
+    // gradient scratch
+    DevAlloc d_list_g, d_list_w, d_list_wu, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch
@@ -447,6 +466,11 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         if (present[t]) { orig_to_c[t] = (int)c_to_orig.size(); c_to_orig.push_back(t); }
     const int NTC = (int)c_to_orig.size();
     b->ntypes_c = NTC;
+    b->c_to_orig = c_to_orig;
+    b->nt_ff = NT;
+    b->ff_tail = ff->tail ? 1 : 0;
+    b->ff_sigma.assign(ff->sigma, ff->sigma + (size_t)NT * NT);
+    b->ff_eps.assign(ff->epsilon, ff->epsilon + (size_t)NT * NT);
     if (NTC > 64) return fail(PA_ERR_ARG, "%d distinct LJ types in the batch; at most 64 are supported", NTC);
 
     // ---- Ewald parameters (src/batch.jl:289-297)
@@ -472,6 +496,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<float2> kc;
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
+    b->fgh.resize(b->nfw);
     for (int u = 0; u < b->nfw; ++u) {
         const pa_framework &F = fws[uniq[u]];
         const int n = F.natoms;
@@ -509,6 +534,31 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
+        FwGradHost &gh = b->fgh[u];
+        gh.frac = frac;
+        gh.counts.assign(NT, 0.0);
+        for (int j = 0; j < n; ++j) gh.counts[F.types[j]] += 1.0;
+        for (int d = 0; d < 3; ++d) gh.rep[d] = F.replication[d];
+        if (!rep1) {
+            // replicate order (src/structure.jl:79-92): image (i, j, k), i fastest, atoms inner;
+            // image m of atom a sits at frac[a] + (i/r0, j/r1, k/r2) modulo 1
+            const int r0 = F.replication[0], r1 = F.replication[1], r2 = F.replication[2];
+            const int nimg = r0 * r1 * r2;
+            gh.image_order = n % nimg == 0;
+            const int nu = gh.image_order ? n / nimg : 0;
+            for (int m = 1; gh.image_order && m < nimg; ++m) {
+                const double sh[3] = {(double)(m % r0) / r0, (double)((m / r0) % r1) / r1, (double)(m / (r0 * r1)) / r2};
+                for (int a = 0; gh.image_order && a < nu; ++a) {
+                    const int j = m * nu + a;
+                    for (int d = 0; d < 3; ++d) {
+                        double x = frac[3 * j + d] - frac[3 * a + d] - sh[d];
+                        x -= std::nearbyint(x);
+                        if (std::fabs(x) > 1.0e-9) gh.image_order = false;
+                    }
+                    if (q[j] != q[a] || F.types[j] != F.types[a]) gh.image_order = false;
+                }
+            }
+        }
         KTable kt;
         double self_mean = 0.0, halfrange = 0.0;
         D.k_off = (int)k64.size();
@@ -552,6 +602,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
                 k64.push_back(make_double4(kt.k[3 * i], kt.k[3 * i + 1], kt.k[3 * i + 2], kt.kpref[i]));
                 S64.push_back(make_double2(kt.Sre[i], kt.Sim[i]));
                 kn.push_back(make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0));
+                gh.kn.insert(gh.kn.end(), &kt.n[3 * i], &kt.n[3 * i] + 3);
+                gh.kpref.push_back(kt.kpref[i]);
                 kc.push_back(make_float2((float)(2.0 * kt.kpref[i] * kt.Sre[i]), (float)(2.0 * kt.kpref[i] * kt.Sim[i])));
             }
             // orientation-averaged guest self term over the FULL k table (src/batch.jl:360-389)
@@ -635,6 +687,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             pair64[(size_t)a * NTC + t] = make_double2(sg, ep);
             pair32[(size_t)a * NTC + t] = make_float2((float)(sg * sg), (float)(4.0 * ep));
         }
+
+    b->pair64_h = pair64;
 
     if (flags & PA_BATCH_HOST_ONLY) {
         b->host_only = true;
@@ -880,8 +934,10 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     return PA_OK;
 }
 
-extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
-                        int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out)
+// pa_widom's body; also returns the flattened system offsets of the run (the energies stay in
+// b->d_dU for a following gradient pass).
+static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
+                      int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out, std::vector<int64_t> *seg_out)
 {
     if (!b || !guest || !out) return fail(PA_ERR_ARG, "NULL argument");
     // argument checks in the reference's order and wording (src/widom.jl:302-330)
@@ -968,7 +1024,7 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
         HIP_TRY(hipEventElapsedTime(&ms, b->kev[i], b->kev[i + 1]));
         kernel_ms += ms;
     }
-    b->last = pa_timing{e02, kernel_ms, launches, e12, G, e01 - kernel_ms};
+    b->last = pa_timing{e02, kernel_ms, launches, e12, G, e01 - kernel_ms, 0.0, 0};
 
     // combine tiles per (system, block) in order, then the reference's block statistics
     std::vector<LseH> blk((size_t)nsys * nblocks, LseH{-INFINITY, 0.0, 0.0});
@@ -1042,6 +1098,293 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
         r.nsamples = (int64_t)ntot;
         r.nblocks = nblocks;
         r.ln_K_H = std::log(h.volume) + lnW - std::log(kT);
+    }
+    if (seg_out) *seg_out = seg;
+    return ok();
+}
+
+extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
+                        int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out)
+{
+    return widom_core(b, guest, T, ninsert, seed, nblocks, run, chunk, precision, out, nullptr);
+}
+
+// Reciprocal part of the per-atom charge derivative over one system's kept k-vectors:
+// dE/dq_j = KE sum_k kpref_k 2 (cos(k.r_j) A_re(k) + sin(k.r_j) A_im(k)), with k.r_j = 2 pi n.f_j
+// from per-axis phase tables (n_1 >= 0 by the half-space enumeration, src/ewald.jl:142-157).
+// acck: 4 per k (w Re S_g, w Im S_g, wU Re S_g, wU Im S_g); adds into gq[2 j], gq[2 j + 1].
+static void reciprocal_dq(const FwGradHost &gh, const double *acck, double *gq)
+{
+    const int64_t nk = (int64_t)gh.kpref.size(), na = (int64_t)gh.frac.size() / 3;
+    if (nk == 0 || na == 0) return;
+    int nmax[3] = {0, 0, 0};
+    for (int64_t i = 0; i < nk; ++i)
+        for (int d = 0; d < 3; ++d) nmax[d] = std::max(nmax[d], std::abs(gh.kn[3 * i + d]));
+    const int w0 = nmax[0] + 1, w1 = 2 * nmax[1] + 1, w2 = 2 * nmax[2] + 1;
+    parallel_for(na, [&](int64_t b0, int64_t e0) {
+        std::vector<double> c0(w0), s0(w0), c1(w1), s1(w1), c2(w2), s2(w2);
+        for (int64_t j = b0; j < e0; ++j) {
+            for (int m = 0; m < w0; ++m) {
+                const double ph = TWO_PI * (m * gh.frac[3 * j]);
+                c0[m] = std::cos(ph); s0[m] = std::sin(ph);
+            }
+            for (int m = 0; m < w1; ++m) {
+                const double ph = TWO_PI * ((m - nmax[1]) * gh.frac[3 * j + 1]);
+                c1[m] = std::cos(ph); s1[m] = std::sin(ph);
+            }
+            for (int m = 0; m < w2; ++m) {
+                const double ph = TWO_PI * ((m - nmax[2]) * gh.frac[3 * j + 2]);
+                c2[m] = std::cos(ph); s2[m] = std::sin(ph);
+            }
+            double gw = 0.0, gu = 0.0;
+            for (int64_t i = 0; i < nk; ++i) {
+                const int m0 = gh.kn[3 * i], m1 = gh.kn[3 * i + 1] + nmax[1], m2 = gh.kn[3 * i + 2] + nmax[2];
+                const double pr = c0[m0] * c1[m1] - s0[m0] * s1[m1], pi = c0[m0] * s1[m1] + s0[m0] * c1[m1];
+                const double cr = pr * c2[m2] - pi * s2[m2], ci = pr * s2[m2] + pi * c2[m2];
+                const double f = 2.0 * gh.kpref[i];
+                gw += f * (cr * acck[4 * i] + ci * acck[4 * i + 1]);
+                gu += f * (cr * acck[4 * i + 2] + ci * acck[4 * i + 3]);
+            }
+            gq[2 * j] += KE * gw;
+            gq[2 * j + 1] += KE * gu;
+        }
+    });
+}
+
+extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed,
+                             int32_t nblocks, int32_t run, uint32_t precision, pa_widom_result *out, double *dlnkh_deps,
+                             double *dlnkh_dsig, double *dqst_deps, double *dqst_dsig, double *dlnkh_dq, double *dqst_dq)
+{
+    if (!b || !guest) return fail(PA_ERR_ARG, "NULL argument");
+    const int nsys = b->nsys, NT = b->nt_ff, NS = b->nsites, NTC = b->ntypes_c;
+    const bool want_lj = dlnkh_deps || dlnkh_dsig || dqst_deps || dqst_dsig;
+    const bool want_q = dlnkh_dq || dqst_dq;
+    if (want_lj) {
+        // the per-type derivatives go through Lorentz-Berthelot mixing (src/forcefield.jl:30-45)
+        for (int x = 0; x < NT; ++x)
+            for (int y = 0; y < NT; ++y) {
+                const double sx = b->ff_sigma[x + NT * x], sy = b->ff_sigma[y + NT * y];
+                const double ex = b->ff_eps[x + NT * x], ey = b->ff_eps[y + NT * y];
+                const double sxy = b->ff_sigma[x + NT * y], exy = b->ff_eps[x + NT * y];
+                if (std::fabs(sxy - 0.5 * (sx + sy)) > 1e-12 * std::max(1.0, std::fabs(sxy)) ||
+                    std::fabs(exy - std::sqrt(ex * ey)) > 1e-12 * std::max(1e-300, std::fabs(exy)))
+                    return fail(PA_ERR_ARG,
+                                "force-field types %d and %d are not Lorentz-Berthelot mixed (sigma %g, epsilon %g); the "
+                                "per-type LJ gradients need sigma_xy = (sigma_x + sigma_y)/2 and epsilon_xy = "
+                                "sqrt(epsilon_x epsilon_y)",
+                                x, y, sxy, exy);
+            }
+    }
+    if (want_q)
+        for (int s = 0; s < nsys; ++s) {
+            const FwGradHost &gh = b->fgh[b->fw_of[s]];
+            if (!gh.image_order)
+                return fail(PA_ERR_ARG,
+                            "system %d: charge gradients of a framework with replication (%d, %d, %d) need its atoms in "
+                            "replicate order (image-major, i fastest, atoms inner)",
+                            s + 1, gh.rep[0], gh.rep[1], gh.rep[2]);
+        }
+    std::vector<pa_widom_result> res_local;
+    if (!out) {
+        res_local.resize(nsys);
+        out = res_local.data();
+    }
+    std::vector<int64_t> seg;
+    int st = widom_core(b, guest, T, ninsert, seed, nblocks, run, 0, precision, out, &seg);
+    if (st) return st;
+    const double kT = KB * T, beta = 1.0 / kT;
+    const int64_t G = seg[nsys];
+    struct EvPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EvPair()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } evp;
+    HIP_TRY(hipEventCreate(&evp.a));
+    HIP_TRY(hipEventCreate(&evp.b));
+    hipEvent_t g0 = evp.a, g1 = evp.b;
+    std::vector<double> dU(G);
+    HIP_TRY(hipMemcpy(dU.data(), b->d_dU.p, sizeof(double) * G, hipMemcpyDeviceToHost));
+
+    // weighted insertions: w = exp(-beta U - m_s) relative to the system's best insertion;
+    // weights below e^-40 change no sum at FP64 resolution
+    std::vector<int64_t> lg;
+    std::vector<double> lw, lwu, Wsum(nsys, 0.0), Rsum(nsys, 0.0);
+    for (int s = 0; s < nsys; ++s) {
+        double m = -INFINITY;
+        for (int64_t g = seg[s]; g < seg[s + 1]; ++g) m = std::max(m, -beta * dU[g]);
+        if (!(m > -INFINITY)) continue;
+        for (int64_t g = seg[s]; g < seg[s + 1]; ++g) {
+            const double x = -beta * dU[g] - m;
+            if (!(x > -40.0)) continue;
+            const double w = std::exp(x);
+            lg.push_back(g);
+            lw.push_back(w);
+            lwu.push_back(w * dU[g]);
+            Wsum[s] += w;
+            Rsum[s] += w * dU[g];
+        }
+    }
+    const int64_t nlist = (int64_t)lg.size();
+    std::vector<int64_t> qoff(nsys + 1, 0), koff(nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        qoff[s + 1] = qoff[s] + b->info[b->fw_of[s]].natoms;
+        koff[s + 1] = koff[s] + b->info[b->fw_of[s]].nk;
+    }
+    std::vector<float4> pairg((size_t)NS * NTC);
+    for (size_t i = 0; i < pairg.size(); ++i) {
+        const double sg = b->pair64_h[i].x, ep = b->pair64_h[i].y;
+        pairg[i] = make_float4((float)(sg * sg), (float)(4.0 * ep), sg > 0 ? (float)(1.0 / sg) : 0.0f, 0.0f);
+    }
+    const size_t nlj = (size_t)nsys * NS * NTC * 4;
+    const bool charged = b->gdev.charged != 0;
+    HIP_TRY(upload(b->d_list_g, lg));
+    HIP_TRY(upload(b->d_list_w, lw));
+    HIP_TRY(upload(b->d_list_wu, lwu));
+    HIP_TRY(upload(b->d_qoff, qoff));
+    HIP_TRY(upload(b->d_koff, koff));
+    HIP_TRY(upload(b->d_pairg, pairg));
+    HIP_TRY(b->d_acc_lj.ensure(sizeof(double) * nlj));
+    HIP_TRY(b->d_acc_q.ensure(sizeof(double) * std::max<int64_t>(1, 2 * qoff[nsys])));
+    HIP_TRY(b->d_acc_k.ensure(sizeof(double) * std::max<int64_t>(1, 4 * koff[nsys])));
+    HIP_TRY(hipEventRecord(g0, b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_acc_lj.p, 0, sizeof(double) * nlj, b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_acc_q.p, 0, sizeof(double) * std::max<int64_t>(1, 2 * qoff[nsys]), b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_acc_k.p, 0, sizeof(double) * std::max<int64_t>(1, 4 * koff[nsys]), b->stream));
+    GradArgs A{};
+    A.fw = b->d_fw.as<FwDesc>();
+    A.fw_of = b->d_fw_of.as<int>();
+    A.seg = b->d_seg.as<int64_t>();
+    A.nsys = nsys;
+    A.seed = seed;
+    A.list_g = b->d_list_g.as<int64_t>();
+    A.list_w = b->d_list_w.as<double>();
+    A.list_wu = b->d_list_wu.as<double>();
+    A.nlist = nlist;
+    A.atomfx = b->d_atomfx.as<uint4>();
+    A.atype = b->d_atype.as<int>();
+    A.kn = b->d_kn.as<int4>();
+    A.pairg = b->d_pairg.as<float4>();
+    A.ntypes_c = NTC;
+    A.rc_lj2f = (float)(b->rc_lj * b->rc_lj);
+    A.rc_ew2f = charged ? (float)(b->rc_ew * b->rc_ew) : 0.0f;
+    A.rc_max2f = std::max(A.rc_lj2f, A.rc_ew2f);
+    A.guest = b->gdev;
+    A.acc_lj = b->d_acc_lj.as<double>();
+    A.acc_q = b->d_acc_q.as<double>();
+    A.qoff = b->d_qoff.as<int64_t>();
+    A.acc_k = b->d_acc_k.as<double>();
+    A.koff = b->d_koff.as<int64_t>();
+    HIP_TRY(launch_grad(A, b->upper_all, b->stream));
+    HIP_TRY(hipEventRecord(g1, b->stream));
+    std::vector<double> acc_lj(nlj), acc_q(2 * qoff[nsys]), acc_k(4 * koff[nsys]);
+    HIP_TRY(hipMemcpyAsync(acc_lj.data(), b->d_acc_lj.p, sizeof(double) * nlj, hipMemcpyDeviceToHost, b->stream));
+    if (!acc_q.empty())
+        HIP_TRY(hipMemcpyAsync(acc_q.data(), b->d_acc_q.p, sizeof(double) * acc_q.size(), hipMemcpyDeviceToHost,
+                               b->stream));
+    if (!acc_k.empty())
+        HIP_TRY(hipMemcpyAsync(acc_k.data(), b->d_acc_k.p, sizeof(double) * acc_k.size(), hipMemcpyDeviceToHost,
+                               b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float gms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&gms, g0, g1));
+    b->last.grad_ms = gms;
+    b->last.grad_insertions = nlist;
+
+    // per-type chain rule through Lorentz-Berthelot mixing:
+    //   d eps_xy / d eps_t = eps_xy / (2 eps_t) ([x == t] + [y == t]),
+    //   d sigma_xy / d sigma_t = ([x == t] + [y == t]) / 2
+    auto mix_eps = [&](int x, int y, int t) -> double {
+        const int c = (x == t) + (y == t);
+        if (!c) return 0.0;
+        const double et = b->ff_eps[t + NT * t];
+        return et == 0.0 ? NAN : b->ff_eps[x + NT * y] / (2.0 * et) * c;
+    };
+    auto mix_sig = [&](int x, int y, int t) -> double { return 0.5 * ((x == t) + (y == t)); };
+    for (int s = 0; s < nsys; ++s) {
+        const int u = b->fw_of[s];
+        const FwGradHost &gh = b->fgh[u];
+        const double W = Wsum[s], R = Rsum[s], V = b->info[u].volume;
+        auto dlnk = [&](double Gs) { return -beta * Gs / W; };
+        auto dqst = [&](double Gs, double Hs) { return -(Gs - beta * Hs) / W - beta * (R / W) * (Gs / W); };
+        if (want_lj) {
+            std::vector<double> Ge(NT, 0.0), He(NT, 0.0), Gs(NT, 0.0), Hs(NT, 0.0);
+            const double *al = acc_lj.data() + (size_t)s * NS * NTC * 4;
+            for (int a = 0; a < NS; ++a)
+                for (int c = 0; c < NTC; ++c) {
+                    const double *x = al + 4 * ((size_t)a * NTC + c);
+                    if (x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0 && x[3] == 0.0) continue;
+                    const int gx = b->gtypes[a], hy = b->c_to_orig[c];
+                    for (int t = 0; t < NT; ++t) {
+                        const double fe = mix_eps(gx, hy, t), fs = mix_sig(gx, hy, t);
+                        if (fe != 0.0) { Ge[t] += fe * x[0]; He[t] += fe * x[1]; }
+                        if (fs != 0.0) { Gs[t] += fs * x[2]; Hs[t] += fs * x[3]; }
+                    }
+                }
+            if (b->ff_tail) {
+                // tail_delta (src/forcefield.jl:103-124): (8 pi / 3V) sum_xy (2 n_x g_y + g_x g_y) c_xy with
+                // c = eps (sigma^12 / (3 rc^9) - sigma^6 / rc^3); every insertion carries it
+                std::vector<double> gcount(NT, 0.0);
+                for (int a = 0; a < NS; ++a) gcount[b->gtypes[a]] += 1.0;
+                const double rc = b->rc_lj, rc3 = rc * rc * rc, rc9 = rc3 * rc3 * rc3, pre = (8.0 * PI / 3.0) / V;
+                for (int x = 0; x < NT; ++x)
+                    for (int y = 0; y < NT; ++y) {
+                        const double mult = 2.0 * gh.counts[x] * gcount[y] + gcount[x] * gcount[y];
+                        if (mult == 0.0) continue;
+                        const double sg = b->ff_sigma[x + NT * y], ep = b->ff_eps[x + NT * y];
+                        const double s3 = sg * sg * sg, s6 = s3 * s3, s5 = s6 / sg, s11 = s6 * s5;
+                        const double dce = s6 * s6 / (3.0 * rc9) - s6 / rc3;
+                        const double dcs = ep * (4.0 * s11 / rc9 - 6.0 * s5 / rc3);
+                        for (int t = 0; t < NT; ++t) {
+                            const double fe = mix_eps(x, y, t), fs = mix_sig(x, y, t);
+                            if (fe != 0.0 && ep != 0.0) {
+                                Ge[t] += W * pre * mult * dce * fe;
+                                He[t] += R * pre * mult * dce * fe;
+                            } else if (fe != 0.0 && std::isnan(fe)) {
+                                Ge[t] = He[t] = NAN;
+                            }
+                            if (fs != 0.0) {
+                                Gs[t] += W * pre * mult * dcs * fs;
+                                Hs[t] += R * pre * mult * dcs * fs;
+                            }
+                        }
+                    }
+            }
+            for (int t = 0; t < NT; ++t) {
+                const size_t o = (size_t)s * NT + t;
+                if (dlnkh_deps) dlnkh_deps[o] = dlnk(Ge[t]);
+                if (dlnkh_dsig) dlnkh_dsig[o] = dlnk(Gs[t]);
+                if (dqst_deps) dqst_deps[o] = dqst(Ge[t], He[t]);
+                if (dqst_dsig) dqst_dsig[o] = dqst(Gs[t], Hs[t]);
+            }
+        }
+        if (want_q) {
+            const int n = b->info[u].natoms;
+            double *gq = acc_q.data() + 2 * qoff[s];
+            // the real-space part of atom j moves with its replication images: average over them
+            const int nimg = gh.rep[0] * gh.rep[1] * gh.rep[2];
+            if (nimg > 1 && n > 0) {
+                const int nu = n / nimg;
+                for (int a = 0; a < nu; ++a) {
+                    double sw = 0.0, su = 0.0;
+                    for (int m = 0; m < nimg; ++m) { sw += gq[2 * (m * nu + a)]; su += gq[2 * (m * nu + a) + 1]; }
+                    for (int m = 0; m < nimg; ++m) { gq[2 * (m * nu + a)] = sw / nimg; gq[2 * (m * nu + a) + 1] = su / nimg; }
+                }
+            }
+            if (charged) reciprocal_dq(gh, acc_k.data() + 4 * koff[s], gq);
+            // net-charge term of constant_offset: d/dq_j of -KE pi/(2 V alpha^2) ((Q_h + Q_g)^2 - Q_h^2)
+            double Qg = 0.0;
+            for (int a = 0; a < NS; ++a) Qg += b->gq[a];
+            const double alpha = b->info[u].alpha;
+            const double cnet = charged ? -KE * PI * Qg / (V * alpha * alpha) : 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double Gj = gq[2 * j] + W * cnet, Hj = gq[2 * j + 1] + R * cnet;
+                if (dlnkh_dq) dlnkh_dq[qoff[s] + j] = dlnk(Gj);
+                if (dqst_dq) dqst_dq[qoff[s] + j] = dqst(Gj, Hj);
+            }
+        }
     }
     return ok();
 }

@@ -26,7 +26,7 @@ PA_BATCH_HOST_ONLY = 2
 # Every symbol include/pureadsorb_amd.h declares.
 EXPORTS = (
     "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
-    "pa_batch_framework_info", "pa_widom", "pa_widom_energies", "pa_widom_poses", "pa_last_timing",
+    "pa_batch_framework_info", "pa_widom", "pa_widom_grad", "pa_widom_energies", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
 )
 
@@ -99,6 +99,7 @@ class PaTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
         ("reduce_ms", C.c_double), ("insertions", C.c_int64), ("binning_ms", C.c_double),
+        ("grad_ms", C.c_double), ("grad_insertions", C.c_int64),
     ]
 
 
@@ -130,6 +131,9 @@ def lib():
     L.pa_widom.restype = C.c_int
     L.pa_widom.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_int64,
                            C.c_uint32, vp]
+    L.pa_widom_grad.restype = C.c_int
+    L.pa_widom_grad.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32,
+                                C.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     L.pa_widom_energies.restype = C.c_int
     L.pa_widom_energies.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int64, C.c_int64, C.c_uint32, vp]
     L.pa_widom_poses.restype = C.c_int

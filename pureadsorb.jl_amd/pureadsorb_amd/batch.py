@@ -42,6 +42,8 @@ class FrameworkBatch:
         self.nsys = len(fws)
         self.fullk = bool(fullk)
         self.cutoff = float(ff.cutoff)
+        self.ntypes = len(ff.names)
+        self.type_names = list(ff.names)
         self.ewald_cutoff = float(ewald.cutoff)
         self.guest_sites_orig = np.array(guest.sites, dtype=np.float64)
         self.guest_types_orig = np.array(guest.types, dtype=np.int32)

@@ -87,6 +87,59 @@ def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str 
                         r.ln_K_H) for r in out]
 
 
+@dataclass(frozen=True)
+class WidomGrad:
+    """Gradients of ln K_H (per eV or per Å or per e) and q_st (eV per eV, Å or e) of one system.
+
+    d*_deps / d*_dsig are indexed by force-field type (the per-type epsilon_t and sigma_t that
+    Lorentz-Berthelot mixing combines); d*_dq by framework atom, each entry the derivative
+    with respect to that atom's charge with its replication images moving with it, divided
+    by the number of images (so a chain rule through `replicate` sums them back)."""
+    result: WidomResult
+    dlnkh_deps: np.ndarray
+    dlnkh_dsig: np.ndarray
+    dqst_deps: np.ndarray
+    dqst_dsig: np.ndarray
+    dlnkh_dq: np.ndarray
+    dqst_dq: np.ndarray
+
+
+def widom_grad(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, seed: int = 0, nblocks: int = 10,
+               run: int | None = None, precision: str = "mixed") -> list[WidomGrad]:
+    """`widom` plus the reweighting gradients of ln K_H and q_st over the force field's LJ
+    parameters and the framework charges, from the same insertions:
+
+        d ln K_H / dθ = -β <w ∂U/∂θ> / <w>
+        d q_st / dθ   = -(<w ∂U/∂θ> - β <w U ∂U/∂θ>) / <w> - β (<w U>/<w>) <w ∂U/∂θ> / <w>
+
+    `precision` selects the energy path whose weights are used; the gradient kernel evaluates
+    its pair and k-space terms in FP32 and sums them in FP64."""
+    if nblocks < 2:
+        raise _lib.ArgumentError(f"nblocks={nblocks}: at least two blocks are needed for a standard error")
+    if run is not None and run < 1:
+        raise _lib.ArgumentError(f"run={run} must be ≥ 1")
+    T = float(getattr(T, "magnitude", T))
+    g = _GuestArgs(guest)
+    ns, nt = batch.nsys, batch.ntypes
+    nat = np.array([batch.natoms[f] for f in batch.framework_of], dtype=np.int64)
+    qoff = np.concatenate([[0], np.cumsum(nat)])
+    out = (_lib.PaWidomResult * ns)()
+    lj = [np.zeros(max(ns * nt, 1)) for _ in range(4)]
+    dq = [np.zeros(max(int(qoff[-1]), 1)) for _ in range(2)]
+    _lib.check(_lib.lib().pa_widom_grad(batch.handle, C.byref(g.s), T, int(ninsert), int(seed) & (2**64 - 1),
+                                        int(nblocks), 0 if run is None else int(run), _precision(precision), out,
+                                        *[a.ctypes.data for a in lj], *[a.ctypes.data for a in dq]))
+    res = []
+    for s, r in enumerate(out):
+        wr = WidomResult(r.mu_ex, r.mu_ex_err, r.K_H, r.K_H_err, r.q_st, r.q_st_err, int(r.nsamples), int(r.nblocks),
+                         r.ln_K_H)
+        sl = slice(s * nt, (s + 1) * nt)
+        ql = slice(int(qoff[s]), int(qoff[s + 1]))
+        res.append(WidomGrad(wr, lj[0][sl].copy(), lj[1][sl].copy(), lj[2][sl].copy(), lj[3][sl].copy(),
+                             dq[0][ql].copy(), dq[1][ql].copy()))
+    return res
+
+
 def widom_energies(batch: FrameworkBatch, sys: int, seed: int, j0: int, n: int, precision: str = "mixed") -> np.ndarray:
     """Device Delta U (eV) of insertions j0..j0+n-1 of system `sys` (0-based)."""
     dU = np.zeros(max(n, 1))
@@ -109,4 +162,5 @@ def last_timing(batch: FrameworkBatch) -> dict:
     t = _lib.PaTiming()
     _lib.check(_lib.lib().pa_last_timing(batch.handle, C.byref(t)))
     return {"total_ms": t.total_ms, "energy_ms": t.energy_ms, "energy_launches": int(t.energy_launches),
-            "reduce_ms": t.reduce_ms, "insertions": int(t.insertions), "binning_ms": t.binning_ms}
+            "reduce_ms": t.reduce_ms, "insertions": int(t.insertions), "binning_ms": t.binning_ms,
+            "grad_ms": t.grad_ms, "grad_insertions": int(t.grad_insertions)}

@@ -339,7 +339,9 @@ def test_config2_batch_subset_against_oracle():
             assert abs(rm[s].ln_K_H - ref) <= LNKH_TOL
             assert abs(rm[s].q_st - r64[s].q_st) <= QST_TOL
         else:
-            assert abs(rm[s].ln_K_H - ref) <= 1e-6 * abs(ref)
+            # no insertion binds: ln K_H is set by the least repulsive pose, whose energy holds
+            # sub-Å contacts where the FP32 r^-12 term carries ~1e-5 relative error
+            assert abs(rm[s].ln_K_H - ref) <= 1e-5 * abs(ref)
 
 
 def test_empty_box_ideal_gas():

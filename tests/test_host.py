@@ -212,3 +212,29 @@ def test_synthetic_frameworks_follow_their_rule():
     assert dense.natoms == 150
     c1 = synth.config1_framework()
     assert c1.natoms == 8100 and c1.replication == (3, 3, 3)
+
+
+def test_grad_rejects_non_lb_tables_and_shuffled_images():
+    """pa_widom_grad's own checks run before any device work (host-only batch)."""
+    from pureadsorb_amd import synth
+    ff = synth.synthetic_forcefield(11.0)
+    g = synth.co2_guest(ff)
+    uc = synth.synthetic_framework(12.0, 12.0, 12.0, 90, 90, 90, 40, 5, charge_sd=0.5)
+    fw = pa.replicate(uc, (2, 2, 2))
+    b = pa.FrameworkBatch([fw], ff, g, pa.EwaldParams(11.0), host_only=True)
+    with pytest.raises(pa.ArgumentError, match="HOST_ONLY"):
+        pa.widom_grad(b, g, T=298.15, ninsert=1000, nblocks=4)
+    # atoms of the replicated cell out of replicate order: the claim still holds, the image map does not
+    perm = np.random.default_rng(0).permutation(fw.natoms)
+    shuffled = pa.Framework(fw.cell, fw.frac[perm], [fw.labels[i] for i in perm], [fw.symbols[i] for i in perm],
+                            np.asarray(fw.charges)[perm], fw.replication)
+    b2 = pa.FrameworkBatch([shuffled], ff, g, pa.EwaldParams(11.0), host_only=True)
+    with pytest.raises(pa.ArgumentError, match="replicate order"):
+        pa.widom_grad(b2, g, T=298.15, ninsert=1000, nblocks=4)
+    # a force field whose cross terms are not Lorentz-Berthelot
+    sig = np.array(ff.sigma, dtype=np.float64)
+    sig[0, 1] = sig[1, 0] = sig[0, 1] * 1.1
+    ff2 = pa.ForceField(ff.names, sig, np.array(ff.epsilon), ff.cutoff, ff.tail)
+    b3 = pa.FrameworkBatch([fw], ff2, g, pa.EwaldParams(11.0), host_only=True)
+    with pytest.raises(pa.ArgumentError, match="Lorentz-Berthelot"):
+        pa.widom_grad(b3, g, T=298.15, ninsert=1000, nblocks=4)
