@@ -31,6 +31,12 @@ extern "C" {
 /* Precision of pa_widom / pa_widom_energies. */
 #define PA_PREC_F64 0   /* FP64 plain path: one work-item per insertion, every pair in FP64 */
 #define PA_PREC_MIXED 1 /* FP32 pair + k-space arithmetic, FP64 accumulation and tables */
+/* OR-ed into the precision of pa_widom / pa_widom_grad: skip the mixed path's phase-0 hard-core
+ * rejection (the reference's widom_singlephase, src/widom.jl:298). Without it the mixed path
+ * first rejects every insertion whose sites come closer than rho_at to an atom of type t, with
+ * radii from the rigorous bound of src/reject.jl, so that its Boltzmann weight is exactly 0.0 in
+ * FP64; results are identical either way. The FP64 plain path always evaluates every insertion. */
+#define PA_SINGLE_PHASE 0x100u
 
 /* pa_batch_create flags */
 #define PA_BATCH_FULLK 1u     /* keep every k-vector (FrameworkBatch(...; fullk = true)) */
@@ -91,6 +97,9 @@ typedef struct pa_framework_info {
     double alpha;                 /* Ewald splitting parameter, 1/Å */
     double constant_offset;       /* eV, src/batch.jl:396-407 */
     double self_term_halfrange;   /* eV, src/batch.jl:376 */
+    double hardcore_bound;        /* eV, B_s of src/reject.jl:101-131 (per-atom terms rounded up on a
+                                     1.02-ratio grid of |K|, so never below the reference's value);
+                                     +inf disables phase-0 rejection for the framework */
 } pa_framework_info;
 
 /* Device timing of the last pa_widom call on a batch, from HIP events on the library's
@@ -153,6 +162,11 @@ int pa_widom_grad(pa_batch *batch, const pa_guest *guest, double T, int64_t nins
  * computed on the device (the widom_kernel! contract, src/widom.jl:157-175). */
 int pa_widom_energies(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, int64_t n, uint32_t precision,
                       double *dU);
+
+/* Phase-0 hard-core flags of insertions j0 .. j0+n-1 of system `sys` at temperature T (K):
+ * flags[i] = 1 when the mixed path rejects the insertion without computing its energy
+ * (hardcore_kernel!, src/widom.jl:99-151, radii of build_rejection_tables :202-253). */
+int pa_hardcore_flags(pa_batch *batch, int32_t sys, uint64_t seed, double T, int64_t j0, int64_t n, uint8_t *flags);
 
 /* Device-drawn poses of the same insertions: pos (n x 3, Å) and quaternions (n x 4, x y z w). */
 int pa_widom_poses(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, int64_t n, double *pos, double *quat);

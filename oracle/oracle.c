@@ -1128,3 +1128,168 @@ int or_widom_grad(const or_system *s, double T, int64_t ns, uint64_t seed, int64
     free(dU);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Hard-core rejection (src/reject.jl, src/widom.jl:99-151,202-253), FP64 throughout.
+ * ---------------------------------------------------------------------------------------- */
+/* theta_F(Float64), src/reject.jl:11-19 */
+double or_theta_f64(void)
+{
+    double lo = 0.0, hi = 2000.0;
+    while (nextafter(lo, hi) != hi) {
+        double mid = lo + (hi - lo) / 2.0;
+        if (exp(-mid) == 0.0) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+static double or_lj(double r2, double sg, double ep)
+{
+    double x = sg * sg / r2;
+    x = x * x * x;
+    return 4.0 * ep * (x * x - x);
+}
+
+/* src/reject.jl:22 */
+double or_pair_energy(double r, double sg, double ep, double K, double alpha)
+{
+    return or_lj(r * r, sg, ep) + K * or_pair_erfc_f64(alpha * r) / r;
+}
+
+/* src/reject.jl:45-69; NaN when no bracket */
+double or_find_r0(double sg, double ep, double K, double alpha, double r_lj)
+{
+    double hi = sg * pow(2.0, 1.0 / 6.0);
+    if (!(or_pair_energy(hi, sg, ep, K, alpha) < 0.0)) return NAN;
+    double lo = hi;
+    for (int it = 0; or_pair_energy(lo, sg, ep, K, alpha) <= 0.0; ++it) {
+        lo /= 2.0;
+        if (it > 200) return NAN;
+    }
+    for (int it = 0; hi - lo > 4.0 * 2.220446049250313e-16 * hi; ++it) {
+        double mid = (lo + hi) / 2.0;
+        if (or_pair_energy(mid, sg, ep, K, alpha) > 0.0) lo = mid; else hi = mid;
+        if (it > 200) return NAN;
+    }
+    double r = (lo + hi) / 2.0;
+    return r < r_lj ? r : r_lj;
+}
+
+/* src/reject.jl:191-209 */
+double or_find_rho2(double sg, double ep, double kmin, double margin, double r_lj)
+{
+    if (isinf(margin) || ep == 0.0) return 0.0;
+#define OR_F(r) (or_lj((r) * (r), sg, ep) - fabs(kmin) / (r) - margin)
+    double hi = sg;
+    for (int it = 0; OR_F(hi) >= 0.0; ++it) { hi *= 2.0; if (it > 200) return NAN; }
+    double lo = hi;
+    for (int it = 0; OR_F(lo) < 0.0; ++it) { lo /= 2.0; if (it > 200) return NAN; }
+    for (int it = 0; hi - lo > 4.0 * 2.220446049250313e-16 * hi; ++it) {
+        double mid = (lo + hi) / 2.0;
+        if (OR_F(mid) > 0.0) lo = mid; else hi = mid;
+        if (it > 200) return NAN;
+    }
+#undef OR_F
+    double r = (lo + hi) / 2.0;
+    if (r > r_lj) r = r_lj;
+    return r * r;
+}
+
+/* hardcore_bound (src/reject.jl:101-131) over every (site, atom) pair of the system, with the
+ * reciprocal bound over the coupled k subset (the k-vectors the energy sums). */
+double or_hardcore_bound(const or_system *s)
+{
+    double acc = 0.0, qg = 0.0;
+    for (int a = 0; a < s->nsites; ++a) {
+        qg += fabs(s->gq[a]);
+        for (int j = 0; j < s->natoms; ++j) {
+            size_t p = (size_t)s->gtype[a] + (size_t)s->ntypes * s->type[j];
+            double sg = s->sigma[p], ep = s->eps[p], K = OR_KE * s->gq[a] * s->q[j];
+            if (K >= 0.0) { acc += ep; continue; }
+            if (ep == 0.0) return INFINITY;
+            double r0 = or_find_r0(sg, ep, K, s->alpha, s->rc_lj);
+            if (isnan(r0)) return INFINITY;
+            acc += ep + fabs(K) * or_pair_erfc_f64(s->alpha * r0) / r0;
+        }
+    }
+    double Rs = 0.0;
+    for (int t = 0; t < s->nk; ++t) {
+        int i = s->kidx[t];
+        Rs += s->kpref_full[i] * hypot(s->Sre_full[i], s->Sim_full[i]);
+    }
+    return acc + 2.0 * OR_KE * qg * Rs;
+}
+
+/* build_rejection_tables for one system (src/widom.jl:222-253) with the rounding allowance of
+ * arithmetic at unit roundoff eps_f/2 (2.220446049250313e-16 for FP64 energies, 1.1920929e-7
+ * for FP32). rho2: nsites x ntypes (force-field types). Returns the margin (eV). */
+double or_rejection_tables(const or_system *s, double kT, double eps_f, double *rho2)
+{
+    double theta = or_theta_f64();
+    double Bs = or_hardcore_bound(s), cs = s->constant_offset;
+    double n = (double)s->nsites * s->natoms + s->nk + 8;
+    double safety = 2.0 * n * eps_f * (Bs + fabs(cs)) + 4.0e-6 * Bs;
+    double margin = isinf(Bs) ? INFINITY : (theta + 2.0) * kT + safety + Bs - cs;
+    for (int a = 0; a < s->nsites; ++a)
+        for (int t = 0; t < s->ntypes; ++t) {
+            double kmin = 0.0;
+            for (int j = 0; j < s->natoms; ++j)
+                if (s->type[j] == t) {
+                    double K = OR_KE * s->gq[a] * s->q[j];
+                    if (K < kmin) kmin = K;
+                }
+            size_t p = (size_t)s->gtype[a] + (size_t)s->ntypes * t;
+            rho2[a * s->ntypes + t] = margin > 0.0 ? or_find_rho2(s->sigma[p], s->eps[p], kmin, margin, s->rc_lj) : NAN;
+        }
+    return margin;
+}
+
+/* Phase-0 flags of insertions j0..j0+n-1 by brute force over every atom, per-site minimum
+ * image: flag = some site within sqrt(rho2[site][type]) of an atom of that type. */
+void or_hardcore_flags(const or_system *s, uint64_t seed, int64_t sysidx, int64_t j0, int64_t n, const double *rho2,
+                       uint8_t *flags, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < n; ++t) {
+        double pos[3], q[4];
+        or_pose(seed, sysidx, j0 + t, s->A, pos, q);
+        uint8_t f = 0;
+        for (int a = 0; a < s->nsites && !f; ++a) {
+            double r[3], gp[3];
+            or_rotate(q, s->gsite + 3 * a, r);
+            for (int d = 0; d < 3; ++d) gp[d] = pos[d] + r[d];
+            for (int j = 0; j < s->natoms; ++j) {
+                double dd[3] = {gp[0] - s->pos[3 * j], gp[1] - s->pos[3 * j + 1], gp[2] - s->pos[3 * j + 2]}, m[3];
+                or_minimum_image(s->A, s->invA, dd, m);
+                if (or_dot3(m, m) < rho2[a * s->ntypes + s->type[j]]) { f = 1; break; }
+            }
+        }
+        flags[t] = f;
+    }
+}
+
+/* The reference's two-phase CPU path (src/widom.jl:390-420): phase-0 flags by the brute-force
+ * check, then energies of the survivors only; rejected insertions get dU = +inf (weight 0). */
+void or_energies_twophase(const or_system *s, uint64_t seed, int64_t sysidx, int64_t j0, int64_t n, const double *rho2,
+                          double *dU, int64_t *npairs, int nthreads)
+{
+    uint8_t *flags = malloc((size_t)(n > 0 ? n : 1));
+    or_hardcore_flags(s, seed, sysidx, j0, n, rho2, flags, nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < n; ++t) {
+        if (flags[t]) {
+            dU[t] = INFINITY;
+            if (npairs) npairs[t] = 0;
+            continue;
+        }
+        double pos[3], q[4];
+        or_pose(seed, sysidx, j0 + t, s->A, pos, q);
+        int64_t np = 0;
+        dU[t] = or_insertion_energy(s, pos, q, &np) + s->constant_offset;
+        if (npairs) npairs[t] = np;
+    }
+    free(flags);
+}

@@ -119,6 +119,23 @@ def lib():
     L.or_reduce.argtypes = [C.c_int, _dp, _dp, _lp, C.c_double, C.c_double, _dp]
     L.or_widom.restype = C.c_int
     L.or_widom.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, C.c_int]
+    L.or_theta_f64.restype = C.c_double
+    L.or_theta_f64.argtypes = []
+    L.or_pair_energy.restype = C.c_double
+    L.or_pair_energy.argtypes = [C.c_double] * 5
+    L.or_find_r0.restype = C.c_double
+    L.or_find_r0.argtypes = [C.c_double] * 5
+    L.or_find_rho2.restype = C.c_double
+    L.or_find_rho2.argtypes = [C.c_double] * 5
+    L.or_hardcore_bound.restype = C.c_double
+    L.or_hardcore_bound.argtypes = [C.c_void_p]
+    L.or_rejection_tables.restype = C.c_double
+    L.or_rejection_tables.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.or_hardcore_flags.restype = None
+    L.or_hardcore_flags.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_void_p, C.c_int]
+    L.or_energies_twophase.restype = None
+    L.or_energies_twophase.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp, _dp, C.c_void_p,
+                                       C.c_int]
     L.or_widom_grad.restype = C.c_int
     L.or_widom_grad.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp]
@@ -127,6 +144,25 @@ def lib():
 
 
 # --- scalar helpers -------------------------------------------------------------------------
+def theta_f64() -> float:
+    """theta_F(Float64) (src/reject.jl:11-19)."""
+    return lib().or_theta_f64()
+
+
+def pair_energy(r, sigma, eps, K, alpha) -> float:
+    """src/reject.jl:22"""
+    return lib().or_pair_energy(r, sigma, eps, K, alpha)
+
+
+def find_r0(sigma, eps, K, alpha, r_lj) -> float:
+    """src/reject.jl:45-69"""
+    return lib().or_find_r0(sigma, eps, K, alpha, r_lj)
+
+
+def find_rho2(sigma, eps, kmin, margin, r_lj) -> float:
+    """src/reject.jl:191-209"""
+    return lib().or_find_rho2(sigma, eps, kmin, margin, r_lj)
+
 def splitmix64(seed: int, index: int) -> int:
     """src/state.jl:225-230"""
     return lib().or_splitmix64(seed & (2**64 - 1), index & (2**64 - 1))
@@ -292,6 +328,7 @@ class OracleSystem:
         epsilon = np.asarray(epsilon, dtype=np.float64)
         nt = sigma.shape[0]
         self.ntypes = nt
+        self.nsites = len(np.asarray(guest_charges).reshape(-1))
         sig = np.ascontiguousarray(sigma.T.reshape(-1))
         eps = np.ascontiguousarray(epsilon.T.reshape(-1))
         gs = np.ascontiguousarray(guest_sites, dtype=np.float64).reshape(-1)
@@ -364,6 +401,31 @@ class OracleSystem:
         out = np.zeros(8)
         st = lib().or_widom(self._h, T, ns, seed & (2**64 - 1), sysidx, nblocks, bs, out, nthreads)
         return st, out, bs.reshape(nblocks, 3)
+
+    def hardcore_bound(self):
+        """B_s of src/reject.jl:101-131 (exact per-atom terms)."""
+        return lib().or_hardcore_bound(self._h)
+
+    def rejection_tables(self, kT, eps_f):
+        """(margin, rho2[nsites, ntypes]) of build_rejection_tables (src/widom.jl:222-253)."""
+        rho2 = np.zeros(self.nsites * self.ntypes)
+        margin = lib().or_rejection_tables(self._h, kT, eps_f, rho2)
+        return margin, rho2.reshape(self.nsites, self.ntypes)
+
+    def hardcore_flags(self, seed, sysidx, j0, n, rho2, nthreads=0):
+        """Phase-0 flags (1 = rejected) of insertions j0..j0+n-1, brute force."""
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        r = np.ascontiguousarray(rho2, dtype=np.float64).reshape(-1)
+        lib().or_hardcore_flags(self._h, seed & (2**64 - 1), sysidx, j0, n, r, flags.ctypes.data, nthreads)
+        return flags[:n]
+
+    def energies_twophase(self, seed, sysidx, j0, n, rho2, nthreads=0):
+        """(dU, in-cutoff pairs) of the two-phase path: rejected insertions get dU = +inf, 0 pairs."""
+        dU = np.zeros(max(n, 1))
+        pairs = np.zeros(max(n, 1), dtype=np.int64)
+        r = np.ascontiguousarray(rho2, dtype=np.float64).reshape(-1)
+        lib().or_energies_twophase(self._h, seed & (2**64 - 1), sysidx, j0, n, r, dU, pairs.ctypes.data, nthreads)
+        return dU[:n], pairs[:n]
 
     def widom_grad(self, T, ns, seed, sysidx, nthreads=0):
         """FP64 reweighting gradients of ln K_H and q_st over insertions 0..ns-1 (or_widom_grad).

@@ -17,6 +17,8 @@ struct FwDesc {
     int atom_off, natoms;  // slice of the atom arrays
     int k_off, nk;         // slice of the k tables
     int upper;             // A is upper triangular (cell_matrix's form)
+    int cg_off;            // hard-core cell grid: first entry of the framework in cell_start
+    int cg_nc[3];          // cells per axis (fractional cell edge 1/cg_nc)
 };
 
 // The guest, with compact type indices.
@@ -68,6 +70,7 @@ struct BinSys {
     float rlist2;   // (max cutoff + guest radius + bin circumradius + margin)^2, Å^2
     float alt[3];   // an atom with |fractional offset| >= alt[i] on axis i also gets the image
                     // across the cell boundary on that axis (alt >= 0.5: never)
+    int reach[3];   // hard-core stencil half-width in grid cells per axis (reach[0] < 0: no check)
 };
 
 struct BinnedArgs {
@@ -97,6 +100,10 @@ struct BinnedArgs {
     float rc_lj2f, rc_ew2f, rc_max2f;
     int listcap;               // neighbour-list capacity (entries) of one pass
     GuestDev guest;
+    // hard-core rejection (phase 0): atoms of each framework sorted by grid cell
+    const uint4 *corefx;       // fixed-point x, y, z and compact type, by cell
+    const int *cell_start;     // per framework: cg_nc[0]*cg_nc[1]*cg_nc[2] + 1 offsets
+    const float *rho2;         // nsys x NS x ntypes_c rejection radii^2 (Å^2), rounded down
 };
 
 // Gradient launch: one weighted insertion per wavefront (widom_grad.hip).

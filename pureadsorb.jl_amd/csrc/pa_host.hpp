@@ -1,0 +1,118 @@
+// pa_host.hpp -- host-side definitions shared by the library's translation units: status
+// helpers, device allocations, the batch's host tables and device storage.
+#pragma once
+
+#include <pureadsorb_amd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "pa_device.hpp"
+
+namespace pa {
+
+// Sets the thread-local message of pa_last_error and returns `code`.
+int fail(int code, const char *fmt, ...);
+// Clears the message and returns PA_OK.
+int ok();
+// PA_OK once a HIP device is present (checked once per process).
+int device_ready();
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return pa::fail(PA_ERR_DEVICE, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, \
+                            __LINE__, #expr);                                                            \
+    } while (0)
+
+struct DevAlloc {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevAlloc() { if (p) (void)hipFree(p); }
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, n ? n : 16);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct FwHost {
+    int natoms, nk, nk_full;
+    double volume, alpha, const_off, halfrange;
+    double perp[3];  // perpendicular widths, Å
+};
+
+// Host copies of one framework's tables that the gradient entry point reads back.
+struct FwGradHost {
+    std::vector<double> frac;    // natoms x 3
+    std::vector<int> kn;         // 3 per kept k-vector: reciprocal-basis coefficients
+    std::vector<double> kpref;   // per kept k-vector
+    std::vector<double> counts;  // atoms per force-field type
+    int rep[3] = {1, 1, 1};
+    bool image_order = true;     // atoms are `replicate` images of the first natoms/nimg (image-major)
+};
+
+template <class T>
+inline hipError_t upload(DevAlloc &d, const std::vector<T> &v)
+{
+    hipError_t e = d.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace pa
+
+// The opaque handle of the C ABI.
+struct pa_batch {
+    int nsys = 0, nfw = 0;
+    std::vector<int> fw_of;
+    std::vector<pa::FwHost> info;
+    std::vector<pa::FwDesc> desc;
+    bool upper_all = true;
+    // guest exactly as given (for the same-guest check) and its device form
+    int nsites = 0;
+    std::vector<double> gsites, gq;
+    std::vector<int> gtypes;
+    pa::GuestDev gdev{};
+    int ntypes_c = 0;
+    double rc_lj = 0, rc_ew = 0;
+    double r_guest = 0;  // largest site distance from the guest's reference point
+    // force field as given, compact type map and per-framework host tables (gradients)
+    int nt_ff = 0, ff_tail = 0;
+    std::vector<double> ff_sigma, ff_eps;
+    std::vector<int> c_to_orig;
+    std::vector<double2> pair64_h;
+    std::vector<pa::FwGradHost> fgh;
+    // device tables
+    pa::DevAlloc d_fw, d_fw_of, d_atom64, d_atype, d_k64, d_S64, d_atomfx, d_kn, d_kc, d_pair64, d_pair32;
+    // per-call scratch
+    pa::DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
+    // mixed-path binning scratch
+    pa::DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_cursor, d_order, d_scan_tmp;
+    // hard-core rejection: per unique framework the bound B_s (src/reject.jl:101-131) and
+    // kmin[site][compact type] (src/reject.jl:141-153); device cell grids
+    std::vector<double> hc_bs, hc_kmin;
+    pa::DevAlloc d_corefx, d_cell_start, d_rho2;
+    // gradient scratch
+    pa::DevAlloc d_list_g, d_list_w, d_list_wu, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch
+    pa_timing last{};
+    bool host_only = false;
+    ~pa_batch()
+    {
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto &e : kev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+

@@ -3,6 +3,7 @@
 #include <pureadsorb_amd.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -14,7 +15,7 @@
 #include <thread>
 #include <vector>
 
-#include "pa_device.hpp"
+#include "pa_host.hpp"
 
 namespace pa {
 hipError_t launch_energy(const WidomArgs &a, int precision, bool upper, hipStream_t st);
@@ -36,7 +37,7 @@ using namespace pa;
 // ------------------------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+int pa::fail(int code, const char *fmt, ...)
 {
     char buf[2048];
     va_list ap;
@@ -47,18 +48,11 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
-static int ok()
+int pa::ok()
 {
     g_err.clear();
     return PA_OK;
 }
-
-#define HIP_TRY(expr)                                                                                    \
-    do {                                                                                                 \
-        hipError_t e_ = (expr);                                                                          \
-        if (e_ != hipSuccess) return fail(PA_ERR_DEVICE, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), \
-                                          __FILE__, __LINE__, #expr);                                    \
-    } while (0)
 
 // ------------------------------------------------------------------------------------------
 // Host numerics (FP64)
@@ -252,84 +246,85 @@ void self_orientation(int o, double q[4])
     shoemake(u1, u2, u3, q);
 }
 
+
+// ---- hard-core rejection bounds (src/reject.jl) ----------------------------------------------
+
+// theta_F(Float64) (src/reject.jl:11-19): smallest theta with exp(-theta) == 0.0, by bisection on
+// the float grid with this host's exp.
+double theta_f64()
+{
+    double lo = 0.0, hi = 2000.0;
+    while (std::nextafter(lo, hi) != hi) {
+        const double mid = lo + (hi - lo) / 2.0;
+        if (std::exp(-mid) == 0.0) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+double lj_pair(double r2, double sg, double ep)
+{
+    double x = sg * sg / r2;
+    x = x * x * x;
+    return 4.0 * ep * (x * x - x);
+}
+
+// pair_energy (src/reject.jl:22): LJ plus K erfc(alpha r)/r with the pair series.
+double pair_energy_h(double r, double sg, double ep, double K, double alpha)
+{
+    return lj_pair(r * r, sg, ep) + K * pair_erfc_f64(alpha * r) / r;
+}
+
+constexpr int MAX_BISECT = 200;
+
+// find_r0 (src/reject.jl:45-69): zero of the pair energy below its minimum, attractive pair
+// (K < 0, eps > 0); clamped to the LJ cutoff. Returns NaN when no bracket is found.
+double find_r0(double sg, double ep, double K, double alpha, double r_lj)
+{
+    double hi = sg * std::pow(2.0, 1.0 / 6.0);
+    if (!(pair_energy_h(hi, sg, ep, K, alpha) < 0.0)) return NAN;
+    double lo = hi;
+    for (int it = 0; pair_energy_h(lo, sg, ep, K, alpha) <= 0.0; ++it) {
+        lo /= 2.0;
+        if (it > MAX_BISECT) return NAN;
+    }
+    for (int it = 0; hi - lo > 4.0 * DBL_EPSILON * hi; ++it) {
+        const double mid = (lo + hi) / 2.0;
+        if (pair_energy_h(mid, sg, ep, K, alpha) > 0.0) lo = mid; else hi = mid;
+        if (it > MAX_BISECT) return NAN;
+    }
+    return std::min((lo + hi) / 2.0, r_lj);
+}
+
+// find_rho2 (src/reject.jl:191-209): first root of LJ(r) - |kmin|/r = margin, clamped to the
+// LJ cutoff, squared; 0 when the margin is infinite or the pair has no LJ wall. NaN on failure.
+double find_rho2(double sg, double ep, double kmin, double margin, double r_lj)
+{
+    if (std::isinf(margin) || ep == 0.0) return 0.0;
+    auto f = [&](double r) { return lj_pair(r * r, sg, ep) - std::fabs(kmin) / r - margin; };
+    double hi = sg;
+    for (int it = 0; f(hi) >= 0.0; ++it) {
+        hi *= 2.0;
+        if (it > MAX_BISECT) return NAN;
+    }
+    double lo = hi;
+    for (int it = 0; f(lo) < 0.0; ++it) {
+        lo /= 2.0;
+        if (it > MAX_BISECT) return NAN;
+    }
+    for (int it = 0; hi - lo > 4.0 * DBL_EPSILON * hi; ++it) {
+        const double mid = (lo + hi) / 2.0;
+        if (f(mid) > 0.0) lo = mid; else hi = mid;
+        if (it > MAX_BISECT) return NAN;
+    }
+    const double r = std::min((lo + hi) / 2.0, r_lj);
+    return r * r;
+}
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
 // Batch
 // ------------------------------------------------------------------------------------------
-struct DevAlloc {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevAlloc() { if (p) (void)hipFree(p); }
-    hipError_t ensure(size_t n)
-    {
-        if (n <= bytes && p) return hipSuccess;
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
-        hipError_t e = hipMalloc(&p, n ? n : 16);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    template <class T> T *as() const { return (T *)p; }
-};
-
-struct FwHost {
-    int natoms, nk, nk_full;
-    double volume, alpha, const_off, halfrange;
-    double perp[3];  // perpendicular widths, Å
-};
-
-// Host copies of one framework's tables that the gradient entry point reads back.
-struct FwGradHost {
-    std::vector<double> frac;    // natoms x 3
-    std::vector<int> kn;         // 3 per kept k-vector: reciprocal-basis coefficients
-    std::vector<double> kpref;   // per kept k-vector
-    std::vector<double> counts;  // atoms per force-field type
-    int rep[3] = {1, 1, 1};
-    bool image_order = true;     // atoms are `replicate` images of the first natoms/nimg (image-major)
-};
-
-struct pa_batch {
-    int nsys = 0, nfw = 0;
-    std::vector<int> fw_of;
-    std::vector<FwHost> info;
-    std::vector<FwDesc> desc;
-    bool upper_all = true;
-    // guest exactly as given (for the same-guest check) and its device form
-    int nsites = 0;
-    std::vector<double> gsites, gq;
-    std::vector<int> gtypes;
-    GuestDev gdev{};
-    int ntypes_c = 0;
-    double rc_lj = 0, rc_ew = 0;
-    double r_guest = 0;  // largest site distance from the guest's reference point
-    // force field as given, compact type map and per-framework host tables (gradients)
-    int nt_ff = 0, ff_tail = 0;
-    std::vector<double> ff_sigma, ff_eps;
-    std::vector<int> c_to_orig;
-    std::vector<double2> pair64_h;
-    std::vector<FwGradHost> fgh;
-    // device tables
-    DevAlloc d_fw, d_fw_of, d_atom64, d_atype, d_k64, d_S64, d_atomfx, d_kn, d_kc, d_pair64, d_pair32;
-    // per-call scratch
-    DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
-    // mixed-path binning scratch
-    DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_cursor, d_order, d_scan_tmp;
-    // gradient scratch
-    DevAlloc d_list_g, d_list_w, d_list_wu, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch
-    pa_timing last{};
-    bool host_only = false;
-    ~pa_batch()
-    {
-        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
-        for (auto &e : kev) if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-static int device_ready()
+int pa::device_ready()
 {
     static std::once_flag once;
     static int status = PA_OK;
@@ -372,15 +367,6 @@ extern "C" int pa_synchronize(void)
     if (st) return st;
     HIP_TRY(hipDeviceSynchronize());
     return ok();
-}
-
-template <class T>
-static hipError_t upload(DevAlloc &d, const std::vector<T> &v)
-{
-    hipError_t e = d.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
-    if (e != hipSuccess) return e;
-    if (v.empty()) return hipSuccess;
-    return hipMemcpy(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
 }
 
 extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_forcefield *ff, const pa_guest *guest,
@@ -494,6 +480,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<double2> S64;
     std::vector<int4> kn;
     std::vector<float2> kc;
+    std::vector<uint4> corefx;
+    std::vector<int> cell_start;
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
@@ -532,6 +520,33 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             uint32_t qb;
             std::memcpy(&qb, &qf, 4);
             atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
+        }
+        // hard-core cell grid: cells of at least 1.6 Å, at most 64 per axis; atoms sorted by cell
+        {
+            int nct = 1;
+            for (int d = 0; d < 3; ++d) {
+                D.cg_nc[d] = std::max(1, std::min(64, (int)std::floor(L[d] / 1.6)));
+                nct *= D.cg_nc[d];
+            }
+            D.cg_off = (int)cell_start.size();
+            std::vector<int> cnt(nct + 1, 0), cell_of(n);
+            for (int j = 0; j < n; ++j) {
+                const uint4 &x = atomfx[D.atom_off + j];
+                const uint32_t fx[3] = {x.x, x.y, x.z};
+                int c[3];
+                for (int d = 0; d < 3; ++d) c[d] = (int)(((uint64_t)fx[d] * (uint64_t)D.cg_nc[d]) >> 32);
+                cell_of[j] = c[0] + D.cg_nc[0] * (c[1] + D.cg_nc[1] * c[2]);
+                ++cnt[cell_of[j] + 1];
+            }
+            for (int c = 0; c < nct; ++c) cnt[c + 1] += cnt[c];
+            cell_start.insert(cell_start.end(), cnt.begin(), cnt.end());
+            std::vector<uint4> sorted(n);
+            std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+            for (int j = 0; j < n; ++j) {
+                const uint4 &x = atomfx[D.atom_off + j];
+                sorted[cur[cell_of[j]]++] = make_uint4(x.x, x.y, x.z, (uint32_t)atype[D.atom_off + j]);
+            }
+            corefx.insert(corefx.end(), sorted.begin(), sorted.end());
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
         FwGradHost &gh = b->fgh[u];
@@ -690,6 +705,59 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
 
     b->pair64_h = pair64;
 
+    // ---- hard-core bound B_s and kmin per framework (src/batch.jl:395,409-423; src/reject.jl).
+    // The attractive-pair term eps + |K| erfc(alpha r0)/r0 grows with |K|; it is evaluated on a
+    // geometric grid |K|_i = |K|_max 1.02^-i per (site, type) and each atom takes the grid point
+    // at or above its own |K|, which bounds the reference's per-atom term from above.
+    b->hc_bs.assign(b->nfw, 0.0);
+    b->hc_kmin.assign((size_t)b->nfw * NS * NTC, 0.0);
+    double qg_abs = 0.0;
+    for (int a = 0; a < NS; ++a) qg_abs += std::fabs(guest->charges[a]);
+    parallel_for(b->nfw, [&](int64_t u0, int64_t u1) {
+        for (int64_t u = u0; u < u1; ++u) {
+            const FwDesc &D = b->desc[u];
+            double acc = 0.0;
+            bool unbounded = false;
+            for (int a = 0; a < NS && !unbounded; ++a) {
+                const double qa = guest->charges[a];
+                std::vector<double> npos(NTC, 0.0), kmax(NTC, 0.0);
+                std::vector<std::vector<double>> negK(NTC);
+                for (int j = 0; j < D.natoms; ++j) {
+                    const int c = atype[D.atom_off + j];
+                    const double K = KE * qa * atom64[D.atom_off + j].w;
+                    if (K >= 0.0) npos[c] += 1.0;
+                    else { negK[c].push_back(-K); kmax[c] = std::max(kmax[c], -K); }
+                }
+                for (int c = 0; c < NTC && !unbounded; ++c) {
+                    const double sg = pair64[(size_t)a * NTC + c].x, ep = pair64[(size_t)a * NTC + c].y;
+                    acc += npos[c] * ep;
+                    b->hc_kmin[((size_t)u * NS + a) * NTC + c] = -kmax[c];
+                    if (negK[c].empty()) continue;
+                    if (ep == 0.0) { unbounded = true; break; }
+                    const double lg = std::log(1.02);
+                    std::map<int, double> term;
+                    for (double k : negK[c]) {
+                        int i = std::min(511, (int)std::floor(std::log(kmax[c] / k) / lg));
+                        while (i > 0 && kmax[c] * std::pow(1.02, -i) < k) --i;
+                        auto it = term.find(i);
+                        if (it == term.end()) {
+                            const double G = kmax[c] * std::pow(1.02, -i);
+                            const double r0 = find_r0(sg, ep, -G, alpha, rc_lj);
+                            const double v = std::isnan(r0) ? INFINITY : ep + G * pair_erfc_f64(alpha * r0) / r0;
+                            it = term.emplace(i, v).first;
+                        }
+                        acc += it->second;
+                    }
+                    if (!std::isfinite(acc)) unbounded = true;
+                }
+            }
+            double Rs = 0.0;
+            for (int k = 0; k < D.nk; ++k)
+                Rs += k64[D.k_off + k].w * std::hypot(S64[D.k_off + k].x, S64[D.k_off + k].y);
+            b->hc_bs[u] = unbounded ? INFINITY : acc + 2.0 * KE * qg_abs * Rs;
+        }
+    });
+
     if (flags & PA_BATCH_HOST_ONLY) {
         b->host_only = true;
         *out = b.release();
@@ -711,6 +779,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     HIP_TRY(upload(b->d_kc, kc));
     HIP_TRY(upload(b->d_pair64, pair64));
     HIP_TRY(upload(b->d_pair32, pair32));
+    HIP_TRY(upload(b->d_corefx, corefx));
+    HIP_TRY(upload(b->d_cell_start, cell_start));
     *out = b.release();
     return ok();
 }
@@ -744,6 +814,7 @@ extern "C" int pa_batch_framework_info(const pa_batch *b, int32_t fw, pa_framewo
     out->alpha = h.alpha;
     out->constant_offset = h.const_off;
     out->self_term_halfrange = h.halfrange;
+    out->hardcore_bound = b->hc_bs[fw];
     return ok();
 }
 
@@ -778,6 +849,7 @@ static WidomArgs base_args(pa_batch *b)
 static int check_precision(const pa_batch *b, uint32_t precision)
 {
     if (b->host_only) return fail(PA_ERR_ARG, "batch was built with PA_BATCH_HOST_ONLY and has no device tables");
+    precision &= ~PA_SINGLE_PHASE;
     if (precision != PA_PREC_F64 && precision != PA_PREC_MIXED)
         return fail(PA_ERR_ARG, "precision=%u: use PA_PREC_F64 (0) or PA_PREC_MIXED (1)", precision);
     if (b->nsites > 6) return fail(PA_ERR_ARG, "guests with %d sites: the device kernels take 1..6 sites", b->nsites);
@@ -814,8 +886,12 @@ constexpr int LISTCAP_MAX = 3072;
 // Delta U of flattened insertions g in [gb, ge) (system s owns g in [seg[s], seg[s+1])) into
 // b->d_dU[g - out_base], in chunks of at most `chunk` insertions. Records a HIP event pair around
 // every energy-kernel launch (b->kev) and returns the launch count in *launches.
+// With kT > 0 and the mixed path, insertions that phase 0 rejects get Delta U = +inf.
+// With keys_out, the mixed path stops after phase 0 and binning and returns each insertion's
+// bin key (-1: rejected) instead of energies.
 static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb, int64_t ge, int64_t out_base,
-                        uint64_t seed, uint32_t precision, int64_t chunk, int64_t *launches)
+                        uint64_t seed, uint32_t precision, int64_t chunk, int64_t *launches, double kT,
+                        std::vector<int> *keys_out = nullptr)
 {
     const int nsys = b->nsys;
     HIP_TRY(upload(b->d_seg, seg));
@@ -827,7 +903,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         b->kev.push_back(e);
     }
     *launches = 0;
-    if (precision == PA_PREC_F64) {
+    if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_F64) {
         WidomArgs a = base_args(b);
         a.seg = b->d_seg.as<int64_t>();
         a.seed = seed;
@@ -872,15 +948,54 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
             }
         const double R = rcmax + b->r_guest + rb + 0.02;
         B.rlist2 = (float)(R * R);
+        B.reach[0] = B.reach[1] = B.reach[2] = -1;
         for (int d = 0; d < 3; ++d) B.alt[d] = (float)std::max(0.0, 1.0 - R / h.perp[d]);
         B.bin_base = base[s];
         base[s + 1] = base[s] + (int)nb;
         cap_est = std::max(cap_est, 1.3 * h.natoms / h.volume * (4.0 / 3.0) * PI * R * R * R + 256.0);
     }
+    // phase-0 rejection radii for this kT (src/widom.jl:202-253): margin = (theta + 2) kT + safety
+    // + B_s - c_s with the FP32 rounding allowance of the mixed path's pair terms
+    std::vector<float> rho2((size_t)nsys * b->nsites * b->ntypes_c, 0.0f);
+    if (kT > 0.0) {
+        static const double theta = theta_f64();
+        const int NS = b->nsites, NTC = b->ntypes_c;
+        for (int s = 0; s < nsys; ++s) {
+            const int u = b->fw_of[s];
+            const FwHost &h = b->info[u];
+            const FwDesc &D = b->desc[u];
+            const double Bs = b->hc_bs[u], cs = h.const_off;
+            const double n = (double)NS * h.natoms + h.nk + 8;
+            const double safety = 2.0 * n * (double)FLT_EPSILON * (Bs + std::fabs(cs)) + 4.0e-6 * Bs;
+            const double margin = std::isinf(Bs) ? INFINITY : (theta + 2.0) * kT + safety + Bs - cs;
+            if (!(margin > 0.0))
+                return fail(PA_ERR_ARG,
+                            "build_rejection_tables: system %d has margin=%g <= 0 (Bs=%g, safety=%g, constant_offset=%g, "
+                            "kT=%g, theta_F=%g); no valid rejection radius exists",
+                            s + 1, margin, Bs, safety, cs, kT, theta);
+            double rmax2 = 0.0;
+            for (int a = 0; a < NS; ++a)
+                for (int c = 0; c < NTC; ++c) {
+                    const double2 p = b->pair64_h[(size_t)a * NTC + c];
+                    double r2 = find_rho2(p.x, p.y, b->hc_kmin[((size_t)u * NS + a) * NTC + c], margin, b->rc_lj);
+                    if (!(r2 >= 0.0)) r2 = 0.0;  // no bracket: no rejection for this pair of types
+                    // FP32 radius rounded down with a 1e-5 relative allowance for the device's
+                    // FP32 distance
+                    float f = (float)(r2 * (1.0 - 1.0e-5));
+                    if ((double)f > r2 * (1.0 - 1.0e-5)) f = std::nextafter(f, 0.0f);
+                    rho2[((size_t)s * NS + a) * NTC + c] = f;
+                    rmax2 = std::max(rmax2, r2);
+                }
+            if (rmax2 > 0.0)
+                for (int d = 0; d < 3; ++d)
+                    bins[s].reach[d] = (int)std::ceil(std::sqrt(rmax2) * D.cg_nc[d] / h.perp[d]);
+        }
+    }
     const int nbins = base[nsys];
     const int listcap = (int)std::min<double>(LISTCAP_MAX, std::ceil(cap_est / 64.0) * 64.0);
     HIP_TRY(upload(b->d_bins, bins));
     HIP_TRY(upload(b->d_bin_base, base));
+    HIP_TRY(upload(b->d_rho2, rho2));
     HIP_TRY(b->d_bin_count.ensure(sizeof(int) * (nbins + 1)));
     HIP_TRY(b->d_bin_start.ensure(sizeof(int) * (nbins + 1)));
     HIP_TRY(b->d_cursor.ensure(sizeof(int) * nbins));
@@ -920,12 +1035,22 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.rc_max2f = std::max(A.rc_lj2f, A.rc_ew2f);
     A.listcap = listcap;
     A.guest = b->gdev;
+    A.corefx = b->d_corefx.as<uint4>();
+    A.cell_start = b->d_cell_start.as<int>();
+    A.rho2 = b->d_rho2.as<float>();
     for (int64_t g0 = gb; g0 < ge; g0 += CH) {
         A.g0 = g0;
         A.g1 = std::min(ge, g0 + CH);
         HIP_TRY(hipMemsetAsync(A.bin_count, 0, sizeof(int) * (nbins + 1), b->stream));
         HIP_TRY(hipMemsetAsync(A.cursor, 0, sizeof(int) * nbins, b->stream));
         HIP_TRY(launch_binning(A, b->d_scan_tmp.p, scan_bytes, b->stream));
+        if (keys_out) {
+            keys_out->resize(ge - gb);
+            HIP_TRY(hipMemcpyAsync(keys_out->data() + (g0 - gb), A.key, sizeof(int) * (A.g1 - A.g0),
+                                   hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            continue;
+        }
         HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
         HIP_TRY(launch_binned_energy(A, b->upper_all, b->stream));
         HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
@@ -1005,7 +1130,9 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
 
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     int64_t launches = 0;
-    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches);
+    // the mixed path runs phase 0 (hard-core rejection) unless PA_SINGLE_PHASE; the FP64 plain
+    // path evaluates every insertion
+    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches, (precision & PA_SINGLE_PHASE) ? 0.0 : kT);
     if (st) return st;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(launch_tiles(b->d_dU.as<double>(), b->d_tile_g0.as<int64_t>(), b->d_tile_n.as<int>(), ntiles, kT,
@@ -1403,10 +1530,32 @@ extern "C" int pa_widom_energies(pa_batch *b, int32_t sys, uint64_t seed, int64_
     for (int s = sys + 1; s <= b->nsys; ++s) seg[s] = j0 + n;
     HIP_TRY(b->d_dU.ensure(sizeof(double) * n));
     int64_t launches = 0;
-    st = run_energies(b, seg, j0, j0 + n, j0, seed, precision, 0, &launches);
+    st = run_energies(b, seg, j0, j0 + n, j0, seed, precision, 0, &launches, 0.0);
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(dU, b->d_dU.p, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return ok();
+}
+
+extern "C" int pa_hardcore_flags(pa_batch *b, int32_t sys, uint64_t seed, double T, int64_t j0, int64_t n,
+                                 uint8_t *flags)
+{
+    if (!b || (!flags && n > 0)) return fail(PA_ERR_ARG, "NULL argument");
+    int st = check_precision(b, PA_PREC_MIXED);
+    if (st) return st;
+    if (sys < 0 || sys >= b->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, b->nsys - 1);
+    if (j0 < 0 || n < 0) return fail(PA_ERR_ARG, "j0=%lld, n=%lld must be >= 0", (long long)j0, (long long)n);
+    if (!(T > 0)) return fail(PA_ERR_ARG, "T=%g must be a positive temperature in K", T);
+    if (n == 0) return ok();
+    std::vector<int64_t> seg(b->nsys + 1, 0);
+    for (int s = 0; s < b->nsys; ++s) seg[s + 1] = seg[s] + (s == sys ? j0 + n : 0);
+    HIP_TRY(b->d_dU.ensure(sizeof(double) * (size_t)n));
+    std::vector<int> keys;
+    int64_t launches = 0;
+    st = run_energies(b, seg, seg[sys] + j0, seg[sys] + j0 + n, seg[sys] + j0, seed, PA_PREC_MIXED, 0, &launches, KB * T,
+                      &keys);
+    if (st) return st;
+    for (int64_t i = 0; i < n; ++i) flags[i] = keys[i] < 0 ? 1 : 0;
     return ok();
 }
 

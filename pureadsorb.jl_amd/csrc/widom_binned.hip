@@ -66,16 +66,60 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
     double f[3], q[4];
     pose_uniforms(a.seed, s, j, f, q);
     uint32_t *out = a.sitefx + (size_t)(g - a.g0) * 3 * NS;
+    uint32_t sp[NS][3];
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
         double r[3], df[3];
         rotate(q, a.guest.site[t], r);
         matvec(F.invA, r, df);
-        out[3 * t + 0] = to_fixed(f[0] + df[0]);
-        out[3 * t + 1] = to_fixed(f[1] + df[1]);
-        out[3 * t + 2] = to_fixed(f[2] + df[2]);
+        for (int d = 0; d < 3; ++d) {
+            sp[t][d] = to_fixed(f[d] + df[d]);
+            out[3 * t + d] = sp[t][d];
+        }
     }
     const BinSys B = a.bins[s];
+    if (B.reach[0] >= 0) {
+        // Phase 0 (src/widom.jl:99-151): the insertion is rejected when some site lies closer
+        // than rho_at to an atom of type t; the radii guarantee exp(-dU/kT) == 0.0 in FP64.
+        // Atoms come from the grid cells within reach of the site's home cell.
+        const int *cs = a.cell_start + F.cg_off;
+        const uint4 *cat = a.corefx + F.atom_off;
+        const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
+        int cnt[3], start[3];
+        bool hit = false;
+        for (int t = 0; t < NS && !hit; ++t) {
+            for (int d = 0; d < 3; ++d) {
+                const int nc = F.cg_nc[d];
+                const int h = (int)(((uint64_t)sp[t][d] * (uint64_t)nc) >> 32);
+                cnt[d] = min(2 * B.reach[d] + 1, nc);
+                start[d] = cnt[d] == nc ? 0 : (h - B.reach[d] + nc) % nc;
+            }
+            for (int i2 = 0; i2 < cnt[2] && !hit; ++i2) {
+                const int c2 = (start[2] + i2) % F.cg_nc[2];
+                for (int i1 = 0; i1 < cnt[1] && !hit; ++i1) {
+                    const int c1 = (start[1] + i1) % F.cg_nc[1];
+                    for (int i0 = 0; i0 < cnt[0] && !hit; ++i0) {
+                        const int c = (start[0] + i0) % F.cg_nc[0] + F.cg_nc[0] * (c1 + F.cg_nc[1] * c2);
+                        for (int jj = cs[c]; jj < cs[c + 1]; ++jj) {
+                            const uint4 at = cat[jj];
+                            const float dx = (float)(int32_t)(sp[t][0] - at.x);
+                            const float dy = (float)(int32_t)(sp[t][1] - at.y);
+                            const float dz = (float)(int32_t)(sp[t][2] - at.z);
+                            const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
+                            const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
+                            const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
+                            if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (hit) {
+            a.key[g - a.g0] = -1;
+            a.dU[g - a.out_base] = INFINITY;
+            return;
+        }
+    }
     const int bx = min(B.nb[0] - 1, (int)(f[0] * B.nb[0]));
     const int by = min(B.nb[1] - 1, (int)(f[1] * B.nb[1]));
     const int bz = min(B.nb[2] - 1, (int)(f[2] * B.nb[2]));
@@ -89,6 +133,7 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.g1 - a.g0) return;
     const int k = a.key[i];
+    if (k < 0) return;  // rejected in phase 0
     a.order[a.bin_start[k] + atomicAdd(&a.cursor[k], 1)] = (int)i;
 }
 

@@ -10,13 +10,13 @@ from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
 from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
                         reciprocal_basis, replicate, total_charge, volume)
-from .widom import (KB, KE, PASCAL, WidomGrad, WidomResult, default_run, last_timing, sys_of_index, system_counts,
-                    widom, widom_energies, widom_grad, widom_poses)
+from .widom import (KB, KE, PASCAL, WidomGrad, WidomResult, default_run, hardcore_flags, last_timing, sys_of_index,
+                    system_counts, widom, widom_energies, widom_grad, widom_poses)
 
 __all__ = [
     "ArgumentError", "DeviceError", "DimensionMismatch", "device_name", "synchronize", "FrameworkBatch", "nframeworks", "EwaldParams",
     "ForceField", "Guest", "read_forcefield", "read_guest", "tail_coefficient", "tail_delta", "typeindex", "Framework",
     "cartesian", "cell_matrix", "min_multiplicity", "natoms", "perpendicular_lengths", "read_cif", "reciprocal_basis",
     "replicate", "total_charge", "volume", "KB", "KE", "PASCAL", "WidomResult", "default_run", "last_timing",
-    "sys_of_index", "system_counts", "widom", "widom_energies", "widom_grad", "widom_poses", "WidomGrad",
+    "sys_of_index", "system_counts", "widom", "widom_energies", "widom_grad", "widom_poses", "WidomGrad", "hardcore_flags",
 ]

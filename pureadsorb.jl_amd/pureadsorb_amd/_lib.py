@@ -19,6 +19,7 @@ PA_ERR_INTERNAL = 4
 
 PA_PREC_F64 = 0
 PA_PREC_MIXED = 1
+PA_SINGLE_PHASE = 0x100
 
 PA_BATCH_FULLK = 1
 PA_BATCH_HOST_ONLY = 2
@@ -26,7 +27,7 @@ PA_BATCH_HOST_ONLY = 2
 # Every symbol include/pureadsorb_amd.h declares.
 EXPORTS = (
     "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
-    "pa_batch_framework_info", "pa_widom", "pa_widom_grad", "pa_widom_energies", "pa_widom_poses", "pa_last_timing",
+    "pa_batch_framework_info", "pa_widom", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
 )
 
@@ -92,6 +93,7 @@ class PaFrameworkInfo(C.Structure):
         ("natoms", C.c_int32), ("nk", C.c_int32), ("nk_full", C.c_int32),
         ("volume", C.c_double), ("alpha", C.c_double),
         ("constant_offset", C.c_double), ("self_term_halfrange", C.c_double),
+        ("hardcore_bound", C.c_double),
     ]
 
 
@@ -134,6 +136,8 @@ def lib():
     L.pa_widom_grad.restype = C.c_int
     L.pa_widom_grad.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32,
                                 C.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    L.pa_hardcore_flags.restype = C.c_int
+    L.pa_hardcore_flags.argtypes = [vp, C.c_int32, C.c_uint64, C.c_double, C.c_int64, C.c_int64, vp]
     L.pa_widom_energies.restype = C.c_int
     L.pa_widom_energies.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int64, C.c_int64, C.c_uint32, vp]
     L.pa_widom_poses.restype = C.c_int

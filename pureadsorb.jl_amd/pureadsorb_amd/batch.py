@@ -95,6 +95,7 @@ class FrameworkBatch:
         self.alphas = np.array([i.alpha for i in infos])
         self.constant_offset = np.array([i.constant_offset for i in infos])
         self.self_term_halfrange = np.array([i.self_term_halfrange for i in infos])
+        self.hardcore_bound = np.array([i.hardcore_bound for i in infos])
 
     def close(self):
         h = getattr(self, "_h", None)

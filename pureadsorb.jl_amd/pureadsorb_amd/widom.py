@@ -61,12 +61,16 @@ def _precision(p) -> int:
 
 def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str = "mi355x", seed: int = 0,
           chunk: int | None = None, nblocks: int = 10, run: int | None = None,
-          precision: str = "mixed") -> list[WidomResult]:
+          precision: str = "mixed", single_phase: bool = False) -> list[WidomResult]:
     """Widom test-particle insertion on the device; one WidomResult per system.
 
     Insertion j (0-based) of system s takes its pose from ChainRNG(splitmix64(seed, s+1), j+1, 0),
     so results are independent of `chunk` (insertions per energy launch; the library picks
     one when None) and, apart from the per-system counts, of `run`.
+
+    The mixed path first rejects insertions whose Boltzmann weight a rigorous bound proves to be
+    exactly 0.0 in Float64 (src/widom.jl:99-151, src/reject.jl); `single_phase=True` evaluates
+    every insertion instead (the reference's `widom_singlephase`). Results are the same.
     """
     if backend not in ("mi355x", "hip"):
         raise _lib.ArgumentError(f"backend={backend!r}: this backend runs on the MI355X ('mi355x')")
@@ -82,7 +86,7 @@ def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str 
     out = (_lib.PaWidomResult * batch.nsys)()
     _lib.check(_lib.lib().pa_widom(batch.handle, C.byref(g.s), T, int(ninsert), int(seed) & (2**64 - 1), int(nblocks),
                                    0 if run is None else int(run), 0 if chunk is None else int(chunk),
-                                   _precision(precision), out))
+                                   _precision(precision) | (_lib.PA_SINGLE_PHASE if single_phase else 0), out))
     return [WidomResult(r.mu_ex, r.mu_ex_err, r.K_H, r.K_H_err, r.q_st, r.q_st_err, int(r.nsamples), int(r.nblocks),
                         r.ln_K_H) for r in out]
 
@@ -146,6 +150,16 @@ def widom_energies(batch: FrameworkBatch, sys: int, seed: int, j0: int, n: int, 
     _lib.check(_lib.lib().pa_widom_energies(batch.handle, int(sys), int(seed) & (2**64 - 1), int(j0), int(n),
                                             _precision(precision), dU.ctypes.data))
     return dU[:n]
+
+
+def hardcore_flags(batch: FrameworkBatch, sys: int, seed: int, T, j0: int, n: int) -> np.ndarray:
+    """Phase-0 flags (1 = rejected without an energy) of insertions j0..j0+n-1 of system `sys`
+    at temperature T, from the mixed path's hard-core check (src/widom.jl:99-151)."""
+    T = float(getattr(T, "magnitude", T))
+    flags = np.zeros(max(n, 1), dtype=np.uint8)
+    _lib.check(_lib.lib().pa_hardcore_flags(batch.handle, int(sys), int(seed) & (2**64 - 1), T, int(j0), int(n),
+                                            flags.ctypes.data))
+    return flags[:n]
 
 
 def widom_poses(batch: FrameworkBatch, sys: int, seed: int, j0: int, n: int) -> tuple[np.ndarray, np.ndarray]:

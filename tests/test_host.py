@@ -238,3 +238,15 @@ def test_grad_rejects_non_lb_tables_and_shuffled_images():
     b3 = pa.FrameworkBatch([fw], ff2, g, pa.EwaldParams(11.0), host_only=True)
     with pytest.raises(pa.ArgumentError, match="Lorentz-Berthelot"):
         pa.widom_grad(b3, g, T=298.15, ninsert=1000, nblocks=4)
+
+
+def test_hardcore_bound_never_below_the_reference_bound(config1):
+    """B_s with the per-atom terms rounded up on the |K| grid: never below the oracle's exact
+    reference bound, and within the grid's 2 % of it."""
+    import oracle as O
+    from conftest import oracle_for
+    fw, ff, g, ew = config1
+    b = pa.FrameworkBatch([fw], ff, g, ew, host_only=True)
+    ref = oracle_for(fw, ff, g, ew).hardcore_bound()
+    print(b.hardcore_bound[0], ref)
+    assert ref <= b.hardcore_bound[0] <= 1.021 * ref
