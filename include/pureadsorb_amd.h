@@ -171,6 +171,87 @@ int pa_hardcore_flags(pa_batch *batch, int32_t sys, uint64_t seed, double T, int
 /* Device-drawn poses of the same insertions: pos (n x 3, Å) and quaternions (n x 4, x y z w). */
 int pa_widom_poses(pa_batch *batch, int32_t sys, uint64_t seed, int64_t j0, int64_t n, double *pos, double *quat);
 
+/* ---------------------------------------------------------------------------------------------
+ * Move-energy path: NVT and μVT Monte Carlo chains, one per system of a batch built with
+ * PA_BATCH_FULLK. Every chain runs on the device; moves are FP64.
+ * ------------------------------------------------------------------------------------------- */
+#define PA_MOVE_TRANSLATION 1 /* src/moves.jl:69-71 */
+#define PA_MOVE_ROTATION 2
+#define PA_MOVE_REINSERTION 3
+#define PA_MOVE_INSERTION 4
+#define PA_MOVE_DELETION 5
+
+typedef struct pa_state pa_state;
+
+/* Replaces SystemState(batch, guest, ncounts, ff; T, seed, capacities) (src/state.jl:328-446):
+ * ncounts[n] initial guests (uniform poses redrawn until they clear phase 0 at T; pose of slot i,
+ * attempt t from ChainRNG(splitmix64(seed ^ 0x1A17A1D05E5A3E11, n+1), i+1, t)) in capacities[n]
+ * slots (NULL: capacities = ncounts), chain seeds splitmix64(seed, n+1), counters 0, running
+ * structure factor, cached host energies and total energy set from scratch in FP64. */
+int pa_state_create(pa_batch *batch, const pa_guest *guest, const int32_t *ncounts, const int32_t *capacities,
+                    double T, uint64_t seed, pa_state **out);
+void pa_state_destroy(pa_state *state);
+
+/* mc_step! (src/moves.jl:577-645): one attempt of `movetype` (translation, rotation or
+ * reinsertion) on every chain; step_trans (Å, Gaussian s.d.) and step_rot (in [0, 1]) per chain. */
+int pa_mc_step(pa_state *state, int32_t movetype, const double *step_trans, const double *step_rot, double T);
+/* mc_insert! / mc_delete! (src/moves.jl:1195-1290): one exchange attempt on every chain at
+ * fugacity[n] (Pa). An insertion the Metropolis test accepts on a chain at capacity is an error. */
+int pa_mc_insert(pa_state *state, const double *fugacity_pa, double T);
+int pa_mc_delete(pa_state *state, const double *fugacity_pa, double T);
+/* `nmoves` attempts of one move type back to back in one launch (no counterpart: the reference
+ * launches three kernels per attempt). trace (nullable): nsys x nmoves x 4 doubles per attempt:
+ * move type, slot within the system, dU (eV), accepted (0/1). */
+int pa_mc_moves(pa_state *state, int32_t movetype, int64_t nmoves, const double *step_trans, const double *step_rot,
+                const double *fugacity_pa, double T, double *trace);
+
+typedef struct pa_chain_info {
+    int32_t occupancy, capacity;
+    double energy;            /* running total energy, eV */
+    double energy_abs_accum;  /* sum of |dU| of accepted moves since the last audit */
+    int64_t rng_counter;
+    int32_t accepted[5], attempted[5]; /* per move type, PA_MOVE_* order */
+    double const_p, const_q;  /* exchange constant term p + q N (src/moves.jl:728-773) */
+} pa_chain_info;
+
+int pa_state_info(const pa_state *state, int32_t sys, pa_chain_info *out);
+/* Live guests of system `sys`: refpoints occupancy x 3 (Å), orientations occupancy x 4 (x y z w),
+ * cached host-guest energies (eV); each nullable. */
+int pa_state_guests(const pa_state *state, int32_t sys, double *refpoints, double *orientations, double *host_energy);
+/* Running structure factor of system `sys`: nk x 2 (re, im). */
+int pa_state_sk(const pa_state *state, int32_t sys, double *sk);
+/* total_energy (src/guest.jl:293-343) of every system from its poses, FP64, with the structure
+ * factor rebuilt from the poses. */
+int pa_total_energy(pa_state *state, double *energy);
+/* audit_energy! (src/audit.jl:97-165) on every system; on success resets the running structure
+ * factor and energy to the rebuilt values. discrepancy (nullable): recomputed - running, eV. */
+int pa_audit_energy(pa_state *state, double *discrepancy);
+
+/* GCMC driver (src/gcmc.jl:161-337). A cycle of chain n is max(its occupancy, min_cycle_length)
+ * attempts; attempt c of chain n draws its move type from ChainRNG(chain seed, c, 1) word 0 with
+ * probabilities p_translation, p_rotation, p_reinsertion, p_exchange/2 (insertion),
+ * p_exchange/2 (deletion); its proposal and acceptance from ChainRNG(chain seed, c, 0) as in
+ * pa_mc_step. Audit every n_audit cycles; counters reset at the start of production. */
+typedef struct pa_gcmc_params {
+    double T;
+    int64_t n_warmup, n_production, n_audit, min_cycle_length;
+    const double *fugacity_pa, *step_trans, *step_rot; /* per system */
+    double p_translation, p_rotation, p_reinsertion, p_exchange;
+    int32_t nblocks;
+} pa_gcmc_params;
+
+/* GCMCResult (src/gcmc.jl:32-43) plus the production move counters. */
+typedef struct pa_gcmc_result {
+    double loading, loading_err, energy, energy_err, q_st, q_st_err, corr_UN;
+    int32_t max_occupancy, capacity;
+    int64_t ncycles;
+    int64_t accepted[5], attempted[5];
+} pa_gcmc_result;
+
+int pa_gcmc_run(pa_state *state, const pa_gcmc_params *params, pa_gcmc_result *out);
+/* Per-cycle production samples of the last pa_gcmc_run: total energy (eV) and loading. */
+int pa_gcmc_samples(const pa_state *state, int32_t sys, int64_t n, double *energy, double *loading);
+
 /* Timing of the last pa_widom or pa_widom_grad call on `batch`. */
 int pa_last_timing(const pa_batch *batch, pa_timing *out);
 

@@ -1293,3 +1293,385 @@ void or_energies_twophase(const or_system *s, uint64_t seed, int64_t sysidx, int
     }
     free(flags);
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Move-energy path (src/guest.jl, src/moves.jl, src/state.jl, src/gcmc.jl): one Monte Carlo
+ * chain over a system built with every k-vector (fullk), FP64, serial. The move logic and
+ * the per-attempt random keying follow the reference's select_and_propose /
+ * evaluate_*_kernel! / decide_*_kernel! (src/moves.jl:275-288, 379-553, 827-1128): attempt c
+ * of a chain with seed S draws from ChainRNG(S, c, 0) -- guest index (Lemire), proposal,
+ * acceptance uniform.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+    const or_system *s;
+    int cap, occ;
+    double *pos, *quat, *host_e;  /* cap x 3, cap x 4, cap */
+    double *Sre, *Sim;            /* running structure factor over the full k table */
+    double energy, energy_abs;
+    uint64_t seed, counter;
+    double const_p, const_q;
+    int acc[5], att[5];
+} or_mc;
+
+static void or_sites_at(const or_system *s, const double p[3], const double q[4], double out[][3])
+{
+    for (int a = 0; a < s->nsites; ++a) {
+        double r[3];
+        or_rotate(q, s->gsite + 3 * a, r);
+        for (int d = 0; d < 3; ++d) out[a][d] = p[d] + r[d];
+    }
+}
+
+static double or_mi_r2(const or_system *s, const double a[3], const double b[3])
+{
+    double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]}, m[3];
+    or_minimum_image(s->A, s->invA, d, m);
+    return or_dot3(m, m);
+}
+
+/* LJ inside rc_lj plus KE q_a q_b erfc(alpha r)/r inside rc_ew, one site pair. */
+static double or_site_pair(const or_system *s, double r2, int ta, int tb, double qa, double qb)
+{
+    double e = 0.0;
+    size_t p = (size_t)ta + (size_t)s->ntypes * tb;
+    if (r2 < s->rc_lj * s->rc_lj) {
+        double x = s->sigma[p] * s->sigma[p] / r2;
+        x = x * x * x;
+        e += 4.0 * s->eps[p] * (x * x - x);
+    }
+    if (s->charged && r2 < s->rc_ew * s->rc_ew) {
+        double r = sqrt(r2);
+        e += OR_KE * qa * qb * or_pair_erfc_f64(s->alpha * r) / r;
+    }
+    return e;
+}
+
+/* host_guest_realspace_energy (src/guest.jl:176-222), per-site minimum image. */
+double or_host_guest(const or_system *s, const double p[3], const double q[4])
+{
+    double sites[OR_MAXSITES][3], e = 0.0;
+    or_sites_at(s, p, q, sites);
+    for (int j = 0; j < s->natoms; ++j)
+        for (int a = 0; a < s->nsites; ++a)
+            e += or_site_pair(s, or_mi_r2(s, sites[a], s->pos + 3 * j), s->gtype[a], s->type[j], s->gq[a], s->q[j]);
+    return e;
+}
+
+/* guest_pair_realspace_energy (src/guest.jl:49-71), LJ + KE E_sr. */
+static double or_guest_pair(const or_system *s, double sa[][3], double sb[][3])
+{
+    double e = 0.0;
+    for (int a = 0; a < s->nsites; ++a)
+        for (int b = 0; b < s->nsites; ++b)
+            e += or_site_pair(s, or_mi_r2(s, sa[a], sb[b]), s->gtype[a], s->gtype[b], s->gq[a], s->gq[b]);
+    return e;
+}
+
+/* KE (self + exclusion) of one guest (guest_self_terms, src/guest.jl:26-42). */
+double or_guest_self(const or_system *s)
+{
+    double q2 = 0.0, excl = 0.0;
+    for (int a = 0; a < s->nsites; ++a) q2 += s->gq[a] * s->gq[a];
+    for (int a = 0; a < s->nsites; ++a)
+        for (int c = a + 1; c < s->nsites; ++c) {
+            double d[3] = {s->gsite[3 * a] - s->gsite[3 * c], s->gsite[3 * a + 1] - s->gsite[3 * c + 1],
+                           s->gsite[3 * a + 2] - s->gsite[3 * c + 2]};
+            double r = or_norm3(d);
+            excl -= s->gq[a] * s->gq[c] * (1.0 - or_erfc_dev(s->alpha * r)) / r;
+        }
+    return OR_KE * (-s->alpha / sqrt(OR_PI) * q2 + excl);
+}
+
+/* tail and net-charge terms at N guests (src/guest.jl:9-14, 332-341). */
+double or_guest_tail(const or_system *s, int64_t N)
+{
+    int64_t *counts = calloc((size_t)s->ntypes, sizeof(int64_t)), *g = calloc((size_t)s->ntypes, sizeof(int64_t));
+    for (int j = 0; j < s->natoms; ++j) counts[s->type[j]] += 1;
+    for (int a = 0; a < s->nsites; ++a) g[s->gtype[a]] += N;
+    double t = or_tail_delta(s->ntypes, s->sigma, s->eps, s->rc_lj, s->tail, counts, g, s->V);
+    free(counts); free(g);
+    return t;
+}
+
+double or_net(const or_system *s, int64_t N)
+{
+    double Qh = 0.0, Qg = 0.0;
+    for (int j = 0; j < s->natoms; ++j) Qh += s->q[j];
+    for (int a = 0; a < s->nsites; ++a) Qg += s->gq[a];
+    return -OR_KE * OR_PI / (2.0 * s->V * s->alpha * s->alpha) * ((Qh + N * Qg) * (Qh + N * Qg) - Qh * Qh);
+}
+
+/* exchange_constant_term (src/moves.jl:728-746) */
+double or_exchange_constant(const or_system *s, int64_t N)
+{
+    return or_guest_self(s) + or_guest_tail(s, N + 1) - or_guest_tail(s, N) + or_net(s, N + 1) - or_net(s, N);
+}
+
+static void or_mc_rebuild_sk(const or_mc *m, double *Sre, double *Sim)
+{
+    const or_system *s = m->s;
+    for (int k = 0; k < s->nkfull; ++k) { Sre[k] = s->Sre_full[k]; Sim[k] = s->Sim_full[k]; }
+    if (!s->charged) return;
+    double sites[OR_MAXSITES][3];
+    for (int i = 0; i < m->occ; ++i) {
+        or_sites_at(s, m->pos + 3 * i, m->quat + 4 * i, sites);
+        for (int k = 0; k < s->nkfull; ++k)
+            for (int a = 0; a < s->nsites; ++a) {
+                double ph = or_dot3(s->kfull + 3 * k, sites[a]);
+                Sre[k] += s->gq[a] * cos(ph);
+                Sim[k] += s->gq[a] * sin(ph);
+            }
+    }
+}
+
+/* total_energy (src/guest.jl:293-343) from the poses, with the structure factor rebuilt. */
+double or_mc_total_energy(const or_mc *m)
+{
+    const or_system *s = m->s;
+    double e = 0.0;
+    for (int i = 0; i < m->occ; ++i) e += or_host_guest(s, m->pos + 3 * i, m->quat + 4 * i);
+    double si[OR_MAXSITES][3], sj[OR_MAXSITES][3];
+    for (int i = 0; i < m->occ; ++i) {
+        or_sites_at(s, m->pos + 3 * i, m->quat + 4 * i, si);
+        for (int j = i + 1; j < m->occ; ++j) {
+            or_sites_at(s, m->pos + 3 * j, m->quat + 4 * j, sj);
+            e += or_guest_pair(s, si, sj);
+        }
+    }
+    double *Sre = malloc(sizeof(double) * (size_t)(s->nkfull + 1)), *Sim = malloc(sizeof(double) * (size_t)(s->nkfull + 1));
+    or_mc_rebuild_sk(m, Sre, Sim);
+    double rec = 0.0;
+    for (int k = 0; k < s->nkfull; ++k)
+        rec += s->kpref_full[k] * ((Sre[k] * Sre[k] + Sim[k] * Sim[k]) -
+                                   (s->Sre_full[k] * s->Sre_full[k] + s->Sim_full[k] * s->Sim_full[k]));
+    free(Sre); free(Sim);
+    e += OR_KE * rec;
+    e += m->occ * or_guest_self(s) + or_guest_tail(s, m->occ) + or_net(s, m->occ);
+    return e;
+}
+
+or_mc *or_mc_new(const or_system *s, int cap, uint64_t chain_seed, int occ, const double *pos, const double *quat)
+{
+    or_mc *m = calloc(1, sizeof(or_mc));
+    m->s = s;
+    m->cap = cap;
+    m->occ = occ;
+    m->pos = calloc((size_t)(cap + 1) * 3, sizeof(double));
+    m->quat = calloc((size_t)(cap + 1) * 4, sizeof(double));
+    m->host_e = calloc((size_t)(cap + 1), sizeof(double));
+    m->Sre = calloc((size_t)(s->nkfull + 1), sizeof(double));
+    m->Sim = calloc((size_t)(s->nkfull + 1), sizeof(double));
+    for (int i = 0; i < cap; ++i) m->quat[4 * i + 3] = 1.0;
+    for (int i = 0; i < occ; ++i) {
+        memcpy(m->pos + 3 * i, pos + 3 * i, 3 * sizeof(double));
+        memcpy(m->quat + 4 * i, quat + 4 * i, 4 * sizeof(double));
+        m->host_e[i] = or_host_guest(s, m->pos + 3 * i, m->quat + 4 * i);
+    }
+    or_mc_rebuild_sk(m, m->Sre, m->Sim);
+    m->energy = or_mc_total_energy(m);
+    m->seed = chain_seed;
+    m->const_p = or_exchange_constant(s, 0);
+    m->const_q = or_exchange_constant(s, 1) - m->const_p;
+    return m;
+}
+
+void or_mc_free(or_mc *m)
+{
+    if (!m) return;
+    free(m->pos); free(m->quat); free(m->host_e); free(m->Sre); free(m->Sim);
+    free(m);
+}
+
+static double or_rand_normal(or_chain *c)
+{
+    double u1 = or_unit_f64(or_next_bits(c)), u2 = or_unit_f64(or_next_bits(c));
+    double r = sqrt(-2.0 * log(1.0 - u1)), sn, cs;
+    or_sincospi(2.0 * u2, &sn, &cs);
+    return r * cs;
+}
+
+static void or_rand_quat(or_chain *c, double q[4])
+{
+    double u1 = or_unit_f64(or_next_bits(c)), u2 = or_unit_f64(or_next_bits(c)), u3 = or_unit_f64(or_next_bits(c));
+    or_shoemake(u1, u2, u3, q);
+}
+
+/* qmul, quaternion_pow (src/moves.jl:81-110) */
+static void or_qmul(const double q2[4], const double q1[4], double o[4])
+{
+    o[3] = q2[3] * q1[3] - (q2[0] * q1[0] + q2[1] * q1[1] + q2[2] * q1[2]);
+    o[0] = q2[3] * q1[0] + q1[3] * q2[0] + (q2[1] * q1[2] - q2[2] * q1[1]);
+    o[1] = q2[3] * q1[1] + q1[3] * q2[1] + (q2[2] * q1[0] - q2[0] * q1[2]);
+    o[2] = q2[3] * q1[2] + q1[3] * q2[2] + (q2[0] * q1[1] - q2[1] * q1[0]);
+}
+
+static void or_qpow(const double q[4], double t, double o[4])
+{
+    double w = q[3] < -1.0 ? -1.0 : (q[3] > 1.0 ? 1.0 : q[3]);
+    double th = 2.0 * acos(w), s2 = 1.0 - w * w, s = sqrt(s2 > 0.0 ? s2 : 0.0);
+    double ax[3] = {1.0, 0.0, 0.0};
+    if (s > 2.220446049250313e-16) { ax[0] = q[0] / s; ax[1] = q[1] / s; ax[2] = q[2] / s; }
+    double half = t * th / 2.0, sh = sin(half);
+    o[0] = ax[0] * sh; o[1] = ax[1] * sh; o[2] = ax[2] * sh; o[3] = cos(half);
+}
+
+/* One attempt of `type` (1 translation, 2 rotation, 3 reinsertion, 4 insertion, 5 deletion).
+ * fug: eV/Å^3. Returns accepted (0/1); *dU receives the energy change. Returns -1 when an
+ * insertion the Metropolis test accepts finds the chain at capacity (nothing is changed). */
+int or_mc_attempt(or_mc *m, int type, double step_t, double step_r, double fug, double kT, double *dU_out,
+                  int *slot_out)
+{
+    const or_system *s = m->s;
+    or_chain rng = or_chain_new(m->seed, m->counter, 0);
+    int slot = 0;
+    double op[3] = {0, 0, 0}, oq[4] = {0, 0, 0, 1}, np[3] = {0, 0, 0}, nq[4] = {0, 0, 0, 1};
+    if (type != 4) {
+        int64_t idx = or_rand_range(or_next_bits(&rng), m->occ > 0 ? m->occ : 1);
+        slot = (int)(idx - 1);
+        if (slot > m->cap - 1) slot = m->cap > 0 ? m->cap - 1 : 0;
+        memcpy(op, m->pos + 3 * slot, sizeof op);
+        memcpy(oq, m->quat + 4 * slot, sizeof oq);
+    }
+    if (type == 1) {
+        double dx = or_rand_normal(&rng), dy = or_rand_normal(&rng), dz = or_rand_normal(&rng);
+        np[0] = op[0] + step_t * dx; np[1] = op[1] + step_t * dy; np[2] = op[2] + step_t * dz;
+        memcpy(nq, oq, sizeof nq);
+    } else if (type == 2) {
+        double qr[4], qp[4];
+        or_rand_quat(&rng, qr);
+        or_qpow(qr, step_r, qp);
+        or_qmul(qp, oq, nq);
+        double nrm = sqrt(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
+        for (int i = 0; i < 4; ++i) nq[i] /= nrm;
+        memcpy(np, op, sizeof np);
+    } else if (type == 3 || type == 4) {
+        double f[3];
+        for (int i = 0; i < 3; ++i) f[i] = or_unit_f64(or_next_bits(&rng));
+        or_matvec(s->A, f, np);
+        or_rand_quat(&rng, nq);
+    }
+    double u = or_unit_f64(or_next_bits(&rng));
+    int has_old = type != 4, has_new = type != 5;
+    double os[OR_MAXSITES][3], ns[OR_MAXSITES][3], sj[OR_MAXSITES][3];
+    or_sites_at(s, op, oq, os);
+    or_sites_at(s, np, nq, ns);
+    double e_new = has_new ? or_host_guest(s, np, nq) : 0.0;
+    double gg = 0.0;
+    for (int j = 0; j < m->occ; ++j) {
+        if (has_old && j == slot) continue;
+        or_sites_at(s, m->pos + 3 * j, m->quat + 4 * j, sj);
+        if (has_new) gg += or_guest_pair(s, ns, sj);
+        if (type <= 3) gg -= or_guest_pair(s, os, sj);
+        else if (type == 5) gg += or_guest_pair(s, os, sj);
+    }
+    double rec = 0.0;
+    double *dre = malloc(sizeof(double) * (size_t)(s->nkfull + 1)), *dim = malloc(sizeof(double) * (size_t)(s->nkfull + 1));
+    for (int k = 0; k < s->nkfull; ++k) {
+        double re = 0.0, im = 0.0;
+        if (s->charged)
+            for (int a = 0; a < s->nsites; ++a) {
+                if (has_new) {
+                    double ph = or_dot3(s->kfull + 3 * k, ns[a]);
+                    re += s->gq[a] * cos(ph); im += s->gq[a] * sin(ph);
+                }
+                if (has_old) {
+                    double ph = or_dot3(s->kfull + 3 * k, os[a]);
+                    re -= s->gq[a] * cos(ph); im -= s->gq[a] * sin(ph);
+                }
+            }
+        dre[k] = re; dim[k] = im;
+        rec += s->kpref_full[k] * (2.0 * (m->Sre[k] * re + m->Sim[k] * im) + (re * re + im * im));
+    }
+    rec *= OR_KE;
+    double dU;
+    int accept, status = 0;
+    if (type <= 3) {
+        dU = (e_new - m->host_e[slot]) + gg + rec;
+        accept = isfinite(dU) && (-dU / kT > log(u)) && m->occ > 0;
+    } else if (type == 4) {
+        dU = e_new + gg + rec + (m->const_p + m->const_q * m->occ);
+        double lp = log(fug * s->V) - log((double)m->occ + 1.0) - log(kT);
+        int would = isfinite(dU) && (lp - dU / kT > log(u));
+        accept = would && m->occ < m->cap;
+        if (would && !accept) status = -1;
+    } else {
+        int occ_eff = m->occ > 0 ? m->occ : 1;
+        dU = -(m->host_e[slot] + gg + (m->const_p + m->const_q * (m->occ - 1))) + rec;
+        double lp = -(log(fug * s->V) - log((double)(occ_eff - 1) + 1.0) - log(kT));
+        accept = isfinite(dU) && (lp - dU / kT > log(u)) && m->occ > 0;
+    }
+    if (accept) {
+        if (type <= 3) {
+            memcpy(m->pos + 3 * slot, np, sizeof np);
+            memcpy(m->quat + 4 * slot, nq, sizeof nq);
+            m->host_e[slot] = e_new;
+        } else if (type == 4) {
+            memcpy(m->pos + 3 * m->occ, np, sizeof np);
+            memcpy(m->quat + 4 * m->occ, nq, sizeof nq);
+            m->host_e[m->occ] = e_new;
+            m->occ += 1;
+        } else {
+            int hi = m->occ - 1;
+            if (slot != hi) {
+                memcpy(m->pos + 3 * slot, m->pos + 3 * hi, 3 * sizeof(double));
+                memcpy(m->quat + 4 * slot, m->quat + 4 * hi, 4 * sizeof(double));
+                m->host_e[slot] = m->host_e[hi];
+            }
+            m->occ -= 1;
+        }
+        m->energy += dU;
+        m->energy_abs += fabs(dU);
+        for (int k = 0; k < s->nkfull; ++k) { m->Sre[k] += dre[k]; m->Sim[k] += dim[k]; }
+        m->acc[type - 1] += 1;
+    }
+    free(dre); free(dim);
+    m->att[type - 1] += 1;
+    m->counter += 1;
+    if (dU_out) *dU_out = dU;
+    if (slot_out) *slot_out = slot;
+    return status < 0 ? status : accept;
+}
+
+/* One GCMC cycle of the chain: max(occupancy, min_cycle) attempts, the move type of attempt c
+ * from ChainRNG(seed, c, 1) word 0 against the cumulative probabilities pcum[5]. Returns the
+ * number of attempts, or -1 on a capacity hit. */
+int64_t or_mc_cycle(or_mc *m, const double pcum[5], int64_t min_cycle, double step_t, double step_r, double fug,
+                    double kT)
+{
+    int64_t len = m->occ > min_cycle ? m->occ : min_cycle;
+    for (int64_t i = 0; i < len; ++i) {
+        or_chain r1 = or_chain_new(m->seed, m->counter, 1);
+        double v = or_unit_f64(or_next_bits(&r1));
+        int t = 5;
+        for (int k = 0; k < 5; ++k)
+            if (v < pcum[k]) { t = k + 1; break; }
+        if (or_mc_attempt(m, t, step_t, step_r, fug, kT, NULL, NULL) < 0) return -1;
+    }
+    return len;
+}
+
+/* accessors */
+void or_mc_scalars(const or_mc *m, double out[6])
+{
+    out[0] = m->occ; out[1] = m->energy; out[2] = m->energy_abs; out[3] = (double)m->counter;
+    out[4] = m->const_p; out[5] = m->const_q;
+}
+
+void or_mc_counts(const or_mc *m, int *acc, int *att)
+{
+    for (int t = 0; t < 5; ++t) { acc[t] = m->acc[t]; att[t] = m->att[t]; }
+}
+
+void or_mc_guests(const or_mc *m, double *pos, double *quat, double *host_e)
+{
+    for (int i = 0; i < m->occ; ++i) {
+        if (pos) memcpy(pos + 3 * i, m->pos + 3 * i, 3 * sizeof(double));
+        if (quat) memcpy(quat + 4 * i, m->quat + 4 * i, 4 * sizeof(double));
+        if (host_e) host_e[i] = m->host_e[i];
+    }
+}
+
+void or_mc_sk(const or_mc *m, double *re, double *im)
+{
+    for (int k = 0; k < m->s->nkfull; ++k) { re[k] = m->Sre[k]; im[k] = m->Sim[k]; }
+}

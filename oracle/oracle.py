@@ -136,6 +136,35 @@ def lib():
     L.or_energies_twophase.restype = None
     L.or_energies_twophase.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp, _dp, C.c_void_p,
                                        C.c_int]
+    L.or_host_guest.restype = C.c_double
+    L.or_host_guest.argtypes = [C.c_void_p, _dp, _dp]
+    L.or_guest_self.restype = C.c_double
+    L.or_guest_self.argtypes = [C.c_void_p]
+    L.or_guest_tail.restype = C.c_double
+    L.or_guest_tail.argtypes = [C.c_void_p, C.c_int64]
+    L.or_net.restype = C.c_double
+    L.or_net.argtypes = [C.c_void_p, C.c_int64]
+    L.or_exchange_constant.restype = C.c_double
+    L.or_exchange_constant.argtypes = [C.c_void_p, C.c_int64]
+    L.or_mc_new.restype = C.c_void_p
+    L.or_mc_new.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p]
+    L.or_mc_free.restype = None
+    L.or_mc_free.argtypes = [C.c_void_p]
+    L.or_mc_total_energy.restype = C.c_double
+    L.or_mc_total_energy.argtypes = [C.c_void_p]
+    L.or_mc_attempt.restype = C.c_int
+    L.or_mc_attempt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.or_mc_cycle.restype = C.c_int64
+    L.or_mc_cycle.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.or_mc_scalars.restype = None
+    L.or_mc_scalars.argtypes = [C.c_void_p, _dp]
+    L.or_mc_counts.restype = None
+    L.or_mc_counts.argtypes = [C.c_void_p, _ip, _ip]
+    L.or_mc_guests.restype = None
+    L.or_mc_guests.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_mc_sk.restype = None
+    L.or_mc_sk.argtypes = [C.c_void_p, _dp, _dp]
     L.or_widom_grad.restype = C.c_int
     L.or_widom_grad.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp]
@@ -441,3 +470,76 @@ class OracleSystem:
             raise RuntimeError(f"or_widom_grad: status {st}")
         return {"ln_K_H": out[0], "q_st": out[1], "dlnkh_deps": lj[0][:nt], "dlnkh_dsig": lj[1][:nt],
                 "dqst_deps": lj[2][:nt], "dqst_dsig": lj[3][:nt], "dlnkh_dq": dq[0][:na], "dqst_dq": dq[1][:na]}
+
+
+PASCAL = 1.0 / (1.0e30 * 1.6021766208e-19)  # Pa -> eV/Å^3, src/constants.jl:10
+KB = 1.38064852e-23 / 1.6021766208e-19
+
+
+class OracleMC:
+    """One FP64 Monte Carlo chain over an OracleSystem built from every k-vector (the system's
+    full table): or_mc_* (the reference's move path restated serially)."""
+
+    def __init__(self, system: "OracleSystem", capacity: int, chain_seed: int, pos=None, quat=None):
+        self.system = system
+        pos = np.zeros((0, 3)) if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        quat = np.zeros((0, 4)) if quat is None else np.ascontiguousarray(quat, dtype=np.float64).reshape(-1, 4)
+        self.capacity = capacity
+        self._p = pos.reshape(-1).copy()
+        self._q = quat.reshape(-1).copy()
+        self._h = lib().or_mc_new(system._h, capacity, chain_seed & (2**64 - 1), len(pos),
+                                  self._p.ctypes.data if len(pos) else None, self._q.ctypes.data if len(pos) else None)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_mc_free(self._h)
+            self._h = None
+
+    def attempt(self, movetype, step_trans=0.0, step_rot=0.0, fugacity_pa=1.0, T=298.15):
+        """(status, dU, slot): status 1 accepted, 0 rejected, -1 capacity hit."""
+        dU = C.c_double(0.0)
+        slot = C.c_int(0)
+        st = lib().or_mc_attempt(self._h, int(movetype), float(step_trans), float(step_rot), float(fugacity_pa) * PASCAL,
+                                 KB * T, C.byref(dU), C.byref(slot))
+        return st, dU.value, slot.value
+
+    def cycle(self, probs, min_cycle, step_trans, step_rot, fugacity_pa, T):
+        """One GCMC cycle; probs = (translation, rotation, reinsertion, exchange)."""
+        pr = np.array([probs[0], probs[1], probs[2], probs[3] / 2, probs[3] / 2], dtype=np.float64)
+        pc = np.cumsum(pr / pr.sum())
+        pc[-1] = 2.0
+        return lib().or_mc_cycle(self._h, pc, int(min_cycle), float(step_trans), float(step_rot),
+                                 float(fugacity_pa) * PASCAL, KB * T)
+
+    def scalars(self):
+        out = np.zeros(6)
+        lib().or_mc_scalars(self._h, out)
+        return {"occupancy": int(out[0]), "energy": out[1], "energy_abs": out[2], "counter": int(out[3]),
+                "const_p": out[4], "const_q": out[5]}
+
+    def counts(self):
+        acc = np.zeros(5, dtype=np.int32)
+        att = np.zeros(5, dtype=np.int32)
+        lib().or_mc_counts(self._h, acc, att)
+        return acc, att
+
+    def guests(self):
+        n = self.scalars()["occupancy"]
+        p, q, h = np.zeros((max(n, 1), 3)), np.zeros((max(n, 1), 4)), np.zeros(max(n, 1))
+        lib().or_mc_guests(self._h, p.ctypes.data, q.ctypes.data, h.ctypes.data)
+        return p[:n], q[:n], h[:n]
+
+    def sk(self):
+        nk = int(lib_scalars(self.system)[11])
+        re, im = np.zeros(max(nk, 1)), np.zeros(max(nk, 1))
+        lib().or_mc_sk(self._h, re, im)
+        return re[:nk] + 1j * im[:nk]
+
+    def total_energy(self):
+        return lib().or_mc_total_energy(self._h)
+
+
+def lib_scalars(system):
+    out = np.zeros(12)
+    lib().or_system_scalars(system._h, out)
+    return out

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "pureadsorb_amd", "libpureadsorb_amd.so")
-SOURCES = ["pureadsorb_amd.hip", "widom_kernels.hip", "widom_binned.hip", "widom_grad.hip"]
+SOURCES = ["pureadsorb_amd.hip", "widom_kernels.hip", "widom_binned.hip", "widom_grad.hip", "mc_kernels.hip",
+           "mc_host.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

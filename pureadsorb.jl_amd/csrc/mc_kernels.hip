@@ -1,0 +1,321 @@
+// mc_kernels.hip -- the move-energy path (NVT and μVT Monte Carlo moves) on gfx950.
+//
+// One workgroup per chain runs that chain's move attempts back to back, so a run of many
+// moves costs one launch instead of the reference's three launches per move
+// (src/moves.jl:577-645). Within an attempt every work-item draws the same proposal from the
+// chain's counter-based stream (select_and_propose, src/moves.jl:275-288), then the workgroup
+// splits the three sums a move needs across its lanes:
+//   * the new pose's host-guest real-space energy over the framework atoms
+//     (host_guest_realspace_energy_range, src/guest.jl:176-200),
+//   * the guest-guest real-space change over (other guest, site, site) triples
+//     (guest_guest_move_delta / guest_pair_realspace_energy_range, src/guest.jl:112-156),
+//   * the reciprocal change over the k-vectors against the running structure factor
+//     (_reciprocal_move_delta_k, src/guest.jl:228-238), keeping each lane's dS(k) for the
+//     update on acceptance;
+// reduces them through LDS, and every work-item takes the same Metropolis decision
+// (src/moves.jl:198-246). Work-item 0 commits poses and occupancy; each lane applies its own
+// k-vectors' dS to the running structure factor. Everything is FP64.
+#include "pa_device.hpp"
+
+namespace pa {
+
+constexpr int MC_WG = 512;
+constexpr int MC_NW = MC_WG / 64;
+
+__device__ inline double wave_sum_d(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NS>
+__device__ inline void sites_at(const GuestDev &g, const double p[3], const double q[4], double out[NS][3])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double r[3];
+        rotate(q, g.site[s], r);
+        for (int d = 0; d < 3; ++d) out[s][d] = p[d] + r[d];
+    }
+}
+
+// Minimum-image squared distance between two points of the cell.
+__device__ inline double mi_r2(const FwDesc &F, const double a[3], const double b[3])
+{
+#pragma clang fp contract(off)
+    double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]}, m[3];
+    minimum_image(F.A, F.invA, d, m);
+    return m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+}
+
+template <int NS>
+__global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[3][MC_NW];
+    const int n = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const FwDesc &F = a.fw[a.fw_of[n]];
+    const int gr0 = a.guest_off[n], cap = a.guest_off[n + 1] - gr0, nslots = a.guest_off[a.nsys];
+    const double4 *atoms = a.atom64 + F.atom_off;
+    const int *types = a.atype + F.atom_off;
+    const double4 *ks = a.k64 + F.k_off;
+    double2 *Sk = a.Sk + a.sk_off[n];
+    double2 *dS = a.dS + a.sk_off[n];
+    double *skabs = a.sk_abs + a.sk_off[n];
+    const int nk = F.nk, natoms = F.natoms;
+    const GuestDev &G = a.guest;
+    const double alpha = F.alpha;
+    double qsum_abs = 0.0;
+    for (int s = 0; s < NS; ++s) qsum_abs += fabs(G.q[s]);
+    const double ds_bound = 2.0 * qsum_abs;
+
+    // chain scalars: identical copies in every work-item (every decision is taken by all)
+    int occ = a.occupancy[n];
+    double E = a.energy[n], Eabs = a.energy_abs[n];
+    uint64_t ctr = a.rng_counter[n];
+    const uint64_t seed = a.rng_seed[n];
+    int acc[NMOVES], att[NMOVES];
+    for (int t = 0; t < NMOVES; ++t) {
+        acc[t] = a.accepted[n * NMOVES + t];
+        att[t] = a.attempted[n * NMOVES + t];
+    }
+    int maxocc = a.max_occ ? a.max_occ[n] : 0;
+    int caphit = 0;
+    int64_t ntrace = 0;
+    const double step_t = a.step_trans ? a.step_trans[n] : 0.0, step_r = a.step_rot ? a.step_rot[n] : 0.0;
+    const double fugV = a.fug ? a.fug[n] * F.volume : 0.0;
+    const double cp = a.const_p ? a.const_p[n] : 0.0, cq = a.const_q ? a.const_q[n] : 0.0;
+
+    auto attempt = [&](int type) {
+        Chain rng(seed, ctr, 0);
+        int slot = 0;
+        double op[3] = {0, 0, 0}, oq[4] = {0, 0, 0, 1}, np[3] = {0, 0, 0}, nq[4] = {0, 0, 0, 1};
+        if (type != MOVE_INSERTION) {
+            const int64_t idx = rand_range(rng, occ > 0 ? occ : 1);
+            int64_t sl = gr0 + idx - 1;
+            slot = (int)(sl < nslots - 1 ? sl : nslots - 1);
+            if (slot < 0) slot = 0;
+            const double4 p4 = a.refpos[slot], q4 = a.quat[slot];
+            op[0] = p4.x; op[1] = p4.y; op[2] = p4.z;
+            oq[0] = q4.x; oq[1] = q4.y; oq[2] = q4.z; oq[3] = q4.w;
+        }
+        if (type == MOVE_TRANSLATION) {
+            const double dx = rand_normal(rng), dy = rand_normal(rng), dz = rand_normal(rng);
+            np[0] = op[0] + step_t * dx;
+            np[1] = op[1] + step_t * dy;
+            np[2] = op[2] + step_t * dz;
+            for (int i = 0; i < 4; ++i) nq[i] = oq[i];
+        } else if (type == MOVE_ROTATION) {
+            double qr[4], qp[4];
+            rand_quaternion(rng, qr);
+            quaternion_pow(qr, step_r, qp);
+            qmul(qp, oq, nq);
+            qnormalize(nq);
+            for (int i = 0; i < 3; ++i) np[i] = op[i];
+        } else if (type == MOVE_REINSERTION || type == MOVE_INSERTION) {
+            double f[3];
+            f[0] = unit_f64(rng.next());
+            f[1] = unit_f64(rng.next());
+            f[2] = unit_f64(rng.next());
+            matvec(F.A, f, np);
+            rand_quaternion(rng, nq);
+        }
+        const double u = unit_f64(rng.next());
+        const bool has_old = type != MOVE_INSERTION, has_new = type != MOVE_DELETION;
+        double os[NS][3], ns[NS][3];
+        sites_at<NS>(G, op, oq, os);
+        sites_at<NS>(G, np, nq, ns);
+
+        // (1) host-guest real space of the new pose
+        double hsum = 0.0;
+        if (has_new)
+            for (int j = tid; j < natoms; j += MC_WG) {
+                const double4 h = atoms[j];
+                const double hp[3] = {h.x, h.y, h.z};
+                const int ht = types[j];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const double r2 = mi_r2(F, ns[s], hp);
+                    if (r2 < a.rc_lj2 || r2 < a.rc_ew2) {
+                        const double2 p = a.pair64[s * a.ntypes_c + ht];
+                        hsum += site_pair_energy(r2, p.x, p.y, KE * G.q[s] * h.w, alpha, a.rc_lj2, a.rc_ew2);
+                    }
+                }
+            }
+        // (2) guest-guest real space: the moving (or inserted, or deleted) guest against every
+        // other live guest, one (other guest, site, site) triple per lane step
+        double gsum = 0.0;
+        const int ntrip = occ * NS * NS;
+        for (int x = tid; x < ntrip; x += MC_WG) {
+            const int jj = x / (NS * NS), sa = (x / NS) % NS, sb = x % NS;
+            const int j = gr0 + jj;
+            if (has_old && j == slot) continue;
+            const double4 p4 = a.refpos[j], q4 = a.quat[j];
+            const double qj[4] = {q4.x, q4.y, q4.z, q4.w};
+            double r[3];
+            rotate(qj, G.site[sb], r);
+            const double pb[3] = {p4.x + r[0], p4.y + r[1], p4.z + r[2]};
+            const double2 p = a.pair64[sa * a.ntypes_c + G.type[sb]];
+            const double qq = KE * G.q[sa] * G.q[sb];
+            if (has_new) gsum += site_pair_energy(mi_r2(F, ns[sa], pb), p.x, p.y, qq, alpha, a.rc_lj2, a.rc_ew2);
+            if (type <= MOVE_REINSERTION)
+                gsum -= site_pair_energy(mi_r2(F, os[sa], pb), p.x, p.y, qq, alpha, a.rc_lj2, a.rc_ew2);
+            else if (type == MOVE_DELETION)
+                gsum += site_pair_energy(mi_r2(F, os[sa], pb), p.x, p.y, qq, alpha, a.rc_lj2, a.rc_ew2);
+        }
+        // (3) reciprocal change against the running structure factor, dS kept per lane
+        double rsum = 0.0;
+        if (G.charged)
+            for (int k = tid; k < nk; k += MC_WG) {
+                const double4 kv = ks[k];
+                double sre = 0.0, sim = 0.0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    double sn, cs;
+                    if (has_new) {
+                        const double ph = kv.x * ns[s][0] + kv.y * ns[s][1] + kv.z * ns[s][2];
+                        sincos(ph, &sn, &cs);
+                        sre += G.q[s] * cs;
+                        sim += G.q[s] * sn;
+                    }
+                    if (has_old) {
+                        const double ph = kv.x * os[s][0] + kv.y * os[s][1] + kv.z * os[s][2];
+                        sincos(ph, &sn, &cs);
+                        sre -= G.q[s] * cs;
+                        sim -= G.q[s] * sn;
+                    }
+                }
+                const double2 S = Sk[k];
+                dS[k] = make_double2(sre, sim);
+                rsum += kv.w * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
+            }
+        hsum = wave_sum_d(hsum);
+        gsum = wave_sum_d(gsum);
+        rsum = wave_sum_d(rsum);
+        if (lane == 0) {
+            red[0][wid] = hsum;
+            red[1][wid] = gsum;
+            red[2][wid] = rsum;
+        }
+        __syncthreads();
+        double e_new = 0.0, gg = 0.0, rec = 0.0;
+        for (int w = 0; w < MC_NW; ++w) {
+            e_new += red[0][w];
+            gg += red[1][w];
+            rec += red[2][w];
+        }
+        rec *= KE;
+        __syncthreads();  // red[] is reused by the next attempt
+
+        // Metropolis decision, identical in every work-item
+        double dU;
+        bool accept;
+        if (type <= MOVE_REINSERTION) {
+            dU = (e_new - a.host_e[slot]) + gg + rec;
+            accept = isfinite(dU) && (-dU / a.kT > log(u)) && occ > 0;
+        } else if (type == MOVE_INSERTION) {
+            dU = e_new + gg + rec + (cp + cq * (double)occ);
+            const double lp = log(fugV) - log((double)occ + 1.0) - log(a.kT);
+            const bool would = isfinite(dU) && (lp - dU / a.kT > log(u));
+            accept = would && occ < cap;
+            if (would && occ >= cap) caphit = 1;
+        } else {
+            const int occ_eff = occ > 0 ? occ : 1;
+            dU = -(a.host_e[slot] + gg + (cp + cq * (double)(occ - 1))) + rec;
+            const double lp = -(log(fugV) - log((double)(occ_eff - 1) + 1.0) - log(a.kT));
+            accept = isfinite(dU) && (lp - dU / a.kT > log(u)) && occ > 0;
+        }
+        if (a.trace && ntrace < a.trace_cap && tid == 0)
+            a.trace[(int64_t)n * a.trace_cap + ntrace] = make_double4((double)type, (double)(slot - gr0), dU, accept ? 1.0 : 0.0);
+        ++ntrace;
+        if (accept) {
+            if (tid == 0) {
+                if (type <= MOVE_REINSERTION) {
+                    a.refpos[slot] = make_double4(np[0], np[1], np[2], 0.0);
+                    a.quat[slot] = make_double4(nq[0], nq[1], nq[2], nq[3]);
+                    a.host_e[slot] = e_new;
+                } else if (type == MOVE_INSERTION) {
+                    const int s2 = gr0 + occ;
+                    a.refpos[s2] = make_double4(np[0], np[1], np[2], 0.0);
+                    a.quat[s2] = make_double4(nq[0], nq[1], nq[2], nq[3]);
+                    a.host_e[s2] = e_new;
+                } else {
+                    const int hi = gr0 + occ - 1;
+                    if (slot != hi) {
+                        a.refpos[slot] = a.refpos[hi];
+                        a.quat[slot] = a.quat[hi];
+                        a.host_e[slot] = a.host_e[hi];
+                    }
+                }
+            }
+            if (type == MOVE_INSERTION) ++occ;
+            if (type == MOVE_DELETION) --occ;
+            E += dU;
+            Eabs += fabs(dU);
+            ++acc[type - 1];
+            if (G.charged)
+                for (int k = tid; k < nk; k += MC_WG) {
+                    const double2 d = dS[k], S = Sk[k];
+                    Sk[k] = make_double2(S.x + d.x, S.y + d.y);
+                    skabs[k] += ds_bound;
+                }
+        }
+        ++att[type - 1];
+        ++ctr;
+        __syncthreads();  // committed poses visible to the next attempt
+    };
+
+    auto draw_type = [&]() {
+        Chain r1(seed, ctr, 1);
+        const double v = unit_f64(r1.next());
+        int t = NMOVES;
+        for (int i = 0; i < NMOVES; ++i)
+            if (v < a.pcum[i]) { t = i + 1; break; }
+        return t;
+    };
+
+    if (a.mode == 0) {
+        for (int64_t m = 0; m < a.nmoves; ++m) attempt(a.movetype);
+    } else {
+        for (int64_t c = 0; c < a.ncycles; ++c) {
+            if (occ > maxocc) maxocc = occ;
+            const int64_t len = occ > a.min_cycle ? occ : a.min_cycle;
+            for (int64_t m = 0; m < len; ++m) attempt(draw_type());
+            if (occ > maxocc) maxocc = occ;
+            if (a.sampling && tid == 0) {
+                a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c] = E;
+                a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c] = (double)occ;
+            }
+        }
+    }
+    if (tid == 0) {
+        a.occupancy[n] = occ;
+        a.energy[n] = E;
+        a.energy_abs[n] = Eabs;
+        a.rng_counter[n] = ctr;
+        for (int t = 0; t < NMOVES; ++t) {
+            a.accepted[n * NMOVES + t] = acc[t];
+            a.attempted[n * NMOVES + t] = att[t];
+        }
+        if (a.max_occ) a.max_occ[n] = maxocc;
+        if (a.capacity_hits) a.capacity_hits[n] = caphit;
+    }
+}
+
+hipError_t launch_mc(const McArgs &a, hipStream_t st)
+{
+    const dim3 grid((unsigned)a.nsys), block(MC_WG);
+    switch (a.guest.nsites) {
+    case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(mc_chain_kernel<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(mc_chain_kernel<3>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(mc_chain_kernel<4>, grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL(mc_chain_kernel<5>, grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL(mc_chain_kernel<6>, grid, block, 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pa

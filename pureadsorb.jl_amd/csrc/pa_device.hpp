@@ -131,4 +131,76 @@ struct GradArgs {
     const int64_t *koff;        // nsys: first k slot of the system in acc_k
 };
 
+// One site pair of the move path (FP64): Lennard-Jones inside rc_lj plus the screened Coulomb
+// qq_ke * erfc(alpha r)/r inside rc_ew (qq_ke = KE q_a q_b), on a minimum-image separation r^2.
+__host__ __device__ inline double site_pair_energy(double r2, double sg, double ep, double qq_ke, double alpha,
+                                                   double rc_lj2, double rc_ew2)
+{
+#pragma clang fp contract(off)
+    double e = 0.0;
+    if (r2 < rc_lj2) {
+        double x = sg * sg / r2;
+        x = x * x * x;
+        e = 4.0 * ep * (x * x - x);
+    }
+    if (r2 < rc_ew2) {
+        const double r = __builtin_sqrt(r2);
+        e += qq_ke * pair_erfc_f64(alpha * r) / r;
+    }
+    return e;
+}
+
+// Move types of the move path: the reference's three NVT moves (src/moves.jl:69-71) and the two
+// exchange moves.
+constexpr int MOVE_TRANSLATION = 1, MOVE_ROTATION = 2, MOVE_REINSERTION = 3, MOVE_INSERTION = 4, MOVE_DELETION = 5;
+constexpr int NMOVES = 5;
+
+// One launch of the move path: every chain (system) is one workgroup that runs its move attempts
+// back to back (mc_kernels.hip).
+struct McArgs {
+    // framework tables: the FP64 layout with every k-vector (FULLK batch)
+    const FwDesc *fw;
+    const int *fw_of;
+    const double4 *atom64;
+    const int *atype;
+    const double4 *k64;         // kx, ky, kz, kprefactor
+    const double2 *pair64;      // [site][type]: sigma, epsilon
+    int ntypes_c;
+    double rc_lj2, rc_ew2;
+    GuestDev guest;
+    int nsys;
+    // chain state (SystemState, src/state.jl:89-104)
+    const int *guest_off;       // nsys + 1: capacity blocks
+    int *occupancy;
+    double4 *refpos;            // per slot: x, y, z (Å), 0
+    double4 *quat;              // per slot: x, y, z, w
+    double *host_e;             // per slot: cached host-guest real-space energy
+    double2 *Sk;                // per system k slice: host + every guest
+    double2 *dS;                // scratch, same layout: the pending move's structure-factor change
+    double *sk_abs;             // per k: accumulated rounding scale
+    const int *sk_off;          // nsys + 1
+    double *energy, *energy_abs;
+    const uint64_t *rng_seed;
+    uint64_t *rng_counter;
+    int *accepted, *attempted;  // nsys x NMOVES
+    // what to run
+    int mode;                   // 0: nmoves attempts of movetype; 1: ncycles GCMC cycles
+    int movetype;
+    int64_t nmoves;
+    int64_t ncycles, min_cycle;
+    double pcum[NMOVES];        // mode 1: cumulative probabilities of the move types
+    const double *step_trans, *step_rot;   // per chain
+    const double *fug;          // per chain, eV/Å^3
+    const double *const_p, *const_q;       // exchange constant term p + q N (src/moves.jl:728-773)
+    double kT;
+    // mode 1 sampling: after every cycle c of this launch, samp_*[n * samp_stride + samp_base + c]
+    int sampling;
+    int64_t samp_base, samp_stride;
+    double *samp_e, *samp_n;
+    int *max_occ;
+    int *capacity_hits;         // per chain: accepted-by-Metropolis insertions refused at capacity
+    double4 *trace;             // optional, per chain trace_cap entries: type, slot, dU, accepted
+    int64_t trace_cap;
+};
+
 }  // namespace pa

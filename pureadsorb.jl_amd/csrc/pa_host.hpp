@@ -19,6 +19,8 @@ int fail(int code, const char *fmt, ...);
 int ok();
 // PA_OK once a HIP device is present (checked once per process).
 int device_ready();
+// erfc by the reference's 28-term Chebyshev series (src/ewald.jl:47-74), FP64.
+double erfc_dev_host(double x);
 
 #define HIP_TRY(expr)                                                                                    \
     do {                                                                                                 \
@@ -100,6 +102,11 @@ struct pa_batch {
     // hard-core rejection: per unique framework the bound B_s (src/reject.jl:101-131) and
     // kmin[site][compact type] (src/reject.jl:141-153); device cell grids
     std::vector<double> hc_bs, hc_kmin;
+    // host copies of the FP64 tables (the move path's from-scratch energies and audit)
+    bool fullk = false;
+    std::vector<double4> atom64_h, k64_h;
+    std::vector<double2> S64_h;
+    std::vector<int> atype_h;
     pa::DevAlloc d_corefx, d_cell_start, d_rho2;
     // gradient scratch
     pa::DevAlloc d_list_g, d_list_w, d_list_wu, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
@@ -116,3 +123,7 @@ struct pa_batch {
     }
 };
 
+namespace pa {
+// Squared phase-0 rejection radii of system s at kT, NS x ntypes_c (build_rejection_tables).
+int rejection_radii(const pa_batch *b, int s, double kT, double eps_f, double *rho2, double *rmax2);
+}  // namespace pa

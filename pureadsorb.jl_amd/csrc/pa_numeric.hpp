@@ -17,6 +17,8 @@ constexpr double KE = 14.399645351950548;
 constexpr double PI = 3.141592653589793;
 constexpr double TWO_PI = 6.283185307179586;
 constexpr int MAX_SITES = 16;
+// src/constants.jl:10: one pascal in eV/Å^3.
+constexpr double PASCAL_EV = 1.0 / (1.0e30 * 1.6021766208e-19);
 
 // ---------------------------------------------------------------------------------------
 // Counter-based RNG (src/state.jl:225-230, src/rng.jl:14-81): a key folded by SplitMix64,
@@ -92,6 +94,84 @@ __host__ __device__ inline void matvec(const double A[9], const double v[3], dou
 {
 #pragma clang fp contract(off)
     for (int i = 0; i < 3; ++i) out[i] = A[i] * v[0] + A[i + 3] * v[1] + A[i + 6] * v[2];
+}
+
+// ---------------------------------------------------------------------------------------
+// Draws of the move path (src/rng.jl:14-24,93-141; src/moves.jl:81-161)
+// ---------------------------------------------------------------------------------------
+// High 64 bits of the 128-bit product, from 32-bit halves (exact).
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b)
+{
+    const uint64_t a0 = a & 0xffffffffULL, a1 = a >> 32, b0 = b & 0xffffffffULL, b1 = b >> 32;
+    const uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
+    const uint64_t mid = (p00 >> 32) + (p01 & 0xffffffffULL) + (p10 & 0xffffffffULL);
+    return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+}
+
+// rand_range: uniform integer in 1..n by Lemire's multiply-shift map (src/rng.jl:93-97).
+__host__ __device__ inline int64_t rand_range(Chain &c, int64_t n) { return (int64_t)mulhi64(c.next(), (uint64_t)n) + 1; }
+
+// rand_normal: Box-Muller cosine branch on u1, u2 (src/rng.jl:109-115); cos(2 pi u2) as
+// cospi(2 u2).
+__host__ __device__ inline double rand_normal(Chain &c)
+{
+#pragma clang fp contract(off)
+    const double u1 = unit_f64(c.next());
+    const double u2 = unit_f64(c.next());
+    const double r = __builtin_sqrt(-2.0 * log(1.0 - u1));
+    double s, co;
+    sincospi(2.0 * u2, &s, &co);
+    return r * co;
+}
+
+// rand_quaternion (src/rng.jl:136-141).
+__host__ __device__ inline void rand_quaternion(Chain &c, double q[4])
+{
+    const double u1 = unit_f64(c.next());
+    const double u2 = unit_f64(c.next());
+    const double u3 = unit_f64(c.next());
+    shoemake(u1, u2, u3, q);
+}
+
+// qmul(q2, q1): apply q1, then q2, (x, y, z, w) order (src/moves.jl:81-89).
+__host__ __device__ inline void qmul(const double q2[4], const double q1[4], double out[4])
+{
+#pragma clang fp contract(off)
+    const double x1 = q1[0], y1 = q1[1], z1 = q1[2], w1 = q1[3];
+    const double x2 = q2[0], y2 = q2[1], z2 = q2[2], w2 = q2[3];
+    out[3] = w2 * w1 - (x2 * x1 + y2 * y1 + z2 * z1);
+    out[0] = w2 * x1 + w1 * x2 + (y2 * z1 - z2 * y1);
+    out[1] = w2 * y1 + w1 * y2 + (z2 * x1 - x2 * z1);
+    out[2] = w2 * z1 + w1 * z2 + (x2 * y1 - y2 * x1);
+}
+
+// quaternion_pow: the same axis with the rotation angle scaled by t (src/moves.jl:102-110).
+__host__ __device__ inline void quaternion_pow(const double q[4], double t, double out[4])
+{
+#pragma clang fp contract(off)
+    const double w = q[3] < -1.0 ? -1.0 : (q[3] > 1.0 ? 1.0 : q[3]);
+    const double theta = 2.0 * acos(w);
+    const double s2 = 1.0 - w * w;
+    const double s = __builtin_sqrt(s2 > 0.0 ? s2 : 0.0);
+    double ax[3] = {1.0, 0.0, 0.0};
+    if (s > 2.220446049250313e-16) {
+        ax[0] = q[0] / s;
+        ax[1] = q[1] / s;
+        ax[2] = q[2] / s;
+    }
+    const double half = t * theta / 2.0;
+    const double sh = sin(half);
+    out[0] = ax[0] * sh;
+    out[1] = ax[1] * sh;
+    out[2] = ax[2] * sh;
+    out[3] = cos(half);
+}
+
+__host__ __device__ inline void qnormalize(double q[4])
+{
+#pragma clang fp contract(off)
+    const double nrm = __builtin_sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) q[i] = q[i] / nrm;
 }
 
 // Widom pose: fractional position f (3 uniforms) and quaternion (3 uniforms) drawn from

@@ -321,6 +321,8 @@ double find_rho2(double sg, double ep, double kmin, double margin, double r_lj)
 }
 }  // namespace
 
+double pa::erfc_dev_host(double x) { return erfc_dev(x); }
+
 // ------------------------------------------------------------------------------------------
 // Batch
 // ------------------------------------------------------------------------------------------
@@ -704,6 +706,11 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         }
 
     b->pair64_h = pair64;
+    b->fullk = fullk;
+    b->atom64_h = atom64;
+    b->atype_h = atype;
+    b->k64_h = k64;
+    b->S64_h = S64;
 
     // ---- hard-core bound B_s and kmin per framework (src/batch.jl:395,409-423; src/reject.jl).
     // The attractive-pair term eps + |K| erfc(alpha r0)/r0 grows with |K|; it is evaluated on a
@@ -856,6 +863,36 @@ static int check_precision(const pa_batch *b, uint32_t precision)
     return PA_OK;
 }
 
+// Rejection radii^2 of system s at kT (build_rejection_tables, src/widom.jl:222-253): margin =
+// (theta_F + 2) kT + safety + B_s - c_s, safety sized for energies summed at unit roundoff
+// eps_f / 2. rho2: NS x ntypes_c, Å^2 (0: no rejection for that pair of types).
+int pa::rejection_radii(const pa_batch *b, int s, double kT, double eps_f, double *rho2, double *rmax2)
+{
+    static const double theta = theta_f64();
+    const int NS = b->nsites, NTC = b->ntypes_c;
+    const int u = b->fw_of[s];
+    const FwHost &h = b->info[u];
+    const double Bs = b->hc_bs[u], cs = h.const_off;
+    const double n = (double)NS * h.natoms + h.nk + 8;
+    const double safety = 2.0 * n * eps_f * (Bs + std::fabs(cs)) + 4.0e-6 * Bs;
+    const double margin = std::isinf(Bs) ? INFINITY : (theta + 2.0) * kT + safety + Bs - cs;
+    if (!(margin > 0.0))
+        return fail(PA_ERR_ARG,
+                    "build_rejection_tables: system %d has margin=%g <= 0 (Bs=%g, safety=%g, constant_offset=%g, "
+                    "kT=%g, theta_F=%g); no valid rejection radius exists",
+                    s + 1, margin, Bs, safety, cs, kT, theta);
+    *rmax2 = 0.0;
+    for (int a = 0; a < NS; ++a)
+        for (int c = 0; c < NTC; ++c) {
+            const double2 p = b->pair64_h[(size_t)a * NTC + c];
+            double r2 = find_rho2(p.x, p.y, b->hc_kmin[((size_t)u * NS + a) * NTC + c], margin, b->rc_lj);
+            if (!(r2 >= 0.0)) r2 = 0.0;  // no bracket: no rejection for this pair of types
+            rho2[(size_t)a * NTC + c] = r2;
+            *rmax2 = std::max(*rmax2, r2);
+        }
+    return PA_OK;
+}
+
 // Per-system sample counts under the run-length assignment (src/widom.jl:44-53).
 static void system_counts(int64_t ninsert, int64_t nsys, int64_t run, std::vector<int64_t> &ns)
 {
@@ -958,34 +995,22 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     // + B_s - c_s with the FP32 rounding allowance of the mixed path's pair terms
     std::vector<float> rho2((size_t)nsys * b->nsites * b->ntypes_c, 0.0f);
     if (kT > 0.0) {
-        static const double theta = theta_f64();
         const int NS = b->nsites, NTC = b->ntypes_c;
+        std::vector<double> r2s((size_t)NS * NTC);
         for (int s = 0; s < nsys; ++s) {
-            const int u = b->fw_of[s];
-            const FwHost &h = b->info[u];
-            const FwDesc &D = b->desc[u];
-            const double Bs = b->hc_bs[u], cs = h.const_off;
-            const double n = (double)NS * h.natoms + h.nk + 8;
-            const double safety = 2.0 * n * (double)FLT_EPSILON * (Bs + std::fabs(cs)) + 4.0e-6 * Bs;
-            const double margin = std::isinf(Bs) ? INFINITY : (theta + 2.0) * kT + safety + Bs - cs;
-            if (!(margin > 0.0))
-                return fail(PA_ERR_ARG,
-                            "build_rejection_tables: system %d has margin=%g <= 0 (Bs=%g, safety=%g, constant_offset=%g, "
-                            "kT=%g, theta_F=%g); no valid rejection radius exists",
-                            s + 1, margin, Bs, safety, cs, kT, theta);
             double rmax2 = 0.0;
-            for (int a = 0; a < NS; ++a)
-                for (int c = 0; c < NTC; ++c) {
-                    const double2 p = b->pair64_h[(size_t)a * NTC + c];
-                    double r2 = find_rho2(p.x, p.y, b->hc_kmin[((size_t)u * NS + a) * NTC + c], margin, b->rc_lj);
-                    if (!(r2 >= 0.0)) r2 = 0.0;  // no bracket: no rejection for this pair of types
-                    // FP32 radius rounded down with a 1e-5 relative allowance for the device's
-                    // FP32 distance
-                    float f = (float)(r2 * (1.0 - 1.0e-5));
-                    if ((double)f > r2 * (1.0 - 1.0e-5)) f = std::nextafter(f, 0.0f);
-                    rho2[((size_t)s * NS + a) * NTC + c] = f;
-                    rmax2 = std::max(rmax2, r2);
-                }
+            int st = rejection_radii(b, s, kT, (double)FLT_EPSILON, r2s.data(), &rmax2);
+            if (st) return st;
+            for (size_t i = 0; i < r2s.size(); ++i) {
+                // FP32 radius rounded down with a 1e-5 relative allowance for the device's FP32
+                // distance
+                const double r2 = r2s[i];
+                float f = (float)(r2 * (1.0 - 1.0e-5));
+                if ((double)f > r2 * (1.0 - 1.0e-5)) f = std::nextafter(f, 0.0f);
+                rho2[(size_t)s * NS * NTC + i] = f;
+            }
+            const FwHost &h = b->info[b->fw_of[s]];
+            const FwDesc &D = b->desc[b->fw_of[s]];
             if (rmax2 > 0.0)
                 for (int d = 0; d < 3; ++d)
                     bins[s].reach[d] = (int)std::ceil(std::sqrt(rmax2) * D.cg_nc[d] / h.perp[d]);

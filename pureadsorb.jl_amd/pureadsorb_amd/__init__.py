@@ -8,6 +8,9 @@ libpureadsorb_amd.so (HIP kernels for gfx950) through its C ABI; there is no CPU
 from ._lib import ArgumentError, DeviceError, DimensionMismatch, device_name, synchronize
 from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
+from .mc import (MOVE_DELETION, MOVE_INSERTION, MOVE_REINSERTION, MOVE_ROTATION, MOVE_TRANSLATION, GCMCResult,
+                 SystemState, audit_energy, gcmc_samples, mc_delete, mc_exchange, mc_insert, mc_moves, mc_step, run_gcmc,
+                 total_energy)
 from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
                         reciprocal_basis, replicate, total_charge, volume)
 from .widom import (KB, KE, PASCAL, WidomGrad, WidomResult, default_run, hardcore_flags, last_timing, sys_of_index,
@@ -19,4 +22,7 @@ __all__ = [
     "cartesian", "cell_matrix", "min_multiplicity", "natoms", "perpendicular_lengths", "read_cif", "reciprocal_basis",
     "replicate", "total_charge", "volume", "KB", "KE", "PASCAL", "WidomResult", "default_run", "last_timing",
     "sys_of_index", "system_counts", "widom", "widom_energies", "widom_grad", "widom_poses", "WidomGrad", "hardcore_flags",
+    "MOVE_TRANSLATION", "MOVE_ROTATION", "MOVE_REINSERTION", "MOVE_INSERTION", "MOVE_DELETION", "SystemState",
+    "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "run_gcmc",
+    "GCMCResult", "gcmc_samples",
 ]

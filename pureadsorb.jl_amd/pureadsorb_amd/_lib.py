@@ -29,6 +29,8 @@ EXPORTS = (
     "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
     "pa_batch_framework_info", "pa_widom", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
+    "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
+    "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples",
 )
 
 
@@ -97,6 +99,31 @@ class PaFrameworkInfo(C.Structure):
     ]
 
 
+class PaChainInfo(C.Structure):
+    _fields_ = [
+        ("occupancy", C.c_int32), ("capacity", C.c_int32), ("energy", C.c_double), ("energy_abs_accum", C.c_double),
+        ("rng_counter", C.c_int64), ("accepted", C.c_int32 * 5), ("attempted", C.c_int32 * 5),
+        ("const_p", C.c_double), ("const_q", C.c_double),
+    ]
+
+
+class PaGcmcParams(C.Structure):
+    _fields_ = [
+        ("T", C.c_double), ("n_warmup", C.c_int64), ("n_production", C.c_int64), ("n_audit", C.c_int64),
+        ("min_cycle_length", C.c_int64), ("fugacity_pa", C.c_void_p), ("step_trans", C.c_void_p),
+        ("step_rot", C.c_void_p), ("p_translation", C.c_double), ("p_rotation", C.c_double),
+        ("p_reinsertion", C.c_double), ("p_exchange", C.c_double), ("nblocks", C.c_int32),
+    ]
+
+
+class PaGcmcResult(C.Structure):
+    _fields_ = [
+        ("loading", C.c_double), ("loading_err", C.c_double), ("energy", C.c_double), ("energy_err", C.c_double),
+        ("q_st", C.c_double), ("q_st_err", C.c_double), ("corr_UN", C.c_double), ("max_occupancy", C.c_int32),
+        ("capacity", C.c_int32), ("ncycles", C.c_int64), ("accepted", C.c_int64 * 5), ("attempted", C.c_int64 * 5),
+    ]
+
+
 class PaTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
@@ -136,6 +163,32 @@ def lib():
     L.pa_widom_grad.restype = C.c_int
     L.pa_widom_grad.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32,
                                 C.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    L.pa_state_create.restype = C.c_int
+    L.pa_state_create.argtypes = [vp, C.POINTER(PaGuest), vp, vp, C.c_double, C.c_uint64, C.POINTER(vp)]
+    L.pa_state_destroy.restype = None
+    L.pa_state_destroy.argtypes = [vp]
+    L.pa_mc_step.restype = C.c_int
+    L.pa_mc_step.argtypes = [vp, C.c_int32, vp, vp, C.c_double]
+    L.pa_mc_insert.restype = C.c_int
+    L.pa_mc_insert.argtypes = [vp, vp, C.c_double]
+    L.pa_mc_delete.restype = C.c_int
+    L.pa_mc_delete.argtypes = [vp, vp, C.c_double]
+    L.pa_mc_moves.restype = C.c_int
+    L.pa_mc_moves.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, C.c_double, vp]
+    L.pa_state_info.restype = C.c_int
+    L.pa_state_info.argtypes = [vp, C.c_int32, C.POINTER(PaChainInfo)]
+    L.pa_state_guests.restype = C.c_int
+    L.pa_state_guests.argtypes = [vp, C.c_int32, vp, vp, vp]
+    L.pa_state_sk.restype = C.c_int
+    L.pa_state_sk.argtypes = [vp, C.c_int32, vp]
+    L.pa_total_energy.restype = C.c_int
+    L.pa_total_energy.argtypes = [vp, vp]
+    L.pa_audit_energy.restype = C.c_int
+    L.pa_audit_energy.argtypes = [vp, vp]
+    L.pa_gcmc_run.restype = C.c_int
+    L.pa_gcmc_run.argtypes = [vp, C.POINTER(PaGcmcParams), vp]
+    L.pa_gcmc_samples.restype = C.c_int
+    L.pa_gcmc_samples.argtypes = [vp, C.c_int32, C.c_int64, vp, vp]
     L.pa_hardcore_flags.restype = C.c_int
     L.pa_hardcore_flags.argtypes = [vp, C.c_int32, C.c_uint64, C.c_double, C.c_int64, C.c_int64, vp]
     L.pa_widom_energies.restype = C.c_int
