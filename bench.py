@@ -127,9 +127,11 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=2, help="timed steps of the FP64 plain path (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=4000, help="insertions timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4"), default="config1",
                     help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions; "
-                         "config3: gradient cost on 64 frameworks x 2e5 insertions")
+                         "config3: gradient cost on 64 frameworks x 2e5 insertions; config4: GCMC move path")
+    ap.add_argument("--gcmc-warmup", type=int, default=20000)
+    ap.add_argument("--gcmc-production", type=int, default=50000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -154,6 +156,8 @@ def main():
         return bench_config2(args)
     if args.workload == "config3":
         return bench_config3(args)
+    if args.workload == "config4":
+        return bench_config4(args)
     fw, ff, g, ew = workload()
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
@@ -326,6 +330,81 @@ def bench_config3(args):
     }
     log(f"config3: widom {tw * 1e3:.1f} ms, widom_grad {tg * 1e3:.1f} ms (cost x{tg / tw:.2f}), grad kernel "
         f"{gms / args.steps:.1f} ms over {nweighted} weighted insertions")
+    print(json.dumps(out), flush=True)
+
+
+def bench_config4(args):
+    """GCMC move path: CO2 in the config-1 cell replicated 3x3x2 (the smallest supercell whose
+    perpendicular widths reach 2 x 12 A; 2x2x2 gives 22.5 A), three chains at fugacity 1e4, 1e5,
+    1e6 Pa, capacity 400, moves insertion/deletion/translation (Gaussian, s.d. 1 A)/rotation at
+    1/4 each, 2e4 equilibration + 5e4 production cycles. Run on the specified 300-atom cell and on
+    the 80-atom porous variant; the first 200 production-like cycles of each chain are replayed
+    on the FP64 oracle from the same state and streams."""
+    import oracle as O
+
+    fug = [1.0e4, 1.0e5, 1.0e6]
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    out = {"metric": "GCMC move attempts/s on 1 GPU (move-energy path, FP64)", "unit": "attempts/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "steps": 1, "warmup": 0,
+           "config": {"workload": "CO2 GCMC, config-1 cell (13/13/15 A, gamma 120) x 3x3x2, 298.15 K, fugacity 1e4/1e5/1e6 "
+                                  "Pa, capacity 400, p(insert)=p(delete)=p(translate, s.d. 1 A)=p(rotate, step 0.5)=1/4, "
+                                  f"{args.gcmc_warmup} + {args.gcmc_production} cycles, audit every 5000 cycles"}}
+    variants = {"config1_cell": synth.synthetic_framework(13.0, 13.0, 15.0, 90, 90, 120, 300, 42, charge_sd=0.5),
+                "porous_80_atoms": synth.synthetic_framework(13.0, 13.0, 15.0, 90, 90, 120, 80, 42, charge_sd=0.5)}
+    total_attempts, total_s = 0, 0.0
+    for name, uc in variants.items():
+        fw = pa.replicate(uc, (3, 3, 2))
+        b = pa.FrameworkBatch([fw] * 3, ff, g, ew, fullk=True)
+        # oracle replay of the first cycles from an empty start
+        st0 = pa.SystemState(b, g, [0, 0, 0], T=T_K, seed=7, capacities=[400] * 3)
+        nrep = 200
+        pa.run_gcmc(st0, T=T_K, n_warmup=0, n_production=nrep, n_audit=nrep, step_trans=1.0, step_rot=0.5,
+                    fugacity=fug, nblocks=2)
+        replay = []
+        types = [pa.typeindex(ff, s + "_") for s in fw.symbols]
+        osys = O.OracleSystem(fw.cell, fw.frac, types, fw.charges, fw.replication, ff.sigma, ff.epsilon, ff.cutoff,
+                              ff.tail, g.sites, g.types, g.charges, ew.cutoff, ew.precision)
+        for n in range(3):
+            e_dev, n_dev = pa.gcmc_samples(st0, n, nrep)
+            m = O.OracleMC(osys, 400, O.splitmix64(7, n + 1))
+            e_or, n_or = [], []
+            for _ in range(nrep):
+                m.cycle((0.25, 0.25, 0.0, 0.5), 1, 1.0, 0.5, fug[n], T_K)
+                sc = m.scalars()
+                e_or.append(sc["energy"])
+                n_or.append(sc["occupancy"])
+            acc_o, _ = m.counts()
+            replay.append({"fugacity_pa": fug[n], "cycles": nrep, "occupancy_sequences_equal": bool(np.array_equal(n_dev, n_or)),
+                           "max_abs_d_energy_eV": float(np.max(np.abs(e_dev - np.array(e_or)))),
+                           "oracle_accepted": [int(x) for x in acc_o]})
+        st0.close()
+        st = pa.SystemState(b, g, [0, 0, 0], T=T_K, seed=42, capacities=[400] * 3)
+        t0 = time.perf_counter()
+        res = pa.run_gcmc(st, T=T_K, n_warmup=args.gcmc_warmup, n_production=args.gcmc_production, n_audit=5000,
+                          step_trans=1.0, step_rot=0.5, fugacity=fug, nblocks=10)
+        wall = time.perf_counter() - t0
+        att = sum(sum(r.attempted) for r in res)
+        # warmup attempts are not in the production counters; count them from the chain counters
+        att_all = sum(st.info(n)["rng_counter"] for n in range(3))
+        total_attempts += att_all
+        total_s += wall
+        st.close()
+        out[name] = {
+            "atoms": fw.natoms, "wall_s": wall, "attempts": att_all, "attempts_per_s": att_all / wall,
+            "results": [{"fugacity_pa": fug[n], "loading": r.loading, "loading_err": r.loading_err,
+                         "energy_eV": r.energy, "energy_err_eV": r.energy_err, "q_st_eV": r.q_st,
+                         "q_st_err_eV": r.q_st_err, "max_occupancy": r.max_occupancy,
+                         "accepted": r.accepted, "attempted": r.attempted} for n, r in enumerate(res)],
+            "oracle_replay": replay,
+        }
+        log(f"config4 {name}: {att_all} attempts in {wall:.1f} s ({att_all / wall:.3e}/s); loadings "
+            + ", ".join(f"{r.loading:.2f}+-{r.loading_err:.2f}" for r in res)
+            + f"; replay equal: {[x['occupancy_sequences_equal'] for x in replay]}")
+    out["value"] = total_attempts / total_s
+    out["ms_per_step"] = total_s * 1e3
     print(json.dumps(out), flush=True)
 
 
