@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4"), default="config1",
                     help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions; "
                          "config3: gradient cost on 64 frameworks x 2e5 insertions; config4: GCMC move path")
+    ap.add_argument("--bin-scale", type=float, default=0.0,
+                    help="mixed-path bin edge multiplier (0: the library tunes it on the first call)")
     ap.add_argument("--gcmc-warmup", type=int, default=20000)
     ap.add_argument("--gcmc-production", type=int, default=50000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -162,12 +164,15 @@ def main():
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
     log(f"batch: {fw.natoms} atoms, {b.nk[0]} coupled k-vectors, built in {time.perf_counter() - t0:.2f} s")
+    if args.bin_scale > 0:
+        b.bin_scale = args.bin_scale
 
     wall, ems, nl, r_mx = time_steps(b, g, args.ninsert, args.steps, args.warmup, "mixed")
     ins_s = args.steps * args.ninsert / wall
     avg_launch_ms = ems / max(nl, 1)
     ins_per_launch = args.steps * args.ninsert / max(nl, 1)
-    log(f"mixed: {ins_s:.4e} ins/s, {wall / args.steps * 1e3:.3f} ms/step, energy kernel {avg_launch_ms:.3f} ms/launch")
+    log(f"mixed: {ins_s:.4e} ins/s, {wall / args.steps * 1e3:.3f} ms/step, energy kernel {avg_launch_ms:.3f} ms/launch, "
+        f"bin scale {b.bin_scale}")
 
     out = {
         "metric": METRIC,
@@ -187,6 +192,7 @@ def main():
                         "rc 12 A, Ewald 1e-6 (184 coupled k), T 298.15 K, 1e6 insertions/step",
             "ninsert_per_step": args.ninsert,
             "precision": "FP32 pair + k-space terms, FP64 accumulation, tables and Boltzmann sums",
+            "bin_scale": b.bin_scale,
             "parallelism": "1 GPU",
         },
     }

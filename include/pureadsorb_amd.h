@@ -252,6 +252,14 @@ int pa_gcmc_run(pa_state *state, const pa_gcmc_params *params, pa_gcmc_result *o
 /* Per-cycle production samples of the last pa_gcmc_run: total energy (eV) and loading. */
 int pa_gcmc_samples(const pa_state *state, int32_t sys, int64_t n, double *energy, double *loading);
 
+/* Mixed-path bin edge multiplier of `batch` (no counterpart in the reference): every system's
+ * spatial bin edge is kappa * max(2 Å, (64 V / insertions)^(1/3)). 0 (the initial value) lets the
+ * first large enough pa_widom call time a few values of kappa and keep the fastest; a fixed
+ * kappa makes results reproducible across processes (the bin only changes FP32 rounding of the
+ * coordinates relative to its centre, ~1e-7 relative in dU). */
+int pa_batch_set_bin_scale(pa_batch *batch, double kappa);
+int pa_batch_bin_scale(const pa_batch *batch, double *kappa);
+
 /* Timing of the last pa_widom or pa_widom_grad call on `batch`. */
 int pa_last_timing(const pa_batch *batch, pa_timing *out);
 

@@ -104,6 +104,7 @@ struct pa_batch {
     std::vector<double> hc_bs, hc_kmin;
     // host copies of the FP64 tables (the move path's from-scratch energies and audit)
     bool fullk = false;
+    double bin_scale = 0.0;  // mixed path: bin edge multiplier (0: not yet tuned, 1 used meanwhile)
     std::vector<double4> atom64_h, k64_h;
     std::vector<double2> S64_h;
     std::vector<int> atype_h;

@@ -966,7 +966,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         const FwDesc &D = b->desc[b->fw_of[s]];
         const int64_t lo = std::max(gb, seg[s]), hi = std::min(ge, seg[s + 1]);
         const double nins = (double)std::max<int64_t>(hi - lo, 1);
-        const double edge = std::max(BIN_EDGE_MIN, std::cbrt(BIN_TARGET_INSERTIONS * h.volume / nins));
+        const double kappa = b->bin_scale > 0.0 ? b->bin_scale : 1.0;
+        const double edge = kappa * std::max(BIN_EDGE_MIN, std::cbrt(BIN_TARGET_INSERTIONS * h.volume / nins));
         BinSys &B = bins[s];
         int64_t nb = 1;
         for (int d = 0; d < 3; ++d) {
@@ -1153,11 +1154,33 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     HIP_TRY(upload(b->d_tile_n, tile_n));
     HIP_TRY(b->d_tiles.ensure(sizeof(double4) * ntiles));
 
+    const double kT_phase0 = (precision & PA_SINGLE_PHASE) ? 0.0 : kT;
+    if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_MIXED && b->bin_scale == 0.0 && G >= ((int64_t)1 << 18)) {
+        // autotune the bin edge multiplier once per batch: time the energy pipeline on the first
+        // (up to) 2^20 insertions for each candidate and keep the fastest
+        const double cand[4] = {0.75, 1.0, 1.25, 1.5};
+        const int64_t ge = std::min<int64_t>(G, (int64_t)1 << 20);
+        double best = 1.0, best_ms = INFINITY;
+        for (int rep = 0; rep < 2; ++rep)
+            for (double kappa : cand) {
+                b->bin_scale = kappa;
+                int64_t nl = 0;
+                HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+                st = run_energies(b, seg, 0, ge, 0, seed, precision, chunk, &nl, kT_phase0);
+                if (st) { b->bin_scale = 0.0; return st; }
+                HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+                HIP_TRY(hipEventSynchronize(b->ev[1]));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+                if (rep == 1 && ms < best_ms) { best_ms = ms; best = kappa; }
+            }
+        b->bin_scale = best;
+    }
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     int64_t launches = 0;
     // the mixed path runs phase 0 (hard-core rejection) unless PA_SINGLE_PHASE; the FP64 plain
     // path evaluates every insertion
-    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches, (precision & PA_SINGLE_PHASE) ? 0.0 : kT);
+    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches, kT_phase0);
     if (st) return st;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(launch_tiles(b->d_dU.as<double>(), b->d_tile_g0.as<int64_t>(), b->d_tile_n.as<int>(), ntiles, kT,
@@ -1598,6 +1621,21 @@ extern "C" int pa_widom_poses(pa_batch *b, int32_t sys, uint64_t seed, int64_t j
     HIP_TRY(hipMemcpyAsync(pos, b->d_pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipMemcpyAsync(quat, b->d_quat.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return ok();
+}
+
+extern "C" int pa_batch_set_bin_scale(pa_batch *b, double kappa)
+{
+    if (!b) return fail(PA_ERR_ARG, "NULL batch");
+    if (!(kappa >= 0.0) || kappa > 8.0) return fail(PA_ERR_ARG, "bin scale %g must be in (0, 8], or 0 to re-tune", kappa);
+    b->bin_scale = kappa;
+    return ok();
+}
+
+extern "C" int pa_batch_bin_scale(const pa_batch *b, double *kappa)
+{
+    if (!b || !kappa) return fail(PA_ERR_ARG, "NULL argument");
+    *kappa = b->bin_scale;
     return ok();
 }
 

@@ -2,6 +2,7 @@
 (src/batch.jl:72-432), built by the library's `pa_batch_create`."""
 from __future__ import annotations
 
+import ctypes
 import ctypes as C
 
 import numpy as np
@@ -114,6 +115,17 @@ class FrameworkBatch:
         if not self._h:
             raise _lib.ArgumentError("batch is closed")
         return self._h
+
+    @property
+    def bin_scale(self) -> float:
+        """Mixed-path bin edge multiplier (0: tuned by the first large widom call)."""
+        k = ctypes.c_double()
+        _lib.check(_lib.lib().pa_batch_bin_scale(self.handle, ctypes.byref(k)))
+        return k.value
+
+    @bin_scale.setter
+    def bin_scale(self, kappa: float):
+        _lib.check(_lib.lib().pa_batch_set_bin_scale(self.handle, float(kappa)))
 
 
 def nframeworks(batch: FrameworkBatch) -> int:

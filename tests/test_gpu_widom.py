@@ -389,3 +389,23 @@ def test_remainder_blocks_and_validation(config1):
     other = pa.Guest(g.sites, g.types, g.charges * 2)
     with pytest.raises(pa.ArgumentError, match="does not match the guest"):
         pa.widom(b, other, T=298.15, ninsert=100)
+
+
+def test_bin_scale_autotune_and_pinning(c1p):
+    """The first large mixed call tunes the bin edge multiplier; a pinned multiplier gives
+    bit-identical repeats, and another multiplier changes only FP32 rounding."""
+    fw, ff, g, ew, b0, o = c1p
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    assert b.bin_scale == 0.0
+    pa.widom(b, g, T=298.15, ninsert=300_000, seed=3)
+    print("tuned bin scale", b.bin_scale, flush=True)
+    assert b.bin_scale in (0.75, 1.0, 1.25, 1.5)
+    b.bin_scale = 1.0
+    r1 = pa.widom(b, g, T=298.15, ninsert=300_000, seed=3)[0]
+    r2 = pa.widom(b, g, T=298.15, ninsert=300_000, seed=3)[0]
+    assert r1 == r2
+    b.bin_scale = 1.25
+    r3 = pa.widom(b, g, T=298.15, ninsert=300_000, seed=3)[0]
+    assert abs(r3.ln_K_H - r1.ln_K_H) <= 1e-6 * max(1.0, abs(r1.ln_K_H))
+    with pytest.raises(pa.ArgumentError):
+        b.bin_scale = -1.0
