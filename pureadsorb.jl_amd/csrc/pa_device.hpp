@@ -114,8 +114,9 @@ struct GradArgs {
     int nsys;
     uint64_t seed;
     const int64_t *list_g;      // flattened insertion index of each weighted insertion
-    const double *list_w;       // its weight w
-    const double *list_wu;      // w * Delta U
+    const double *dU;           // Delta U of every insertion of the run
+    const double *msys;         // per system: max of -beta dU (the weight of insertion g is exp(-beta dU - m_s))
+    double beta;
     int64_t nlist;
     const uint4 *atomfx;
     const int *atype;

@@ -110,7 +110,7 @@ struct pa_batch {
     std::vector<int> atype_h;
     pa::DevAlloc d_corefx, d_cell_start, d_rho2;
     // gradient scratch
-    pa::DevAlloc d_list_g, d_list_w, d_list_wu, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
+    pa::DevAlloc d_list_g, d_msys, d_flag, d_count, d_sel_tmp, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch

@@ -28,6 +28,9 @@ hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
 size_t binned_lds_bytes_host(int cap, int ns);
 hipError_t launch_grad(const GradArgs &a, bool upper, hipStream_t st);
+hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
+                       unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
+hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -1088,7 +1091,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
 // pa_widom's body; also returns the flattened system offsets of the run (the energies stay in
 // b->d_dU for a following gradient pass).
 static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
-                      int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out, std::vector<int64_t> *seg_out)
+                      int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out, std::vector<int64_t> *seg_out,
+                      std::vector<double3> *sums_out = nullptr)
 {
     if (!b || !guest || !out) return fail(PA_ERR_ARG, "NULL argument");
     // argument checks in the reference's order and wording (src/widom.jl:302-330)
@@ -1236,6 +1240,10 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
             tUW += sUW[k];
             ntot += nb[k];
         }
+        if (sums_out) {
+            sums_out->resize(nsys);
+            (*sums_out)[s] = make_double3(M, tW, tUW);
+        }
         // _reduce (src/widom.jl:451-492) on the scaled sums: W = Ws * e^M
         const double Ws = tW / ntot, UWs = tUW / ntot;
         double varW = 0.0, varUW = 0.0, cov = 0.0;
@@ -1284,48 +1292,6 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
     return widom_core(b, guest, T, ninsert, seed, nblocks, run, chunk, precision, out, nullptr);
 }
 
-// Reciprocal part of the per-atom charge derivative over one system's kept k-vectors:
-// dE/dq_j = KE sum_k kpref_k 2 (cos(k.r_j) A_re(k) + sin(k.r_j) A_im(k)), with k.r_j = 2 pi n.f_j
-// from per-axis phase tables (n_1 >= 0 by the half-space enumeration, src/ewald.jl:142-157).
-// acck: 4 per k (w Re S_g, w Im S_g, wU Re S_g, wU Im S_g); adds into gq[2 j], gq[2 j + 1].
-static void reciprocal_dq(const FwGradHost &gh, const double *acck, double *gq)
-{
-    const int64_t nk = (int64_t)gh.kpref.size(), na = (int64_t)gh.frac.size() / 3;
-    if (nk == 0 || na == 0) return;
-    int nmax[3] = {0, 0, 0};
-    for (int64_t i = 0; i < nk; ++i)
-        for (int d = 0; d < 3; ++d) nmax[d] = std::max(nmax[d], std::abs(gh.kn[3 * i + d]));
-    const int w0 = nmax[0] + 1, w1 = 2 * nmax[1] + 1, w2 = 2 * nmax[2] + 1;
-    parallel_for(na, [&](int64_t b0, int64_t e0) {
-        std::vector<double> c0(w0), s0(w0), c1(w1), s1(w1), c2(w2), s2(w2);
-        for (int64_t j = b0; j < e0; ++j) {
-            for (int m = 0; m < w0; ++m) {
-                const double ph = TWO_PI * (m * gh.frac[3 * j]);
-                c0[m] = std::cos(ph); s0[m] = std::sin(ph);
-            }
-            for (int m = 0; m < w1; ++m) {
-                const double ph = TWO_PI * ((m - nmax[1]) * gh.frac[3 * j + 1]);
-                c1[m] = std::cos(ph); s1[m] = std::sin(ph);
-            }
-            for (int m = 0; m < w2; ++m) {
-                const double ph = TWO_PI * ((m - nmax[2]) * gh.frac[3 * j + 2]);
-                c2[m] = std::cos(ph); s2[m] = std::sin(ph);
-            }
-            double gw = 0.0, gu = 0.0;
-            for (int64_t i = 0; i < nk; ++i) {
-                const int m0 = gh.kn[3 * i], m1 = gh.kn[3 * i + 1] + nmax[1], m2 = gh.kn[3 * i + 2] + nmax[2];
-                const double pr = c0[m0] * c1[m1] - s0[m0] * s1[m1], pi = c0[m0] * s1[m1] + s0[m0] * c1[m1];
-                const double cr = pr * c2[m2] - pi * s2[m2], ci = pr * s2[m2] + pi * c2[m2];
-                const double f = 2.0 * gh.kpref[i];
-                gw += f * (cr * acck[4 * i] + ci * acck[4 * i + 1]);
-                gu += f * (cr * acck[4 * i + 2] + ci * acck[4 * i + 3]);
-            }
-            gq[2 * j] += KE * gw;
-            gq[2 * j + 1] += KE * gu;
-        }
-    });
-}
-
 extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed,
                              int32_t nblocks, int32_t run, uint32_t precision, pa_widom_result *out, double *dlnkh_deps,
                              double *dlnkh_dsig, double *dqst_deps, double *dqst_dsig, double *dlnkh_dq, double *dqst_dq)
@@ -1365,7 +1331,8 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
         out = res_local.data();
     }
     std::vector<int64_t> seg;
-    int st = widom_core(b, guest, T, ninsert, seed, nblocks, run, 0, precision, out, &seg);
+    std::vector<double3> sums;
+    int st = widom_core(b, guest, T, ninsert, seed, nblocks, run, 0, precision, out, &seg, &sums);
     if (st) return st;
     const double kT = KB * T, beta = 1.0 / kT;
     const int64_t G = seg[nsys];
@@ -1380,29 +1347,31 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     HIP_TRY(hipEventCreate(&evp.a));
     HIP_TRY(hipEventCreate(&evp.b));
     hipEvent_t g0 = evp.a, g1 = evp.b;
-    std::vector<double> dU(G);
-    HIP_TRY(hipMemcpy(dU.data(), b->d_dU.p, sizeof(double) * G, hipMemcpyDeviceToHost));
 
-    // weighted insertions: w = exp(-beta U - m_s) relative to the system's best insertion;
+    // weighted insertions, selected on the device: w = exp(-beta U - m_s) relative to the system's
+    // best insertion (m_s and the scaled sums W = sum w, R = sum w U come from the Widom tiles);
     // weights below e^-40 change no sum at FP64 resolution
-    std::vector<int64_t> lg;
-    std::vector<double> lw, lwu, Wsum(nsys, 0.0), Rsum(nsys, 0.0);
+    std::vector<double> msys(nsys), Wsum(nsys), Rsum(nsys);
     for (int s = 0; s < nsys; ++s) {
-        double m = -INFINITY;
-        for (int64_t g = seg[s]; g < seg[s + 1]; ++g) m = std::max(m, -beta * dU[g]);
-        if (!(m > -INFINITY)) continue;
-        for (int64_t g = seg[s]; g < seg[s + 1]; ++g) {
-            const double x = -beta * dU[g] - m;
-            if (!(x > -40.0)) continue;
-            const double w = std::exp(x);
-            lg.push_back(g);
-            lw.push_back(w);
-            lwu.push_back(w * dU[g]);
-            Wsum[s] += w;
-            Rsum[s] += w * dU[g];
-        }
+        msys[s] = sums[s].x;
+        Wsum[s] = sums[s].y;
+        Rsum[s] = sums[s].z;
     }
-    const int64_t nlist = (int64_t)lg.size();
+    HIP_TRY(upload(b->d_msys, msys));
+    size_t sel_bytes = 0;
+    HIP_TRY(grad_select(nullptr, nullptr, nsys, nullptr, beta, G, nullptr, nullptr, nullptr, nullptr, &sel_bytes,
+                        b->stream));
+    HIP_TRY(b->d_sel_tmp.ensure(sel_bytes));
+    HIP_TRY(b->d_flag.ensure(std::max<int64_t>(G, 1)));
+    HIP_TRY(b->d_list_g.ensure(sizeof(int64_t) * std::max<int64_t>(G, 1)));
+    HIP_TRY(b->d_count.ensure(sizeof(int64_t)));
+    HIP_TRY(hipEventRecord(g0, b->stream));
+    HIP_TRY(grad_select(b->d_dU.as<double>(), b->d_seg.as<int64_t>(), nsys, b->d_msys.as<double>(), beta, G,
+                        b->d_flag.as<unsigned char>(), b->d_list_g.as<int64_t>(), b->d_count.as<int64_t>(),
+                        b->d_sel_tmp.p, &sel_bytes, b->stream));
+    int64_t nlist = 0;
+    HIP_TRY(hipMemcpyAsync(&nlist, b->d_count.p, sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     std::vector<int64_t> qoff(nsys + 1, 0), koff(nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
         qoff[s + 1] = qoff[s] + b->info[b->fw_of[s]].natoms;
@@ -1415,16 +1384,12 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     }
     const size_t nlj = (size_t)nsys * NS * NTC * 4;
     const bool charged = b->gdev.charged != 0;
-    HIP_TRY(upload(b->d_list_g, lg));
-    HIP_TRY(upload(b->d_list_w, lw));
-    HIP_TRY(upload(b->d_list_wu, lwu));
     HIP_TRY(upload(b->d_qoff, qoff));
     HIP_TRY(upload(b->d_koff, koff));
     HIP_TRY(upload(b->d_pairg, pairg));
     HIP_TRY(b->d_acc_lj.ensure(sizeof(double) * nlj));
     HIP_TRY(b->d_acc_q.ensure(sizeof(double) * std::max<int64_t>(1, 2 * qoff[nsys])));
     HIP_TRY(b->d_acc_k.ensure(sizeof(double) * std::max<int64_t>(1, 4 * koff[nsys])));
-    HIP_TRY(hipEventRecord(g0, b->stream));
     HIP_TRY(hipMemsetAsync(b->d_acc_lj.p, 0, sizeof(double) * nlj, b->stream));
     HIP_TRY(hipMemsetAsync(b->d_acc_q.p, 0, sizeof(double) * std::max<int64_t>(1, 2 * qoff[nsys]), b->stream));
     HIP_TRY(hipMemsetAsync(b->d_acc_k.p, 0, sizeof(double) * std::max<int64_t>(1, 4 * koff[nsys]), b->stream));
@@ -1435,8 +1400,9 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     A.nsys = nsys;
     A.seed = seed;
     A.list_g = b->d_list_g.as<int64_t>();
-    A.list_w = b->d_list_w.as<double>();
-    A.list_wu = b->d_list_wu.as<double>();
+    A.dU = b->d_dU.as<double>();
+    A.msys = b->d_msys.as<double>();
+    A.beta = beta;
     A.nlist = nlist;
     A.atomfx = b->d_atomfx.as<uint4>();
     A.atype = b->d_atype.as<int>();
@@ -1453,14 +1419,12 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     A.acc_k = b->d_acc_k.as<double>();
     A.koff = b->d_koff.as<int64_t>();
     HIP_TRY(launch_grad(A, b->upper_all, b->stream));
+    if (charged && want_q) HIP_TRY(launch_grad_recip(A, b->d_k64.as<double4>(), qoff[nsys], b->stream));
     HIP_TRY(hipEventRecord(g1, b->stream));
-    std::vector<double> acc_lj(nlj), acc_q(2 * qoff[nsys]), acc_k(4 * koff[nsys]);
+    std::vector<double> acc_lj(nlj), acc_q(2 * qoff[nsys]);
     HIP_TRY(hipMemcpyAsync(acc_lj.data(), b->d_acc_lj.p, sizeof(double) * nlj, hipMemcpyDeviceToHost, b->stream));
     if (!acc_q.empty())
         HIP_TRY(hipMemcpyAsync(acc_q.data(), b->d_acc_q.p, sizeof(double) * acc_q.size(), hipMemcpyDeviceToHost,
-                               b->stream));
-    if (!acc_k.empty())
-        HIP_TRY(hipMemcpyAsync(acc_k.data(), b->d_acc_k.p, sizeof(double) * acc_k.size(), hipMemcpyDeviceToHost,
                                b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     float gms = 0.0f;
@@ -1548,7 +1512,6 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
                     for (int m = 0; m < nimg; ++m) { gq[2 * (m * nu + a)] = sw / nimg; gq[2 * (m * nu + a) + 1] = su / nimg; }
                 }
             }
-            if (charged) reciprocal_dq(gh, acc_k.data() + 4 * koff[s], gq);
             // net-charge term of constant_offset: d/dq_j of -KE pi/(2 V alpha^2) ((Q_h + Q_g)^2 - Q_h^2)
             double Qg = 0.0;
             for (int a = 0; a < NS; ++a) Qg += b->gq[a];

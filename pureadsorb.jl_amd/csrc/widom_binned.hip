@@ -85,6 +85,7 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
         const int *cs = a.cell_start + F.cg_off;
         const uint4 *cat = a.corefx + F.atom_off;
         const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
+        const int nc0 = F.cg_nc[0], nc1 = F.cg_nc[1], nc2 = F.cg_nc[2];
         int cnt[3], start[3];
         bool hit = false;
         for (int t = 0; t < NS && !hit; ++t) {
@@ -92,14 +93,23 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
                 const int nc = F.cg_nc[d];
                 const int h = (int)(((uint64_t)sp[t][d] * (uint64_t)nc) >> 32);
                 cnt[d] = min(2 * B.reach[d] + 1, nc);
-                start[d] = cnt[d] == nc ? 0 : (h - B.reach[d] + nc) % nc;
+                // start in [0, nc) without a division: h - reach lies in (-nc, nc) because reach < nc
+                int st = h - B.reach[d];
+                st = st < 0 ? st + nc : st;
+                start[d] = cnt[d] == nc ? 0 : st;
             }
+            // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo
             for (int i2 = 0; i2 < cnt[2] && !hit; ++i2) {
-                const int c2 = (start[2] + i2) % F.cg_nc[2];
+                int c2 = start[2] + i2;
+                c2 = c2 >= nc2 ? c2 - nc2 : c2;
                 for (int i1 = 0; i1 < cnt[1] && !hit; ++i1) {
-                    const int c1 = (start[1] + i1) % F.cg_nc[1];
+                    int c1 = start[1] + i1;
+                    c1 = c1 >= nc1 ? c1 - nc1 : c1;
+                    const int row = nc0 * (c1 + nc1 * c2);
                     for (int i0 = 0; i0 < cnt[0] && !hit; ++i0) {
-                        const int c = (start[0] + i0) % F.cg_nc[0] + F.cg_nc[0] * (c1 + F.cg_nc[1] * c2);
+                        int c0 = start[0] + i0;
+                        c0 = c0 >= nc0 ? c0 - nc0 : c0;
+                        const int c = c0 + row;
                         for (int jj = cs[c]; jj < cs[c + 1]; ++jj) {
                             const uint4 at = cat[jj];
                             const float dx = (float)(int32_t)(sp[t][0] - at.x);

@@ -16,6 +16,8 @@
 // Pair and k-space terms are FP32; every sum is FP64.
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "pa_device.hpp"
 
 namespace pa {
@@ -52,8 +54,9 @@ __global__ __launch_bounds__(64) void widom_grad_kernel(GradArgs a)
     const int64_t idx = blockIdx.x;
     if (idx >= a.nlist) return;
     const int64_t g = a.list_g[idx];
-    const double w = a.list_w[idx], wu = a.list_wu[idx];
     const int s = find_sys_g(a.seg, a.nsys, g);
+    const double du = a.dU[g];
+    const double w = exp(-a.beta * du - a.msys[s]), wu = w * du;
     const int64_t j = g - a.seg[s];
     const FwDesc &F = a.fw[a.fw_of[s]];
     double f[3], q[4];
@@ -181,6 +184,76 @@ __global__ __launch_bounds__(64) void widom_grad_kernel(GradArgs a)
     }
 }
 
+// Flags the insertions whose weight relative to their system's best insertion exceeds e^-40.
+__global__ __launch_bounds__(256) void grad_flag_kernel(const double *dU, const int64_t *seg, int nsys, const double *msys,
+                                                        double beta, int64_t G, unsigned char *flag)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= G) return;
+    const int s = find_sys_g(seg, nsys, g);
+    const double x = -beta * dU[g] - msys[s];
+    flag[g] = x > -40.0 ? 1 : 0;
+}
+
+// Reciprocal part of every framework atom's charge derivative, one work-item per (system, atom):
+// KE sum_k kprefactor 2 (cos(k.r_j) A_re(k) + sin(k.r_j) A_im(k)), with k.r_j = 2 pi n.f_j taken
+// from the fixed-point coordinates (an exact wrapping integer phase), FP64 sin/cos of pi x.
+__global__ __launch_bounds__(256) void grad_recip_kernel(GradArgs a, const double4 *k64, int64_t natoms_total)
+{
+#pragma clang fp contract(off)
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (x >= natoms_total) return;
+    int lo = 0, hi = a.nsys - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.qoff[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    const int s = lo;
+    const int j = (int)(x - a.qoff[s]);
+    const FwDesc &F = a.fw[a.fw_of[s]];
+    const uint4 at = a.atomfx[F.atom_off + j];
+    const int4 *kn = a.kn + F.k_off;
+    const double4 *kp = k64 + F.k_off;
+    const double *acck = a.acc_k + 4 * a.koff[s];
+    double gw = 0.0, gu = 0.0;
+    for (int k = 0; k < F.nk; ++k) {
+        const int4 nn = kn[k];
+        const uint32_t ph = (uint32_t)nn.x * at.x + (uint32_t)nn.y * at.y + (uint32_t)nn.z * at.z;
+        const double rev = (double)(int32_t)ph * 2.3283064365386963e-10;
+        double sn, cs;
+        sincospi(2.0 * rev, &sn, &cs);
+        const double f = 2.0 * kp[k].w;
+        gw += f * (cs * acck[4 * k] + sn * acck[4 * k + 1]);
+        gu += f * (cs * acck[4 * k + 2] + sn * acck[4 * k + 3]);
+    }
+    double *accq = a.acc_q + 2 * a.qoff[s];
+    accq[2 * j] += KE * gw;
+    accq[2 * j + 1] += KE * gu;
+}
+
+hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
+                       unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st)
+{
+    if (!tmp) {
+        return hipcub::DeviceSelect::Flagged(nullptr, *tmp_bytes, hipcub::CountingInputIterator<int64_t>(0), flag, list,
+                                             count, G, st);
+    }
+    hipLaunchKernelGGL(grad_flag_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, st, dU, seg, nsys, msys, beta,
+                       G, flag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceSelect::Flagged(tmp, *tmp_bytes, hipcub::CountingInputIterator<int64_t>(0), flag, list, count,
+                                         G, st);
+}
+
+hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st)
+{
+    if (natoms_total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(grad_recip_kernel, dim3((unsigned)((natoms_total + 255) / 256)), dim3(256), 0, st, a, k64,
+                       natoms_total);
+    return hipGetLastError();
+}
+
 template <int NS>
 static void launch_grad_ns(const GradArgs &a, bool upper, hipStream_t st)
 {
@@ -202,8 +275,6 @@ hipError_t launch_grad(const GradArgs &a, bool upper, hipStream_t st)
     for (int64_t off = 0; off < a.nlist; off += CH) {
         GradArgs c = a;
         c.list_g = a.list_g + off;
-        c.list_w = a.list_w + off;
-        c.list_wu = a.list_wu + off;
         c.nlist = std::min(CH, a.nlist - off);
         switch (a.guest.nsites) {
         case 1: launch_grad_ns<1>(c, upper, st); break;
