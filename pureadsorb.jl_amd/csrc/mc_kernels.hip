@@ -130,6 +130,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         // (1) host-guest real space of the new pose
         double hsum = 0.0;
         if (has_new)
+            #pragma unroll 1
             for (int j = tid; j < natoms; j += MC_WG) {
                 const double4 h = atoms[j];
                 const double hp[3] = {h.x, h.y, h.z};
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         // other live guest, one (other guest, site, site) triple per lane step
         double gsum = 0.0;
         const int ntrip = occ * NS * NS;
+        #pragma unroll 1
         for (int x = tid; x < ntrip; x += MC_WG) {
             const int jj = x / (NS * NS), sa = (x / NS) % NS, sb = x % NS;
             const int j = gr0 + jj;
@@ -167,6 +169,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         // (3) reciprocal change against the running structure factor, dS kept per lane
         double rsum = 0.0;
         if (G.charged)
+            #pragma unroll 1
             for (int k = tid; k < nk; k += MC_WG) {
                 const double4 kv = ks[k];
                 double sre = 0.0, sim = 0.0;
@@ -275,19 +278,35 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         return t;
     };
 
-    if (a.mode == 0) {
-        for (int64_t m = 0; m < a.nmoves; ++m) attempt(a.movetype);
-    } else {
-        for (int64_t c = 0; c < a.ncycles; ++c) {
-            if (occ > maxocc) maxocc = occ;
-            const int64_t len = occ > a.min_cycle ? occ : a.min_cycle;
-            for (int64_t m = 0; m < len; ++m) attempt(draw_type());
-            if (occ > maxocc) maxocc = occ;
-            if (a.sampling && tid == 0) {
-                a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c] = E;
-                a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c] = (double)occ;
+    // one attempt site for both modes; in cycle mode a cycle of chain n is
+    // max(occupancy at its start, min_cycle) attempts, and its sample is taken at its end
+    int64_t c = 0, m = 0, len = 0;
+    for (;;) {
+        int type;
+        if (a.mode == 0) {
+            if (m >= a.nmoves) break;
+            ++m;
+            type = a.movetype;
+        } else {
+            if (m == len) {
+                if (c > 0) {
+                    if (occ > maxocc) maxocc = occ;
+                    if (a.sampling && tid == 0) {
+                        a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = E;
+                        a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = (double)occ;
+                    }
+                }
+                if (c == a.ncycles) break;
+                if (occ > maxocc) maxocc = occ;
+                len = occ > a.min_cycle ? occ : a.min_cycle;
+                m = 0;
+                ++c;
+                continue;
             }
+            ++m;
+            type = draw_type();
         }
+        attempt(type);
     }
     if (tid == 0) {
         a.occupancy[n] = occ;

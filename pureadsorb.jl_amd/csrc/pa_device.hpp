@@ -130,6 +130,14 @@ struct GradArgs {
     const int64_t *qoff;        // nsys: first atom slot of the system in acc_q
     double *acc_k;              // per system slice of 4 x nk: w Re S_g, w Im S_g, wU Re S_g, wU Im S_g
     const int64_t *koff;        // nsys: first k slot of the system in acc_k
+    // gather form
+    const int64_t *loff;        // nsys + 1: each system's range of the weighted-insertion list
+    uint32_t *lsite;            // per listed insertion: NS x 3 fixed-point site coordinates
+    double2 *lw;                // per listed insertion: w, w dU
+    const uint4 *corefx;        // atoms sorted by hard-core grid cell: fixed point + compact type
+    const int *coreidx;         // original index of each sorted atom
+    const int4 *blk;            // per gather workgroup: system, first sorted atom, list range (relative)
+    const int4 *kblk;           // per k-space workgroup: system, first k-vector, list range (relative)
 };
 
 // One site pair of the move path (FP64): Lennard-Jones inside rc_lj plus the screened Coulomb

@@ -27,7 +27,8 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
 size_t binned_lds_bytes_host(int cap, int ns);
-hipError_t launch_grad(const GradArgs &a, bool upper, hipStream_t st);
+hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
+hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
 hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st);
@@ -486,7 +487,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<int4> kn;
     std::vector<float2> kc;
     std::vector<uint4> corefx;
-    std::vector<int> cell_start;
+    std::vector<int> coreidx, cell_start;
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
@@ -546,12 +547,16 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             for (int c = 0; c < nct; ++c) cnt[c + 1] += cnt[c];
             cell_start.insert(cell_start.end(), cnt.begin(), cnt.end());
             std::vector<uint4> sorted(n);
+            std::vector<int> sidx(n);
             std::vector<int> cur(cnt.begin(), cnt.end() - 1);
             for (int j = 0; j < n; ++j) {
                 const uint4 &x = atomfx[D.atom_off + j];
-                sorted[cur[cell_of[j]]++] = make_uint4(x.x, x.y, x.z, (uint32_t)atype[D.atom_off + j]);
+                const int pos = cur[cell_of[j]]++;
+                sorted[pos] = make_uint4(x.x, x.y, x.z, (uint32_t)atype[D.atom_off + j]);
+                sidx[pos] = j;
             }
             corefx.insert(corefx.end(), sorted.begin(), sorted.end());
+            coreidx.insert(coreidx.end(), sidx.begin(), sidx.end());
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
         FwGradHost &gh = b->fgh[u];
@@ -790,6 +795,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     HIP_TRY(upload(b->d_pair64, pair64));
     HIP_TRY(upload(b->d_pair32, pair32));
     HIP_TRY(upload(b->d_corefx, corefx));
+    HIP_TRY(upload(b->d_coreidx, coreidx));
     HIP_TRY(upload(b->d_cell_start, cell_start));
     *out = b.release();
     return ok();
@@ -1372,6 +1378,40 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     int64_t nlist = 0;
     HIP_TRY(hipMemcpyAsync(&nlist, b->d_count.p, sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    // each system's slice of the list (ascending g, so systems are contiguous) and the gather
+    // workgroups: 256 cell-sorted atoms of one system that has weighted insertions
+    std::vector<int64_t> loff(nsys + 1, 0);
+    HIP_TRY(b->d_loff.ensure(sizeof(int64_t) * (nsys + 1)));
+    HIP_TRY(grad_offsets(b->d_list_g.as<int64_t>(), nlist, b->d_seg.as<int64_t>(), nsys, b->d_loff.as<int64_t>(),
+                         b->stream));
+    HIP_TRY(hipMemcpyAsync(loff.data(), b->d_loff.p, sizeof(int64_t) * (nsys + 1), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    // gather workgroups: (256 cell-sorted atoms, chunk of the system's list) and k-space
+    // workgroups: (256 coupled k-vectors, chunk of the list); chunks keep a system with many
+    // weighted insertions from serialising on a few workgroups
+    const bool charged = b->gdev.charged != 0;
+    const bool kspace = charged && want_q;
+    constexpr int64_t GRAD_CHUNK = 1024;
+    std::vector<int4> blk, kblk;
+    for (int s = 0; s < nsys; ++s) {
+        const int64_t nl = loff[s + 1] - loff[s];
+        if (nl <= 0) continue;
+        if (nl > 0x7fffffff) return fail(PA_ERR_ARG, "system %d: %lld weighted insertions exceed the int32 range", s + 1,
+                                         (long long)nl);
+        const int n = b->info[b->fw_of[s]].natoms, nk = b->info[b->fw_of[s]].nk;
+        for (int64_t c0 = 0; c0 < nl; c0 += GRAD_CHUNK) {
+            const int c1 = (int)std::min<int64_t>(nl, c0 + GRAD_CHUNK);
+            for (int a0 = 0; a0 < n; a0 += 256) blk.push_back(make_int4(s, a0, (int)c0, c1));
+            if (kspace)
+                for (int k0 = 0; k0 < nk; k0 += 256) kblk.push_back(make_int4(s, k0, (int)c0, c1));
+        }
+    }
+    if (blk.size() > 0x7fffffffu || kblk.size() > 0x7fffffffu)
+        return fail(PA_ERR_ARG, "too many gradient workgroups (%zu, %zu)", blk.size(), kblk.size());
+    HIP_TRY(upload(b->d_blk, blk));
+    HIP_TRY(upload(b->d_kblk, kblk));
+    HIP_TRY(b->d_lsite.ensure(sizeof(uint32_t) * 3 * NS * std::max<int64_t>(nlist, 1)));
+    HIP_TRY(b->d_lw.ensure(sizeof(double2) * std::max<int64_t>(nlist, 1)));
     std::vector<int64_t> qoff(nsys + 1, 0), koff(nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
         qoff[s + 1] = qoff[s] + b->info[b->fw_of[s]].natoms;
@@ -1383,7 +1423,6 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
         pairg[i] = make_float4((float)(sg * sg), (float)(4.0 * ep), sg > 0 ? (float)(1.0 / sg) : 0.0f, 0.0f);
     }
     const size_t nlj = (size_t)nsys * NS * NTC * 4;
-    const bool charged = b->gdev.charged != 0;
     HIP_TRY(upload(b->d_qoff, qoff));
     HIP_TRY(upload(b->d_koff, koff));
     HIP_TRY(upload(b->d_pairg, pairg));
@@ -1418,7 +1457,16 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     A.qoff = b->d_qoff.as<int64_t>();
     A.acc_k = b->d_acc_k.as<double>();
     A.koff = b->d_koff.as<int64_t>();
-    HIP_TRY(launch_grad(A, b->upper_all, b->stream));
+    A.loff = b->d_loff.as<int64_t>();
+    A.lsite = b->d_lsite.as<uint32_t>();
+    A.lw = b->d_lw.as<double2>();
+    A.corefx = b->d_corefx.as<uint4>();
+    A.coreidx = b->d_coreidx.as<int>();
+    A.blk = b->d_blk.as<int4>();
+    A.kblk = b->d_kblk.as<int4>();
+    // the Coulomb and k-space sums feed only the charge derivatives
+    if (!want_q) A.guest.charged = 0;
+    HIP_TRY(launch_grad(A, b->upper_all, (int)blk.size(), (int)kblk.size(), b->stream));
     if (charged && want_q) HIP_TRY(launch_grad_recip(A, b->d_k64.as<double4>(), qoff[nsys], b->stream));
     HIP_TRY(hipEventRecord(g1, b->stream));
     std::vector<double> acc_lj(nlj), acc_q(2 * qoff[nsys]);

@@ -1,18 +1,21 @@
-// widom_grad.hip -- adjoint kernel of the Widom estimators (gradients of ln K_H and q_st).
+// widom_grad.hip -- adjoint kernels of the Widom estimators (gradients of ln K_H and q_st).
 //
-// The host runs the energy pipeline first, so every insertion's Delta U is known, and lists the
-// insertions whose Boltzmann weight relative to the system's best insertion matters in FP64
-// (w = exp(-beta U - m_s) > e^-36). This kernel takes one such insertion per wavefront:
-//  * lanes sweep the framework's atoms with the exact 32-bit fixed-point minimum image; pairs
-//    inside the cutoff are compacted into an LDS ring (ballot + mbcnt) and evaluated on full
-//    wavefronts;
-//  * each pair adds its Lennard-Jones derivatives dU/d eps_sh = 4 (x^12 - x^6) and
-//    dU/d sigma_sh = 4 eps (12 x^12 - 6 x^6) / sigma into per-(site, host type) sums in LDS,
-//    and its Coulomb factor dU/dq_j = KE q_s erfc(alpha r)/r, times w and w U, into the atom's
-//    FP64 accumulators (global atomics);
-//  * lanes over the coupled k-vectors add w S_g(k) and w U S_g(k) (the guest structure factor)
-//    into per-system FP64 accumulators, from which the host forms the reciprocal part
-//    KE sum_k kprefactor 2 Re(conj(e^{i k r_j}) A_k) of every atom's charge derivative.
+// The Widom pipeline runs first, so every insertion's Delta U is known. grad_select lists, on
+// the device, the insertions whose weight relative to their system's best insertion matters in
+// FP64 (w = exp(-beta U - m_s) > e^-40). Then, in gather form (no per-pair atomics):
+//  * grad_prep_kernel stores every listed insertion's sites as 32-bit fixed-point fractional
+//    coordinates and its (w, w dU);
+//  * widom_grad_gather_kernel: one workgroup per (block of 256 framework atoms of one system,
+//    chunk of that system's listed insertions), atoms in hard-core grid-cell order, insertions
+//    streamed through LDS in tiles; each work-item owns one atom and
+//    sums over every (insertion, site) pair inside the cutoff, with the exact fixed-point minimum
+//    image: the Coulomb factor KE q_s erfc(alpha r)/r times w and w dU into the atom's own FP64
+//    sums, and the Lennard-Jones derivatives dU/d eps_sh = 4 (x^12 - x^6) and
+//    dU/d sigma_sh = 4 eps (12 x^12 - 6 x^6)/sigma times w and w dU into per-site sums of the
+//    atom's type, reduced over the workgroup per (site, type);
+//  * grad_kspace_kernel: one workgroup per (256 coupled k-vectors of one system, chunk of that
+//    system's listed insertions) sums w S_g(k) and w dU S_g(k), one work-item per k-vector;
+//  * grad_recip_kernel turns those into every atom's reciprocal charge derivative.
 // Pair and k-space terms are FP32; every sum is FP64.
 #include <algorithm>
 
@@ -22,7 +25,8 @@
 
 namespace pa {
 
-constexpr int GRAD_RING = 512;   // holds a partial wave (< 64) plus one 64-atom sweep of 6 sites
+constexpr int GRAD_WG = 256;
+constexpr int GRAD_TILE = 256;
 
 __device__ inline int find_sys_g(const int64_t *seg, int nsys, int64_t g)
 {
@@ -34,154 +38,189 @@ __device__ inline int find_sys_g(const int64_t *seg, int nsys, int64_t g)
     return lo;
 }
 
-template <int NS, bool CHARGED, bool UPPER>
-__global__ __launch_bounds__(64) void widom_grad_kernel(GradArgs a)
+template <int NS>
+__global__ __launch_bounds__(256) void grad_prep_kernel(GradArgs a)
 {
-    __shared__ float ring_r2[GRAD_RING];
-    __shared__ int ring_atom[GRAD_RING];
-    __shared__ int ring_site[GRAD_RING];
-    __shared__ double lj_sum[NS * 64 * 2];   // per (site, type): dU/deps, dU/dsig for this insertion
-    __shared__ float4 pair_lds[NS * 64];
-
-    static_assert(63 + 64 * NS <= GRAD_RING, "ring too small for one sweep");
-    const int lane = threadIdx.x;
-    const uint64_t below = (1ull << lane) - 1ull;
-    const int ntc = a.ntypes_c;
-    for (int i = lane; i < NS * ntc; i += 64) pair_lds[i] = a.pairg[i];
-    for (int i = lane; i < NS * ntc * 2; i += 64) lj_sum[i] = 0.0;
-    __syncthreads();
-
-    const int64_t idx = blockIdx.x;
+#pragma clang fp contract(off)
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= a.nlist) return;
     const int64_t g = a.list_g[idx];
     const int s = find_sys_g(a.seg, a.nsys, g);
-    const double du = a.dU[g];
-    const double w = exp(-a.beta * du - a.msys[s]), wu = w * du;
     const int64_t j = g - a.seg[s];
     const FwDesc &F = a.fw[a.fw_of[s]];
     double f[3], q[4];
     pose_uniforms(a.seed, s, j, f, q);
-    uint32_t sx[NS], sy[NS], sz[NS];
+    uint32_t *out = a.lsite + (size_t)idx * NS * 3;
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
         double r[3], df[3];
         rotate(q, a.guest.site[t], r);
         matvec(F.invA, r, df);
-        sx[t] = __builtin_amdgcn_readfirstlane(to_fixed(f[0] + df[0]));
-        sy[t] = __builtin_amdgcn_readfirstlane(to_fixed(f[1] + df[1]));
-        sz[t] = __builtin_amdgcn_readfirstlane(to_fixed(f[2] + df[2]));
+        for (int d = 0; d < 3; ++d) out[3 * t + d] = to_fixed(f[d] + df[d]);
     }
+    const double du = a.dU[g];
+    const double w = exp(-a.beta * du - a.msys[s]);
+    a.lw[idx] = make_double2(w, w * du);
+}
+
+template <int NS, bool CHARGED, bool UPPER>
+__global__ __launch_bounds__(GRAD_WG) void widom_grad_gather_kernel(GradArgs a)
+{
+    __shared__ uint32_t t_site[GRAD_TILE * NS * 3];
+    __shared__ double2 t_w[GRAD_TILE];
+    __shared__ double lj_red[NS * 64 * 4];
+    const int tid = threadIdx.x;
+    const int4 bk = a.blk[blockIdx.x];   // system, first sorted atom, list range
+    const int s = bk.x;
+    const FwDesc &F = a.fw[a.fw_of[s]];
+    const int ntc = a.ntypes_c;
+    for (int i = tid; i < NS * ntc * 4; i += GRAD_WG) lj_red[i] = 0.0;
+    const int j = bk.y + tid;
+    const bool valid = j < F.natoms;
+    uint4 at = make_uint4(0u, 0u, 0u, 0u);
+    if (valid) at = a.corefx[F.atom_off + j];
+    const int ht = valid ? (int)at.w : 0;
+    float4 pr[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) pr[t] = a.pairg[t * ntc + ht];
     const float A0 = F.Af[0], A1 = F.Af[1], A2 = F.Af[2], A3 = F.Af[3], A4 = F.Af[4], A5 = F.Af[5], A6 = F.Af[6],
                 A7 = F.Af[7], A8 = F.Af[8];
     const float alpha = F.alphaf;
-    const uint4 *atoms = a.atomfx + F.atom_off;
-    const int *types = a.atype + F.atom_off;
-    double *accq = a.acc_q + 2 * a.qoff[s];
-
-    auto drain = [&](uint32_t slot, bool active) {
-        if (!active) return;
-        float r2 = fmaxf(ring_r2[slot], 1.0e-30f);
-        const int n = ring_atom[slot], site = ring_site[slot];
-        const int ht = types[n];
-        if (r2 < a.rc_lj2f) {
-            const float4 p = pair_lds[site * ntc + ht];
-            const float x2 = p.x * __builtin_amdgcn_rcpf(r2);
-            const float x6 = x2 * x2 * x2;
-            const float dde = 4.0f * x6 * (x6 - 1.0f);                       // dU/d eps_sh
-            const float dds = p.y * (12.0f * x6 * x6 - 6.0f * x6) * p.z;     // dU/d sigma_sh
-            unsafeAtomicAdd(&lj_sum[2 * (site * ntc + ht)], (double)dde);
-            unsafeAtomicAdd(&lj_sum[2 * (site * ntc + ht) + 1], (double)dds);
-        }
-        if (CHARGED && r2 < a.rc_ew2f) {
-            constexpr float XLO = (float)PAIR_ERFC_XLO;
-            constexpr float XA = 4.0f / (1.0f - XLO);
-            constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float z = alpha * (r2 * rinv);
-            const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
-            const float xi = XA * tt + XB, xi2 = 2.0f * xi;
-            float d = 0.0f, dd = 0.0f;
+    double ge[NS], he[NS], gs[NS], hs[NS];
 #pragma unroll
-            for (int jj = 7; jj >= 1; --jj) {
-                const float tmp = d;
-                d = xi2 * d - dd + pair_erfc_c32(jj);
-                dd = tmp;
+    for (int t = 0; t < NS; ++t) ge[t] = he[t] = gs[t] = hs[t] = 0.0;
+    double gq = 0.0, hq = 0.0;
+    const int64_t l0 = a.loff[s] + bk.z, l1 = a.loff[s] + bk.w;
+    for (int64_t base = l0; base < l1; base += GRAD_TILE) {
+        const int nt = (int)((l1 - base) < GRAD_TILE ? (l1 - base) : GRAD_TILE);
+        __syncthreads();
+        for (int i = tid; i < nt * NS * 3; i += GRAD_WG) t_site[i] = a.lsite[(size_t)base * NS * 3 + i];
+        for (int i = tid; i < nt; i += GRAD_WG) t_w[i] = a.lw[base + i];
+        __syncthreads();
+        if (!valid) continue;
+        for (int i = 0; i < nt; ++i) {
+            const double2 w = t_w[i];
+            float fq = 0.0f;
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                const float dx = (float)(int32_t)(t_site[(i * NS + t) * 3 + 0] - at.x);
+                const float dy = (float)(int32_t)(t_site[(i * NS + t) * 3 + 1] - at.y);
+                const float dz = (float)(int32_t)(t_site[(i * NS + t) * 3 + 2] - at.z);
+                const float ex = A0 * dx + A3 * dy + A6 * dz;
+                const float ey = UPPER ? A4 * dy + A7 * dz : A1 * dx + A4 * dy + A7 * dz;
+                const float ez = UPPER ? A8 * dz : A2 * dx + A5 * dy + A8 * dz;
+                const float r2 = fmaxf(ex * ex + ey * ey + ez * ez, 1.0e-30f);
+                if (r2 < a.rc_lj2f) {
+                    const float x2 = pr[t].x * __builtin_amdgcn_rcpf(r2);
+                    const float x6 = x2 * x2 * x2;
+                    const float de = 4.0f * x6 * (x6 - 1.0f);
+                    const float ds = pr[t].y * (12.0f * x6 * x6 - 6.0f * x6) * pr[t].z;
+                    ge[t] += w.x * (double)de;
+                    he[t] += w.y * (double)de;
+                    gs[t] += w.x * (double)ds;
+                    hs[t] += w.y * (double)ds;
+                }
+                if (CHARGED && r2 < a.rc_ew2f) {
+                    constexpr float XLO = (float)PAIR_ERFC_XLO;
+                    constexpr float XA = 4.0f / (1.0f - XLO);
+                    constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    const float z = alpha * (r2 * rinv);
+                    const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
+                    const float xi = XA * tt + XB, xi2 = 2.0f * xi;
+                    float d = 0.0f, dd = 0.0f;
+#pragma unroll
+                    for (int jj = 7; jj >= 1; --jj) {
+                        const float tmp = d;
+                        d = xi2 * d - dd + pair_erfc_c32(jj);
+                        dd = tmp;
+                    }
+                    const float arg = xi * d - dd + 0.5f * pair_erfc_c32(0) - alpha * alpha * r2;
+                    fq += a.guest.qke[t] * (tt * __builtin_amdgcn_exp2f(arg * 1.4426950408889634f)) * rinv;
+                }
             }
-            const float arg = xi * d - dd + 0.5f * pair_erfc_c32(0) - alpha * alpha * r2;
-            const double fq = (double)(a.guest.qke[site] * (tt * __builtin_amdgcn_exp2f(arg * 1.4426950408889634f)) * rinv);
-            unsafeAtomicAdd(&accq[2 * n], w * fq);
-            unsafeAtomicAdd(&accq[2 * n + 1], wu * fq);
+            if (CHARGED) {
+                gq += w.x * (double)fq;
+                hq += w.y * (double)fq;
+            }
         }
-    };
-
-    uint32_t head = 0, tail = 0;
-    for (int base = 0; base < F.natoms; base += 64) {
-        const int n = base + lane;
-        const bool valid = n < F.natoms;
-        uint4 at = make_uint4(0u, 0u, 0u, 0u);
-        if (valid) at = atoms[n];
+    }
+    __syncthreads();
+    if (valid) {
+        const int orig = a.coreidx[F.atom_off + j];
+        double *accq = a.acc_q + 2 * a.qoff[s];
+        if (CHARGED) {
+            unsafeAtomicAdd(&accq[2 * orig], gq);
+            unsafeAtomicAdd(&accq[2 * orig + 1], hq);
+        }
 #pragma unroll
         for (int t = 0; t < NS; ++t) {
-            const float dx = (float)(int32_t)(sx[t] - at.x);
-            const float dy = (float)(int32_t)(sy[t] - at.y);
-            const float dz = (float)(int32_t)(sz[t] - at.z);
-            const float ex = A0 * dx + A3 * dy + A6 * dz;
-            const float ey = UPPER ? A4 * dy + A7 * dz : A1 * dx + A4 * dy + A7 * dz;
-            const float ez = UPPER ? A8 * dz : A2 * dx + A5 * dy + A8 * dz;
-            const float r2 = ex * ex + ey * ey + ez * ez;
-            const bool in = valid && r2 < a.rc_max2f;
-            const uint64_t mask = __ballot(in);
-            if (in) {
-                const uint32_t slot = (tail + (uint32_t)__popcll(mask & below)) & (GRAD_RING - 1);
-                ring_r2[slot] = r2;
-                ring_atom[slot] = n;
-                ring_site[slot] = t;
+            double *r = lj_red + 4 * (t * ntc + ht);
+            if (ge[t] != 0.0 || gs[t] != 0.0) {
+                unsafeAtomicAdd(r + 0, ge[t]);
+                unsafeAtomicAdd(r + 1, he[t]);
+                unsafeAtomicAdd(r + 2, gs[t]);
+                unsafeAtomicAdd(r + 3, hs[t]);
             }
-            tail += (uint32_t)__popcll(mask);
         }
-        __syncthreads();
-        while (tail - head >= 64u) {
-            drain((head + lane) & (GRAD_RING - 1), true);
-            head += 64u;
-        }
-        __syncthreads();
     }
-    drain((head + lane) & (GRAD_RING - 1), (uint32_t)lane < tail - head);
     __syncthreads();
-    // per-(site, type) LJ sums of this insertion, weighted into the system's accumulators
     double *acclj = a.acc_lj + (size_t)s * NS * ntc * 4;
-    for (int c = lane; c < NS * ntc; c += 64) {
-        const double de = lj_sum[2 * c], ds = lj_sum[2 * c + 1];
-        if (de != 0.0 || ds != 0.0) {
-            unsafeAtomicAdd(&acclj[4 * c + 0], w * de);
-            unsafeAtomicAdd(&acclj[4 * c + 1], wu * de);
-            unsafeAtomicAdd(&acclj[4 * c + 2], w * ds);
-            unsafeAtomicAdd(&acclj[4 * c + 3], wu * ds);
-        }
-    }
-    if (CHARGED) {
-        double *acck = a.acc_k + 4 * a.koff[s];
-        const int4 *kn = a.kn + F.k_off;
-        for (int kb = 0; kb < F.nk; kb += 64) {
-            const int k = kb + lane;
-            if (k < F.nk) {
-                const int4 nn = kn[k];
-                float re = 0.0f, im = 0.0f;
+    for (int i = tid; i < NS * ntc * 4; i += GRAD_WG)
+        if (lj_red[i] != 0.0) unsafeAtomicAdd(&acclj[i], lj_red[i]);
+}
+
+// Sums of w S_g(k) and w dU S_g(k) over a chunk of a system's listed insertions, one work-item
+// per coupled k-vector, the chunk's sites staged through LDS; exact wrapping fixed-point phases,
+// hardware sin/cos of revolutions.
+template <int NS>
+__global__ __launch_bounds__(256) void grad_kspace_kernel(GradArgs a)
+{
+    __shared__ uint32_t t_site[GRAD_TILE * NS * 3];
+    __shared__ double2 t_w[GRAD_TILE];
+    const int tid = threadIdx.x;
+    const int4 bk = a.kblk[blockIdx.x];   // system, first k-vector, list range
+    const int s = bk.x;
+    const FwDesc &F = a.fw[a.fw_of[s]];
+    const int k = bk.y + tid;
+    const bool valid = k < F.nk;
+    int4 nn = make_int4(0, 0, 0, 0);
+    if (valid) nn = a.kn[F.k_off + k];
+    float qs[NS];
 #pragma unroll
-                for (int t = 0; t < NS; ++t) {
-                    const uint32_t ph = (uint32_t)nn.x * sx[t] + (uint32_t)nn.y * sy[t] + (uint32_t)nn.z * sz[t];
-                    const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
-                    const float qs = (float)a.guest.q[t];
-                    re += qs * __builtin_amdgcn_cosf(rev);
-                    im += qs * __builtin_amdgcn_sinf(rev);
-                }
-                unsafeAtomicAdd(&acck[4 * k + 0], w * re);
-                unsafeAtomicAdd(&acck[4 * k + 1], w * im);
-                unsafeAtomicAdd(&acck[4 * k + 2], wu * re);
-                unsafeAtomicAdd(&acck[4 * k + 3], wu * im);
+    for (int t = 0; t < NS; ++t) qs[t] = (float)a.guest.q[t];
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    const int64_t l0 = a.loff[s] + bk.z, l1 = a.loff[s] + bk.w;
+    for (int64_t base = l0; base < l1; base += GRAD_TILE) {
+        const int nt = (int)((l1 - base) < GRAD_TILE ? (l1 - base) : GRAD_TILE);
+        __syncthreads();
+        for (int i = tid; i < nt * NS * 3; i += 256) t_site[i] = a.lsite[(size_t)base * NS * 3 + i];
+        for (int i = tid; i < nt; i += 256) t_w[i] = a.lw[base + i];
+        __syncthreads();
+        if (!valid) continue;
+        for (int i = 0; i < nt; ++i) {
+            float re = 0.0f, im = 0.0f;
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                const uint32_t *sf = t_site + (i * NS + t) * 3;
+                const uint32_t ph = (uint32_t)nn.x * sf[0] + (uint32_t)nn.y * sf[1] + (uint32_t)nn.z * sf[2];
+                const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
+                re += qs[t] * __builtin_amdgcn_cosf(rev);
+                im += qs[t] * __builtin_amdgcn_sinf(rev);
             }
+            const double2 w = t_w[i];
+            ar += w.x * (double)re;
+            ai += w.x * (double)im;
+            br += w.y * (double)re;
+            bi += w.y * (double)im;
         }
     }
+    if (!valid) return;
+    double *acck = a.acc_k + 4 * (a.koff[s] + k);
+    unsafeAtomicAdd(acck + 0, ar);
+    unsafeAtomicAdd(acck + 1, ai);
+    unsafeAtomicAdd(acck + 2, br);
+    unsafeAtomicAdd(acck + 3, bi);
 }
 
 // Flags the insertions whose weight relative to their system's best insertion exceeds e^-40.
@@ -246,6 +285,32 @@ hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const dou
                                          G, st);
 }
 
+// Each system's range of the (ascending) weighted-insertion list: loff[s] = first listed g >= seg[s].
+__global__ __launch_bounds__(256) void grad_loff_kernel(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys,
+                                                        int64_t *loff)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s > nsys) return;
+    if (s == nsys) {
+        loff[s] = nlist;
+        return;
+    }
+    const int64_t key = seg[s];
+    int64_t lo = 0, hi = nlist;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (list[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    loff[s] = lo;
+}
+
+hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st)
+{
+    hipLaunchKernelGGL(grad_loff_kernel, dim3((unsigned)((nsys + 1 + 255) / 256)), dim3(256), 0, st, list, nlist, seg,
+                       nsys, loff);
+    return hipGetLastError();
+}
+
 hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st)
 {
     if (natoms_total <= 0) return hipSuccess;
@@ -254,40 +319,39 @@ hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t nato
     return hipGetLastError();
 }
 
+
 template <int NS>
-static void launch_grad_ns(const GradArgs &a, bool upper, hipStream_t st)
+static void launch_grad_ns(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st)
 {
-    const dim3 grid((unsigned)a.nlist), block(64);
-    if (a.guest.charged) {
-        if (upper) hipLaunchKernelGGL((widom_grad_kernel<NS, true, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((widom_grad_kernel<NS, true, false>), grid, block, 0, st, a);
-    } else {
-        if (upper) hipLaunchKernelGGL((widom_grad_kernel<NS, false, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((widom_grad_kernel<NS, false, false>), grid, block, 0, st, a);
+    if (a.nlist > 0)
+        hipLaunchKernelGGL(grad_prep_kernel<NS>, dim3((unsigned)((a.nlist + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 grid((unsigned)nblk), block(GRAD_WG);
+    if (nblk > 0) {
+        if (a.guest.charged) {
+            if (upper) hipLaunchKernelGGL((widom_grad_gather_kernel<NS, true, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((widom_grad_gather_kernel<NS, true, false>), grid, block, 0, st, a);
+        } else {
+            if (upper) hipLaunchKernelGGL((widom_grad_gather_kernel<NS, false, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((widom_grad_gather_kernel<NS, false, false>), grid, block, 0, st, a);
+        }
     }
+    if (nkblk > 0) hipLaunchKernelGGL(grad_kspace_kernel<NS>, dim3((unsigned)nkblk), dim3(256), 0, st, a);
 }
 
-hipError_t launch_grad(const GradArgs &a, bool upper, hipStream_t st)
+// prep + gather + k-space sums; `nblk` gather workgroups described by a.blk, `nkblk` k-space
+// workgroups described by a.kblk.
+hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st)
 {
-    if (a.guest.nsites < 1 || a.guest.nsites > 6) return hipErrorInvalidValue;
-    // one block per weighted insertion, at most 2^24 blocks per launch
-    const int64_t CH = (int64_t)1 << 24;
-    for (int64_t off = 0; off < a.nlist; off += CH) {
-        GradArgs c = a;
-        c.list_g = a.list_g + off;
-        c.nlist = std::min(CH, a.nlist - off);
-        switch (a.guest.nsites) {
-        case 1: launch_grad_ns<1>(c, upper, st); break;
-        case 2: launch_grad_ns<2>(c, upper, st); break;
-        case 3: launch_grad_ns<3>(c, upper, st); break;
-        case 4: launch_grad_ns<4>(c, upper, st); break;
-        case 5: launch_grad_ns<5>(c, upper, st); break;
-        default: launch_grad_ns<6>(c, upper, st); break;
-        }
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+    switch (a.guest.nsites) {
+    case 1: launch_grad_ns<1>(a, upper, nblk, nkblk, st); break;
+    case 2: launch_grad_ns<2>(a, upper, nblk, nkblk, st); break;
+    case 3: launch_grad_ns<3>(a, upper, nblk, nkblk, st); break;
+    case 4: launch_grad_ns<4>(a, upper, nblk, nkblk, st); break;
+    case 5: launch_grad_ns<5>(a, upper, nblk, nkblk, st); break;
+    case 6: launch_grad_ns<6>(a, upper, nblk, nkblk, st); break;
+    default: return hipErrorInvalidValue;
     }
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 }  // namespace pa
