@@ -22,7 +22,7 @@
 
 namespace pa {
 
-constexpr int BIN_WG = 256;
+constexpr int BIN_WG = 512;
 constexpr int BIN_NW = BIN_WG / 64;
 constexpr int BRING = 128;  // per-wave ring; drained after every site sweep (< 64 + 64 pending)
 constexpr int BIN_MAX_TYPES = 64;
@@ -98,7 +98,12 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
                 st = st < 0 ? st + nc : st;
                 start[d] = cnt[d] == nc ? 0 : st;
             }
-            // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo
+            // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo.
+            // Atoms are sorted by cell with c0 fastest, so the stencil's cells of one (c1, c2) row
+            // are one contiguous atom range (two when the row wraps): one dependent cell_start
+            // lookup per row instead of one per cell.
+            const int a0 = start[0], a1 = start[0] + cnt[0];
+            const int e0 = a1 < nc0 ? a1 : nc0, w1 = a1 > nc0 ? a1 - nc0 : 0;
             for (int i2 = 0; i2 < cnt[2] && !hit; ++i2) {
                 int c2 = start[2] + i2;
                 c2 = c2 >= nc2 ? c2 - nc2 : c2;
@@ -106,20 +111,18 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
                     int c1 = start[1] + i1;
                     c1 = c1 >= nc1 ? c1 - nc1 : c1;
                     const int row = nc0 * (c1 + nc1 * c2);
-                    for (int i0 = 0; i0 < cnt[0] && !hit; ++i0) {
-                        int c0 = start[0] + i0;
-                        c0 = c0 >= nc0 ? c0 - nc0 : c0;
-                        const int c = c0 + row;
-                        for (int jj = cs[c]; jj < cs[c + 1]; ++jj) {
-                            const uint4 at = cat[jj];
-                            const float dx = (float)(int32_t)(sp[t][0] - at.x);
-                            const float dy = (float)(int32_t)(sp[t][1] - at.y);
-                            const float dz = (float)(int32_t)(sp[t][2] - at.z);
-                            const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
-                            const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
-                            const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
-                            if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
-                        }
+                    const int j0 = cs[row + a0], j1 = cs[row + e0];
+                    const int k0 = cs[row], k1 = cs[row + w1];
+                    const int n1 = j1 - j0, ntot = n1 + (k1 - k0);
+                    for (int q = 0; q < ntot; ++q) {
+                        const uint4 at = cat[q < n1 ? j0 + q : k0 + (q - n1)];
+                        const float dx = (float)(int32_t)(sp[t][0] - at.x);
+                        const float dy = (float)(int32_t)(sp[t][1] - at.y);
+                        const float dz = (float)(int32_t)(sp[t][2] - at.z);
+                        const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
+                        const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
+                        const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
+                        if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
                     }
                 }
             }
@@ -153,10 +156,12 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
 {
     const int site = info >> 16, t = info & 0xffff;
     r2 = fmaxf(r2, 1.0e-30f);
+    // charged guests take 1/r from one reciprocal square root and 1/r^2 as its square
+    const float rinv = CHARGED ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     float e = 0.0f;
     if (r2 < rc_lj2) {
         const float2 p = pair[site * ntc + t];
-        const float x = p.x * __builtin_amdgcn_rcpf(r2);
+        const float x = p.x * (CHARGED ? rinv * rinv : __builtin_amdgcn_rcpf(r2));
         const float x3 = x * x * x;
         e = p.y * x3 * (x3 - 1.0f);
     }
@@ -165,7 +170,6 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
         constexpr float XLO = (float)PAIR_ERFC_XLO;
         constexpr float XA = 4.0f / (1.0f - XLO);
         constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
-        const float rinv = __builtin_amdgcn_rsqf(r2);
         const float z = alpha * (r2 * rinv);
         const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
         const float xi = XA * tt + XB;
