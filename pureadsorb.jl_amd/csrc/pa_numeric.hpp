@@ -265,4 +265,42 @@ __host__ __device__ constexpr float pair_erfc_c32(int j)
                   : -8.397902e-7f;
 }
 
+// The same Float32 series as an ordinary polynomial in xi, coefficient k of
+// c0/2 + sum_{j=1..7} c_j T_j(xi) (T_j the Chebyshev polynomials, expanded by
+// T_{j+1} = 2 xi T_j - T_{j-1} in FP64 at compile time). Evaluated by Horner in FP32 it takes
+// 7 FMAs instead of Clenshaw's 14 operations; over xi in [-1, 1] both forms stay within 1e-7 of
+// the FP64 value of the series.
+__host__ __device__ constexpr double pair_erfc_mono32_d(int k)
+{
+    double T[8][8] = {};
+    T[0][0] = 1.0;
+    T[1][1] = 1.0;
+    for (int j = 2; j < 8; ++j)
+        for (int i = 0; i < 8; ++i) T[j][i] = (i > 0 ? 2.0 * T[j - 1][i - 1] : 0.0) - T[j - 2][i];
+    double v = 0.5 * (double)pair_erfc_c32(0) * T[0][k];
+    for (int j = 1; j < 8; ++j) v += (double)pair_erfc_c32(j) * T[j][k];
+    return v;
+}
+
+__host__ __device__ constexpr float pair_erfc_mono32(int k) { return (float)pair_erfc_mono32_d(k); }
+
+// log(erfc(z)/t) + z^2 of the Float32 series at xi (Horner form of pair_erfc_c32).
+__host__ __device__ inline float pair_erfc_poly32(float xi)
+{
+    float p = pair_erfc_mono32(7);
+#pragma unroll
+    for (int k = 6; k >= 0; --k) p = p * xi + pair_erfc_mono32(k);
+    return p;
+}
+
+// The same polynomial times log2(e), for an exp2 of the result.
+__host__ __device__ inline float pair_erfc_poly32_log2e(float xi)
+{
+    constexpr double L2E = 1.4426950408889634;
+    float p = (float)(pair_erfc_mono32_d(7) * L2E);
+#pragma unroll
+    for (int k = 6; k >= 0; --k) p = p * xi + (float)(pair_erfc_mono32_d(k) * L2E);
+    return p;
+}
+
 }  // namespace pa

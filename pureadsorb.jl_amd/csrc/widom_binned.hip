@@ -166,23 +166,16 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
         e = p.y * x3 * (x3 - 1.0f);
     }
     if (CHARGED && r2 < rc_ew2) {
-        // pair_erfc_dev in Float32 (src/ewald.jl:242-254): the reference's 8-term series
+        // pair_erfc_dev in Float32 (src/ewald.jl:242-254): the reference's 8-term series, as
+        // the same polynomial in Horner form
         constexpr float XLO = (float)PAIR_ERFC_XLO;
         constexpr float XA = 4.0f / (1.0f - XLO);
         constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
-        const float z = alpha * (r2 * rinv);
-        const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
+        // t = 2/(2 + alpha r) = 1/(1 + (alpha/2) r); exp2 of the log2(e)-scaled exponent
+        const float tt = __builtin_amdgcn_rcpf(1.0f + (0.5f * alpha) * (r2 * rinv));
         const float xi = XA * tt + XB;
-        const float xi2 = 2.0f * xi;
-        float d = 0.0f, dd = 0.0f;
-#pragma unroll
-        for (int jj = 7; jj >= 1; --jj) {
-            const float tmp = d;
-            d = xi2 * d - dd + pair_erfc_c32(jj);
-            dd = tmp;
-        }
-        const float arg = xi * d - dd + 0.5f * pair_erfc_c32(0) - alpha2 * r2;
-        e += qke[site] * qj * (tt * __builtin_amdgcn_exp2f(arg * 1.4426950408889634f)) * rinv;
+        const float arg2 = pair_erfc_poly32_log2e(xi) - (alpha2 * 1.4426950408889634f) * r2;
+        e += qke[site] * qj * (tt * __builtin_amdgcn_exp2f(arg2)) * rinv;
     }
     return e;
 }

@@ -127,15 +127,8 @@ __global__ __launch_bounds__(GRAD_WG) void widom_grad_gather_kernel(GradArgs a)
                     const float rinv = __builtin_amdgcn_rsqf(r2);
                     const float z = alpha * (r2 * rinv);
                     const float tt = 2.0f * __builtin_amdgcn_rcpf(2.0f + z);
-                    const float xi = XA * tt + XB, xi2 = 2.0f * xi;
-                    float d = 0.0f, dd = 0.0f;
-#pragma unroll
-                    for (int jj = 7; jj >= 1; --jj) {
-                        const float tmp = d;
-                        d = xi2 * d - dd + pair_erfc_c32(jj);
-                        dd = tmp;
-                    }
-                    const float arg = xi * d - dd + 0.5f * pair_erfc_c32(0) - alpha * alpha * r2;
+                    const float xi = XA * tt + XB;
+                    const float arg = pair_erfc_poly32(xi) - alpha * alpha * r2;
                     fq += a.guest.qke[t] * (tt * __builtin_amdgcn_exp2f(arg * 1.4426950408889634f)) * rinv;
                 }
             }
