@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -33,11 +34,14 @@ double erfc_dev_host(double x);
 struct DevAlloc {
     void *p = nullptr;
     size_t bytes = 0;
+    // host copy of what upload_cached last wrote (buffers that only uploads write)
+    std::vector<unsigned char> shadow;
     ~DevAlloc() { if (p) (void)hipFree(p); }
     hipError_t ensure(size_t n)
     {
         if (n <= bytes && p) return hipSuccess;
         if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        shadow.clear();
         hipError_t e = hipMalloc(&p, n ? n : 16);
         if (e == hipSuccess) bytes = n;
         return e;
@@ -70,6 +74,22 @@ inline hipError_t upload(DevAlloc &d, const std::vector<T> &v)
     return hipMemcpy(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
 }
 
+// upload() that skips the copy when the buffer already holds exactly these bytes: repeated
+// calls with the same arguments re-send nothing.
+template <class T>
+inline hipError_t upload_cached(DevAlloc &d, const std::vector<T> &v)
+{
+    const size_t n = sizeof(T) * v.size();
+    if (d.p && d.shadow.size() == n && (n == 0 || std::memcmp(d.shadow.data(), v.data(), n) == 0)) return hipSuccess;
+    hipError_t e = upload(d, v);
+    if (e != hipSuccess) {
+        d.shadow.clear();
+        return e;
+    }
+    d.shadow.assign((const unsigned char *)v.data(), (const unsigned char *)v.data() + n);
+    return hipSuccess;
+}
+
 }  // namespace pa
 
 // The opaque handle of the C ABI.
@@ -87,6 +107,10 @@ struct pa_batch {
     int ntypes_c = 0;
     double rc_lj = 0, rc_ew = 0;
     double r_guest = 0;  // largest site distance from the guest's reference point
+    // phase-0 radii of the last temperature (they depend on kT only): nsys x NS x ntypes_c Å^2
+    // and the largest per system
+    double rr_kT = -1.0;
+    std::vector<double> rr_r2, rr_max2;
     // force field as given, compact type map and per-framework host tables (gradients)
     int nt_ff = 0, ff_tail = 0;
     std::vector<double> ff_sigma, ff_eps;

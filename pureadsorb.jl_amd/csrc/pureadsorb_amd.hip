@@ -940,7 +940,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
                         std::vector<int> *keys_out = nullptr)
 {
     const int nsys = b->nsys;
-    HIP_TRY(upload(b->d_seg, seg));
+    HIP_TRY(upload_cached(b->d_seg, seg));
     const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
     const int64_t nchunks = (ge - gb + CH - 1) / CH;
     while ((int64_t)b->kev.size() < 2 * nchunks) {
@@ -1006,12 +1006,22 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     std::vector<float> rho2((size_t)nsys * b->nsites * b->ntypes_c, 0.0f);
     if (kT > 0.0) {
         const int NS = b->nsites, NTC = b->ntypes_c;
-        std::vector<double> r2s((size_t)NS * NTC);
+        if (b->rr_kT != kT) {
+            // the radii depend on kT only: solved once per temperature
+            b->rr_kT = -1.0;
+            b->rr_r2.assign((size_t)nsys * NS * NTC, 0.0);
+            b->rr_max2.assign(nsys, 0.0);
+            for (int s = 0; s < nsys; ++s) {
+                int st = rejection_radii(b, s, kT, (double)FLT_EPSILON, b->rr_r2.data() + (size_t)s * NS * NTC,
+                                         &b->rr_max2[s]);
+                if (st) return st;
+            }
+            b->rr_kT = kT;
+        }
         for (int s = 0; s < nsys; ++s) {
-            double rmax2 = 0.0;
-            int st = rejection_radii(b, s, kT, (double)FLT_EPSILON, r2s.data(), &rmax2);
-            if (st) return st;
-            for (size_t i = 0; i < r2s.size(); ++i) {
+            const double *r2s = b->rr_r2.data() + (size_t)s * NS * NTC;
+            const double rmax2 = b->rr_max2[s];
+            for (size_t i = 0; i < (size_t)NS * NTC; ++i) {
                 // FP32 radius rounded down with a 1e-5 relative allowance for the device's FP32
                 // distance
                 const double r2 = r2s[i];
@@ -1028,9 +1038,9 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     }
     const int nbins = base[nsys];
     const int listcap = (int)std::min<double>(LISTCAP_MAX, std::ceil(cap_est / 64.0) * 64.0);
-    HIP_TRY(upload(b->d_bins, bins));
-    HIP_TRY(upload(b->d_bin_base, base));
-    HIP_TRY(upload(b->d_rho2, rho2));
+    HIP_TRY(upload_cached(b->d_bins, bins));
+    HIP_TRY(upload_cached(b->d_bin_base, base));
+    HIP_TRY(upload_cached(b->d_rho2, rho2));
     HIP_TRY(b->d_bin_count.ensure(sizeof(int) * (nbins + 1)));
     HIP_TRY(b->d_bin_start.ensure(sizeof(int) * (nbins + 1)));
     HIP_TRY(b->d_cursor.ensure(sizeof(int) * nbins));
@@ -1076,8 +1086,6 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     for (int64_t g0 = gb; g0 < ge; g0 += CH) {
         A.g0 = g0;
         A.g1 = std::min(ge, g0 + CH);
-        HIP_TRY(hipMemsetAsync(A.bin_count, 0, sizeof(int) * (nbins + 1), b->stream));
-        HIP_TRY(hipMemsetAsync(A.cursor, 0, sizeof(int) * nbins, b->stream));
         HIP_TRY(launch_binning(A, b->d_scan_tmp.p, scan_bytes, b->stream));
         if (keys_out) {
             keys_out->resize(ge - gb);
@@ -1160,8 +1168,8 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
         }
     }
     const int ntiles = (int)tile_g0.size();
-    HIP_TRY(upload(b->d_tile_g0, tile_g0));
-    HIP_TRY(upload(b->d_tile_n, tile_n));
+    HIP_TRY(upload_cached(b->d_tile_g0, tile_g0));
+    HIP_TRY(upload_cached(b->d_tile_n, tile_n));
     HIP_TRY(b->d_tiles.ensure(sizeof(double4) * ntiles));
 
     const double kT_phase0 = (precision & PA_SINGLE_PHASE) ? 0.0 : kT;

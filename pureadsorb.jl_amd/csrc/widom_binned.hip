@@ -141,6 +141,14 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
     atomicAdd(&a.bin_count[key], 1);
 }
 
+// Clears the bin counts (nbins + 1) and scatter cursors (nbins) of one chunk in one launch.
+__global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i <= a.nbins) a.bin_count[i] = 0;
+    if (i < a.nbins) a.cursor[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -443,13 +451,15 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes)
     return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (int *)nullptr, (int *)nullptr, nbins + 1);
 }
 
-// Steps 1-3 of the pipeline (pose + bin counts, scan, scatter). The caller has zeroed
-// bin_count[0 .. nbins] and cursor[0 .. nbins-1].
+// Steps 1-3 of the pipeline (pose + bin counts, scan, scatter), after clearing bin_count and
+// cursor.
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st)
 {
     const int64_t n = a.g1 - a.g0;
     if (n <= 0) return hipSuccess;
-    hipError_t e;
+    hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((a.nbins + 1 + 255) / 256)), dim3(256), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     switch (a.guest.nsites) {
     case 1: e = launch_binned_ns<1>(a, true, st); break;
     case 2: e = launch_binned_ns<2>(a, true, st); break;
