@@ -104,6 +104,8 @@ struct BinnedArgs {
     const uint4 *corefx;       // fixed-point x, y, z and compact type, by cell
     const int *cell_start;     // per framework: cg_nc[0]*cg_nc[1]*cg_nc[2] + 1 offsets
     const float *rho2;         // nsys x NS x ntypes_c rejection radii^2 (Å^2), rounded down
+    int *cand;                 // chunk-relative insertions left for the full phase-0 stencil
+    int *ncand;                // their count
 };
 
 // Gradient launch: one weighted insertion per wavefront (widom_grad.hip).

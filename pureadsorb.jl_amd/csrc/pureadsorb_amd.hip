@@ -1051,6 +1051,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     HIP_TRY(b->d_sitefx.ensure(sizeof(uint32_t) * 3 * b->nsites * chunk_n));
     HIP_TRY(b->d_key.ensure(sizeof(int) * chunk_n));
     HIP_TRY(b->d_order.ensure(sizeof(int) * chunk_n));
+    HIP_TRY(b->d_cand.ensure(sizeof(int) * chunk_n));
+    HIP_TRY(b->d_ncand.ensure(sizeof(int)));
 
     BinnedArgs A{};
     A.fw = b->d_fw.as<FwDesc>();
@@ -1083,6 +1085,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.corefx = b->d_corefx.as<uint4>();
     A.cell_start = b->d_cell_start.as<int>();
     A.rho2 = b->d_rho2.as<float>();
+    A.cand = b->d_cand.as<int>();
+    A.ncand = b->d_ncand.as<int>();
     for (int64_t g0 = gb; g0 < ge; g0 += CH) {
         A.g0 = g0;
         A.g1 = std::min(ge, g0 + CH);

@@ -54,6 +54,65 @@ __device__ inline void wave_lds_fence()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Phase-0 test of one insertion (src/widom.jl:99-151): true when some site lies closer than
+// rho_at to an atom of type t (the radii guarantee exp(-dU/kT) == 0.0 in FP64). The atoms come
+// from the hard-core grid cells within (h0, h1, h2) cells of each site's home cell (h_d < nc_d),
+// with the exact fixed-point minimum image. Atoms are sorted by cell with c0 fastest, so the
+// stencil's cells of one (c1, c2) row are one contiguous atom range (two when the row wraps):
+// one dependent cell_start lookup per row.
+template <int NS>
+__device__ inline bool stencil_hit(const BinnedArgs &a, const FwDesc &F, int s, const uint32_t (*sp)[3], int h0, int h1,
+                                   int h2)
+{
+    const int *cs = a.cell_start + F.cg_off;
+    const uint4 *cat = a.corefx + F.atom_off;
+    const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
+    const int nc[3] = {F.cg_nc[0], F.cg_nc[1], F.cg_nc[2]};
+    const int hr[3] = {h0, h1, h2};
+    int cnt[3], start[3];
+    bool hit = false;
+    for (int t = 0; t < NS && !hit; ++t) {
+        for (int d = 0; d < 3; ++d) {
+            const int home = (int)(((uint64_t)sp[t][d] * (uint64_t)nc[d]) >> 32);
+            cnt[d] = min(2 * hr[d] + 1, nc[d]);
+            // start in [0, nc) without a division: home - h lies in (-nc, nc) because h < nc
+            int st = home - hr[d];
+            st = st < 0 ? st + nc[d] : st;
+            start[d] = cnt[d] == nc[d] ? 0 : st;
+        }
+        // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo
+        const int a0 = start[0], a1 = start[0] + cnt[0];
+        const int e0 = a1 < nc[0] ? a1 : nc[0], w1 = a1 > nc[0] ? a1 - nc[0] : 0;
+        for (int i2 = 0; i2 < cnt[2] && !hit; ++i2) {
+            int c2 = start[2] + i2;
+            c2 = c2 >= nc[2] ? c2 - nc[2] : c2;
+            for (int i1 = 0; i1 < cnt[1] && !hit; ++i1) {
+                int c1 = start[1] + i1;
+                c1 = c1 >= nc[1] ? c1 - nc[1] : c1;
+                const int row = nc[0] * (c1 + nc[1] * c2);
+                const int j0 = cs[row + a0], j1 = cs[row + e0];
+                const int k0 = cs[row], k1 = cs[row + w1];
+                const int n1 = j1 - j0, ntot = n1 + (k1 - k0);
+                for (int q = 0; q < ntot; ++q) {
+                    const uint4 at = cat[q < n1 ? j0 + q : k0 + (q - n1)];
+                    const float dx = (float)(int32_t)(sp[t][0] - at.x);
+                    const float dy = (float)(int32_t)(sp[t][1] - at.y);
+                    const float dz = (float)(int32_t)(sp[t][2] - at.z);
+                    const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
+                    const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
+                    const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
+                    if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
+                }
+            }
+        }
+    }
+    return hit;
+}
+
+// Pose, bin and the near part of phase 0 of every insertion of the chunk. The near stencil
+// (the home cell and its neighbours) rejects most insertions that phase 0 rejects; an insertion
+// whose full stencil reaches further and that the near stencil did not reject is appended to
+// the candidate list for bin_reject_kernel, which finishes phase 0 on full wavefronts.
 template <int NS>
 __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
 {
@@ -78,67 +137,59 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
         }
     }
     const BinSys B = a.bins[s];
-    if (B.reach[0] >= 0) {
-        // Phase 0 (src/widom.jl:99-151): the insertion is rejected when some site lies closer
-        // than rho_at to an atom of type t; the radii guarantee exp(-dU/kT) == 0.0 in FP64.
-        // Atoms come from the grid cells within reach of the site's home cell.
-        const int *cs = a.cell_start + F.cg_off;
-        const uint4 *cat = a.corefx + F.atom_off;
-        const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
-        const int nc0 = F.cg_nc[0], nc1 = F.cg_nc[1], nc2 = F.cg_nc[2];
-        int cnt[3], start[3];
-        bool hit = false;
-        for (int t = 0; t < NS && !hit; ++t) {
-            for (int d = 0; d < 3; ++d) {
-                const int nc = F.cg_nc[d];
-                const int h = (int)(((uint64_t)sp[t][d] * (uint64_t)nc) >> 32);
-                cnt[d] = min(2 * B.reach[d] + 1, nc);
-                // start in [0, nc) without a division: h - reach lies in (-nc, nc) because reach < nc
-                int st = h - B.reach[d];
-                st = st < 0 ? st + nc : st;
-                start[d] = cnt[d] == nc ? 0 : st;
-            }
-            // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo.
-            // Atoms are sorted by cell with c0 fastest, so the stencil's cells of one (c1, c2) row
-            // are one contiguous atom range (two when the row wraps): one dependent cell_start
-            // lookup per row instead of one per cell.
-            const int a0 = start[0], a1 = start[0] + cnt[0];
-            const int e0 = a1 < nc0 ? a1 : nc0, w1 = a1 > nc0 ? a1 - nc0 : 0;
-            for (int i2 = 0; i2 < cnt[2] && !hit; ++i2) {
-                int c2 = start[2] + i2;
-                c2 = c2 >= nc2 ? c2 - nc2 : c2;
-                for (int i1 = 0; i1 < cnt[1] && !hit; ++i1) {
-                    int c1 = start[1] + i1;
-                    c1 = c1 >= nc1 ? c1 - nc1 : c1;
-                    const int row = nc0 * (c1 + nc1 * c2);
-                    const int j0 = cs[row + a0], j1 = cs[row + e0];
-                    const int k0 = cs[row], k1 = cs[row + w1];
-                    const int n1 = j1 - j0, ntot = n1 + (k1 - k0);
-                    for (int q = 0; q < ntot; ++q) {
-                        const uint4 at = cat[q < n1 ? j0 + q : k0 + (q - n1)];
-                        const float dx = (float)(int32_t)(sp[t][0] - at.x);
-                        const float dy = (float)(int32_t)(sp[t][1] - at.y);
-                        const float dz = (float)(int32_t)(sp[t][2] - at.z);
-                        const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
-                        const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
-                        const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
-                        if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
-                    }
-                }
-            }
-        }
-        if (hit) {
-            a.key[g - a.g0] = -1;
-            a.dU[g - a.out_base] = INFINITY;
-            return;
-        }
-    }
     const int bx = min(B.nb[0] - 1, (int)(f[0] * B.nb[0]));
     const int by = min(B.nb[1] - 1, (int)(f[1] * B.nb[1]));
     const int bz = min(B.nb[2] - 1, (int)(f[2] * B.nb[2]));
     const int key = B.bin_base + bx + B.nb[0] * (by + B.nb[1] * bz);
+    bool pending = false;
+    if (B.reach[0] >= 0) {
+        const int n0 = min(B.reach[0], 1), n1 = min(B.reach[1], 1), n2 = min(B.reach[2], 1);
+        if (stencil_hit<NS>(a, F, s, sp, n0, n1, n2)) {
+            a.key[g - a.g0] = -1;
+            a.dU[g - a.out_base] = INFINITY;
+            return;
+        }
+        pending = B.reach[0] > 1 || B.reach[1] > 1 || B.reach[2] > 1;
+    }
     a.key[g - a.g0] = key;
-    atomicAdd(&a.bin_count[key], 1);
+    if (!pending) {
+        atomicAdd(&a.bin_count[key], 1);
+        return;
+    }
+    // wavefront-aggregated append to the candidate list
+    const uint64_t m = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(a.ncand, __popcll(m));
+    base = __shfl(base, leader, 64);
+    a.cand[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)(g - a.g0);
+}
+
+// Phase 0 on the full stencil for the candidates the near stencil left (grid-stride over the
+// device-side candidate count): rejected -> key -1 and dU = +inf, else counted in its bin.
+template <int NS>
+__global__ __launch_bounds__(256) void bin_reject_kernel(BinnedArgs a)
+{
+    const int n = *a.ncand;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int rel = a.cand[i];
+        const int64_t g = a.g0 + rel;
+        const int s = find_sys64(a.seg, a.nsys, g);
+        const FwDesc &F = a.fw[a.fw_of[s]];
+        const BinSys B = a.bins[s];
+        const uint32_t *sf = a.sitefx + (size_t)rel * 3 * NS;
+        uint32_t sp[NS][3];
+#pragma unroll
+        for (int t = 0; t < NS; ++t)
+            for (int d = 0; d < 3; ++d) sp[t][d] = sf[3 * t + d];
+        if (stencil_hit<NS>(a, F, s, sp, B.reach[0], B.reach[1], B.reach[2])) {
+            a.key[rel] = -1;
+            a.dU[g - a.out_base] = INFINITY;
+        } else {
+            atomicAdd(&a.bin_count[a.key[rel]], 1);
+        }
+    }
 }
 
 // Clears the bin counts (nbins + 1) and scatter cursors (nbins) of one chunk in one launch.
@@ -147,6 +198,7 @@ __global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i <= a.nbins) a.bin_count[i] = 0;
     if (i < a.nbins) a.cursor[i] = 0;
+    if (i == 0) *a.ncand = 0;
 }
 
 __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
@@ -424,6 +476,8 @@ static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t 
     const int64_t n = a.g1 - a.g0;
     const dim3 g256((unsigned)((n + 255) / 256)), b256(256);
     hipLaunchKernelGGL((bin_pose_kernel<NS>), g256, b256, 0, st, a);
+    const dim3 gr((unsigned)std::min<int64_t>((n + 255) / 256, 2048));
+    hipLaunchKernelGGL((bin_reject_kernel<NS>), gr, b256, 0, st, a);
     return hipGetLastError();
 }
 
