@@ -272,6 +272,10 @@ double lj_pair(double r2, double sg, double ep)
     return 4.0 * ep * (x * x - x);
 }
 
+// Edge (Å) of the hard-core grid cells of phase 0. Typical rejection radii are 1-2.3 Å, so the
+// near stencil (the home cell and its neighbours) usually decides an insertion.
+constexpr double HC_CELL = 1.2;
+
 // pair_energy (src/reject.jl:22): LJ plus K erfc(alpha r)/r with the pair series.
 double pair_energy_h(double r, double sg, double ep, double K, double alpha)
 {
@@ -527,11 +531,11 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             std::memcpy(&qb, &qf, 4);
             atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
         }
-        // hard-core cell grid: cells of at least 1.6 Å, at most 64 per axis; atoms sorted by cell
+        // hard-core cell grid: cells of at least HC_CELL Å, at most 64 per axis; atoms sorted by cell
         {
             int nct = 1;
             for (int d = 0; d < 3; ++d) {
-                D.cg_nc[d] = std::max(1, std::min(64, (int)std::floor(L[d] / 1.6)));
+                D.cg_nc[d] = std::max(1, std::min(64, (int)std::floor(L[d] / HC_CELL)));
                 nct *= D.cg_nc[d];
             }
             D.cg_off = (int)cell_start.size();
