@@ -288,9 +288,16 @@ def bench_config2(args):
         "speedup_mixed_vs_f64_plain": ins_s / (ninsert / w64),
         "energy_kernel_ms_per_step": ems / args.steps,
         "K_H_mixed": [r.K_H for r in rm], "K_H_f64": [r.K_H for r in r64],
-        "ln_K_H_f64": [r.ln_K_H for r in r64],
+        "ln_K_H_mixed": [r.ln_K_H for r in rm], "ln_K_H_f64": [r.ln_K_H for r in r64],
         "max_abs_d_ln_K_H": float(dln.max()), "max_rel_d_ln_K_H": float(rel.max()),
     }
+    # systems whose ln K_H shows binding insertions (above -50) against the |d ln K_H| <= 1e-4
+    # gate; the others relative to |ln K_H|
+    lnf = np.array([c.ln_K_H for c in r64])
+    bind = lnf > -50.0
+    out["binding_systems"] = int(bind.sum())
+    out["binding_max_abs_d_ln_K_H"] = float(dln[bind].max()) if bind.any() else None
+    out["nonbinding_max_rel_d_ln_K_H"] = float(rel[~bind].max()) if (~bind).any() else None
     log(f"config2 mixed: {ins_s:.4e} ins/s; f64 {ninsert / w64:.4e}; max |d ln K_H| {dln.max():.2e}")
     print(json.dumps(out), flush=True)
 

@@ -216,12 +216,10 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
 {
     const int site = info >> 16, t = info & 0xffff;
     r2 = fmaxf(r2, 1.0e-30f);
-    // charged guests take 1/r from one reciprocal square root and 1/r^2 as its square
-    const float rinv = CHARGED ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     float e = 0.0f;
     if (r2 < rc_lj2) {
         const float2 p = pair[site * ntc + t];
-        const float x = p.x * (CHARGED ? rinv * rinv : __builtin_amdgcn_rcpf(r2));
+        const float x = p.x * __builtin_amdgcn_rcpf(r2);
         const float x3 = x * x * x;
         e = p.y * x3 * (x3 - 1.0f);
     }
@@ -232,6 +230,7 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
         constexpr float XA = 4.0f / (1.0f - XLO);
         constexpr float XB = -2.0f * (1.0f + XLO) / (1.0f - XLO) - 1.0f;
         // t = 2/(2 + alpha r) = 1/(1 + (alpha/2) r); exp2 of the log2(e)-scaled exponent
+        const float rinv = __builtin_amdgcn_rsqf(r2);
         const float tt = __builtin_amdgcn_rcpf(1.0f + (0.5f * alpha) * (r2 * rinv));
         const float xi = XA * tt + XB;
         const float arg2 = pair_erfc_poly32_log2e(xi) - (alpha2 * 1.4426950408889634f) * r2;
