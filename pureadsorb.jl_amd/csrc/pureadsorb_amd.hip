@@ -26,7 +26,6 @@ hipError_t launch_poses(const FwDesc *fw, const int *fw_of, int sys, uint64_t se
 hipError_t binned_scan_bytes(int nbins, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
-size_t binned_lds_bytes_host(int cap, int ns);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
 hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,

@@ -495,8 +495,6 @@ static hipError_t launch_binned_energy_ns(const BinnedArgs &a, bool upper, hipSt
     return hipGetLastError();
 }
 
-size_t binned_lds_bytes_host(int cap, int ns) { return binned_lds_bytes(cap, ns); }
-
 // Scan temporary storage size for `nbins + 1` counts.
 hipError_t binned_scan_bytes(int nbins, size_t *bytes)
 {
