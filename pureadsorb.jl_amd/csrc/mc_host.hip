@@ -154,17 +154,27 @@ void rebuild_sk(const pa_batch *b, int u, const std::vector<double4> &pos, const
     out.assign(F.nk, make_double2(0.0, 0.0));
     for (int k = 0; k < F.nk; ++k) out[k] = b->S64_h[F.k_off + k];
     if (!b->gdev.charged) return;
+    // the guests' terms exactly as the move kernel forms them: per-axis powers, then products
     double sites[MAX_SITES][3];
+    const int m0 = F.kn_max[0] + 1, m1 = F.kn_max[1] + 1, tw = m0 + m1 + F.kn_max[2] + 1;
+    std::vector<double2> tab((size_t)MAX_SITES * tw);
     for (int i = 0; i < occ; ++i) {
         const double p[3] = {pos[g0 + i].x, pos[g0 + i].y, pos[g0 + i].z};
         const double q[4] = {quat[g0 + i].x, quat[g0 + i].y, quat[g0 + i].z, quat[g0 + i].w};
         guest_sites_h(b, p, q, sites);
+        for (int s = 0; s < b->nsites; ++s)
+            for (int x = 0; x < tw; ++x) {
+                const int d = x < m0 ? 0 : (x < m0 + m1 ? 1 : 2);
+                tab[(size_t)s * tw + x] = axis_power(F.invA, sites[s], d, x - (d == 0 ? 0 : (d == 1 ? m0 : m0 + m1)));
+            }
         for (int k = 0; k < F.nk; ++k) {
-            const double4 kv = b->k64_h[F.k_off + k];
+            const int4 n = b->kn_h[F.k_off + k];
             for (int s = 0; s < b->nsites; ++s) {
-                const double ph = kv.x * sites[s][0] + kv.y * sites[s][1] + kv.z * sites[s][2];
-                out[k].x += b->gq[s] * std::cos(ph);
-                out[k].y += b->gq[s] * std::sin(ph);
+                const double2 *tb = tab.data() + (size_t)s * tw;
+                const double2 c = cis_from_axes(tb[std::abs(n.x)], tb[m0 + std::abs(n.y)], tb[m0 + m1 + std::abs(n.z)],
+                                                n.x, n.y, n.z);
+                out[k].x += b->gq[s] * c.x;
+                out[k].y += b->gq[s] * c.y;
             }
         }
     }
@@ -224,6 +234,9 @@ McArgs base_mc(pa_state *st)
     a.atom64 = b->d_atom64.as<double4>();
     a.atype = b->d_atype.as<int>();
     a.k64 = b->d_k64.as<double4>();
+    a.kn = b->d_kn.as<int4>();
+    a.ktab_w = 0;
+    for (const FwDesc &D : b->desc) a.ktab_w = std::max(a.ktab_w, D.kn_max[0] + D.kn_max[1] + D.kn_max[2] + 3);
     a.pair64 = b->d_pair64.as<double2>();
     a.ntypes_c = b->ntypes_c;
     a.rc_lj2 = b->rc_lj * b->rc_lj;

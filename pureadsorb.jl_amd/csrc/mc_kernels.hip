@@ -54,6 +54,8 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
 {
 #pragma clang fp contract(off)
     __shared__ double red[3][MC_NW];
+    // per-axis phase tables of the attempt's old and new sites: e^{2 pi i m f_d}, m = 0..kn_max[d]
+    extern __shared__ double2 ktab[];
     const int n = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const FwDesc &F = a.fw[a.fw_of[n]];
     const int gr0 = a.guest_off[n], cap = a.guest_off[n + 1] - gr0, nslots = a.guest_off[a.nsys];
@@ -166,33 +168,44 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             else if (type == MOVE_DELETION)
                 gsum += site_pair_energy(mi_r2(F, os[sa], pb), p.x, p.y, qq, alpha, a.rc_lj2, a.rc_ew2);
         }
-        // (3) reciprocal change against the running structure factor, dS kept per lane
+        // (3) reciprocal change against the running structure factor, dS kept per lane. The
+        // guest term e^{i k.r} of k = 2 pi A^-T n at r = A f is the product over axes of
+        // e^{2 pi i n_d f_d}: per attempt every site's per-axis powers are tabulated in LDS
+        // (own sincospi of 2 m f_d), then each k-vector takes two complex products per site.
         double rsum = 0.0;
-        if (G.charged)
-            #pragma unroll 1
+        if (G.charged) {
+            const int m0 = F.kn_max[0] + 1, m1 = F.kn_max[1] + 1, tw = m0 + m1 + F.kn_max[2] + 1;
+            for (int e = tid; e < 2 * NS * tw; e += MC_WG) {
+                const int pose = e / (NS * tw), rem = e - pose * NS * tw, t = rem / tw, x = rem - t * tw;
+                const int d = x < m0 ? 0 : (x < m0 + m1 ? 1 : 2);
+                const int m = x - (d == 0 ? 0 : (d == 1 ? m0 : m0 + m1));
+                ktab[e] = axis_power(F.invA, pose ? ns[t] : os[t], d, m);
+            }
+            __syncthreads();
+            const int4 *kn = a.kn + F.k_off;
+#pragma unroll 1
             for (int k = tid; k < nk; k += MC_WG) {
                 const double4 kv = ks[k];
+                const int4 n = kn[k];
+                const int ax = abs(n.x), ay = abs(n.y), az = abs(n.z);
                 double sre = 0.0, sim = 0.0;
 #pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    double sn, cs;
-                    if (has_new) {
-                        const double ph = kv.x * ns[s][0] + kv.y * ns[s][1] + kv.z * ns[s][2];
-                        sincos(ph, &sn, &cs);
-                        sre += G.q[s] * cs;
-                        sim += G.q[s] * sn;
-                    }
-                    if (has_old) {
-                        const double ph = kv.x * os[s][0] + kv.y * os[s][1] + kv.z * os[s][2];
-                        sincos(ph, &sn, &cs);
-                        sre -= G.q[s] * cs;
-                        sim -= G.q[s] * sn;
+                for (int pose = 0; pose < 2; ++pose) {
+                    if (pose == 1 ? !has_new : !has_old) continue;
+                    const double sg = pose == 1 ? 1.0 : -1.0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const double2 *tb = ktab + (pose * NS + s) * tw;
+                        const double2 c = cis_from_axes(tb[ax], tb[m0 + ay], tb[m0 + m1 + az], n.x, n.y, n.z);
+                        sre += sg * G.q[s] * c.x;
+                        sim += sg * G.q[s] * c.y;
                     }
                 }
                 const double2 S = Sk[k];
                 dS[k] = make_double2(sre, sim);
                 rsum += kv.w * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
             }
+        }
         hsum = wave_sum_d(hsum);
         gsum = wave_sum_d(gsum);
         rsum = wave_sum_d(rsum);
@@ -325,13 +338,17 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
 hipError_t launch_mc(const McArgs &a, hipStream_t st)
 {
     const dim3 grid((unsigned)a.nsys), block(MC_WG);
-    switch (a.guest.nsites) {
-    case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(mc_chain_kernel<2>, grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL(mc_chain_kernel<3>, grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(mc_chain_kernel<4>, grid, block, 0, st, a); break;
-    case 5: hipLaunchKernelGGL(mc_chain_kernel<5>, grid, block, 0, st, a); break;
-    case 6: hipLaunchKernelGGL(mc_chain_kernel<6>, grid, block, 0, st, a); break;
+    const int ns = a.guest.nsites;
+    // the per-axis phase tables: old and new pose, every site, kn_max[d] + 1 powers per axis
+    const size_t lds = a.guest.charged ? (size_t)2 * ns * a.ktab_w * sizeof(double2) : 0;
+    if (lds > 96 * 1024) return hipErrorInvalidValue;
+    switch (ns) {
+    case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL(mc_chain_kernel<2>, grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL(mc_chain_kernel<3>, grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL(mc_chain_kernel<4>, grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL(mc_chain_kernel<5>, grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL(mc_chain_kernel<6>, grid, block, lds, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

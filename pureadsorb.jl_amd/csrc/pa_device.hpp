@@ -19,6 +19,7 @@ struct FwDesc {
     int upper;             // A is upper triangular (cell_matrix's form)
     int cg_off;            // hard-core cell grid: first entry of the framework in cell_start
     int cg_nc[3];          // cells per axis (fractional cell edge 1/cg_nc)
+    int kn_max[3];         // largest |n_d| of the framework's k-vectors k = 2 pi A^-T n
 };
 
 // The guest, with compact type indices.
@@ -161,6 +162,27 @@ __host__ __device__ inline double site_pair_energy(double r2, double sg, double 
     return e;
 }
 
+// e^{i k.r} of a guest site for k = 2 pi A^-T n at the Cartesian point r = A f, as the product
+// over axes of e^{2 pi i n_d f_d}. axis_power gives e^{2 pi i m f_d} (m >= 0); cis_from_axes
+// multiplies the three, conjugating the axes with negative n_d. The move path's kernel (from
+// per-attempt LDS tables) and the host's structure-factor rebuild evaluate both identically.
+__host__ __device__ inline double2 axis_power(const double *invA, const double r[3], int d, int m)
+{
+#pragma clang fp contract(off)
+    const double f = invA[d] * r[0] + invA[d + 3] * r[1] + invA[d + 6] * r[2];
+    double sn, cs;
+    sincospi(2.0 * ((double)m * f), &sn, &cs);
+    return make_double2(cs, sn);
+}
+
+__host__ __device__ inline double2 cis_from_axes(double2 e0, double2 e1, double2 e2, int nx, int ny, int nz)
+{
+#pragma clang fp contract(off)
+    const double i0 = nx < 0 ? -e0.y : e0.y, i1 = ny < 0 ? -e1.y : e1.y, i2 = nz < 0 ? -e2.y : e2.y;
+    const double pr = e0.x * e1.x - i0 * i1, pi = e0.x * i1 + i0 * e1.x;
+    return make_double2(pr * e2.x - pi * i2, pr * i2 + pi * e2.x);
+}
+
 // Move types of the move path: the reference's three NVT moves (src/moves.jl:69-71) and the two
 // exchange moves.
 constexpr int MOVE_TRANSLATION = 1, MOVE_ROTATION = 2, MOVE_REINSERTION = 3, MOVE_INSERTION = 4, MOVE_DELETION = 5;
@@ -175,6 +197,8 @@ struct McArgs {
     const double4 *atom64;
     const int *atype;
     const double4 *k64;         // kx, ky, kz, kprefactor
+    const int4 *kn;             // n of k = 2 pi A^-T n, same order as k64
+    int ktab_w;                 // max over frameworks of kn_max[0] + kn_max[1] + kn_max[2] + 3
     const double2 *pair64;      // [site][type]: sigma, epsilon
     int ntypes_c;
     double rc_lj2, rc_ew2;

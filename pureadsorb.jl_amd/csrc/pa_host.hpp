@@ -130,6 +130,7 @@ struct pa_batch {
     bool fullk = false;
     double bin_scale = 0.0;  // mixed path: bin edge multiplier (0: not yet tuned, 1 used meanwhile)
     std::vector<double4> atom64_h, k64_h;
+    std::vector<int4> kn_h;    // n of every k-vector of k64_h
     std::vector<double2> S64_h;
     std::vector<int> atype_h;
     pa::DevAlloc d_corefx, d_coreidx, d_cell_start, d_rho2, d_cand, d_ncand;

@@ -663,6 +663,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
                             halfrange, uniq[u] + 1, halfrange, 1.0e-3 * KT_REF);
         }
         D.nk = (int)k64.size() - D.k_off;
+        D.kn_max[0] = D.kn_max[1] = D.kn_max[2] = 0;
+        for (size_t i = D.k_off; i < kn.size(); ++i) {
+            D.kn_max[0] = std::max(D.kn_max[0], std::abs(kn[i].x));
+            D.kn_max[1] = std::max(D.kn_max[1], std::abs(kn[i].y));
+            D.kn_max[2] = std::max(D.kn_max[2], std::abs(kn[i].z));
+        }
         // constant_offset (src/batch.jl:396-407; tail_delta src/forcefield.jl:103-124)
         double tail = 0.0;
         if (ff->tail) {
@@ -721,6 +727,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     b->atom64_h = atom64;
     b->atype_h = atype;
     b->k64_h = k64;
+    b->kn_h = kn;
     b->S64_h = S64;
 
     // ---- hard-core bound B_s and kmin per framework (src/batch.jl:395,409-423; src/reject.jl).
