@@ -104,6 +104,8 @@ struct BinnedArgs {
     // hard-core rejection (phase 0): atoms of each framework sorted by grid cell
     const uint4 *corefx;       // fixed-point x, y, z and compact type, by cell
     const int *cell_start;     // per framework: cg_nc[0]*cg_nc[1]*cg_nc[2] + 1 offsets
+    const int *nbr_start;      // same layout: each cell's range of nbr_atoms
+    const uint4 *nbr_atoms;    // per cell, the atoms of the cells within one cell (as corefx)
     const float *rho2;         // nsys x NS x ntypes_c rejection radii^2 (Å^2), rounded down
     int *cand;                 // chunk-relative insertions left for the full phase-0 stencil
     int *ncand;                // their count

@@ -491,6 +491,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<float2> kc;
     std::vector<uint4> corefx;
     std::vector<int> coreidx, cell_start;
+    std::vector<int> nbr_start;   // per framework cell (cell_start layout): first entry in nbr_atoms
+    std::vector<uint4> nbr_atoms;
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
@@ -560,6 +562,26 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             }
             corefx.insert(corefx.end(), sorted.begin(), sorted.end());
             coreidx.insert(coreidx.end(), sidx.begin(), sidx.end());
+            // near-stencil atom lists: for every cell, the atoms of the cells within one cell of
+            // it (each cell once when an axis has fewer than 3 cells), contiguous, so phase 0's
+            // near test reads one range per site
+            const int nc0 = D.cg_nc[0], nc1 = D.cg_nc[1], nc2 = D.cg_nc[2];
+            const int k0 = std::min(3, nc0), k1 = std::min(3, nc1), k2 = std::min(3, nc2);
+            for (int c = 0; c < nct; ++c) {
+                if (nbr_atoms.size() > (size_t)INT32_MAX)
+                    return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
+                nbr_start.push_back((int)nbr_atoms.size());
+                const int h0 = c % nc0, h1 = (c / nc0) % nc1, h2 = c / (nc0 * nc1);
+                auto first = [](int h, int k, int nc) { return k == nc ? 0 : (h - 1 + nc) % nc; };
+                const int f0 = first(h0, k0, nc0), f1 = first(h1, k1, nc1), f2 = first(h2, k2, nc2);
+                for (int i2 = 0; i2 < k2; ++i2)
+                    for (int i1 = 0; i1 < k1; ++i1)
+                        for (int i0 = 0; i0 < k0; ++i0) {
+                            const int cc = (f0 + i0) % nc0 + nc0 * ((f1 + i1) % nc1 + nc1 * ((f2 + i2) % nc2));
+                            nbr_atoms.insert(nbr_atoms.end(), sorted.begin() + cnt[cc], sorted.begin() + cnt[cc + 1]);
+                        }
+            }
+            nbr_start.push_back((int)nbr_atoms.size());
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
         FwGradHost &gh = b->fgh[u];
@@ -807,6 +829,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     HIP_TRY(upload(b->d_corefx, corefx));
     HIP_TRY(upload(b->d_coreidx, coreidx));
     HIP_TRY(upload(b->d_cell_start, cell_start));
+    HIP_TRY(upload(b->d_nbr_start, nbr_start));
+    HIP_TRY(upload(b->d_nbr_atoms, nbr_atoms));
     *out = b.release();
     return ok();
 }
@@ -1094,6 +1118,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.guest = b->gdev;
     A.corefx = b->d_corefx.as<uint4>();
     A.cell_start = b->d_cell_start.as<int>();
+    A.nbr_start = b->d_nbr_start.as<int>();
+    A.nbr_atoms = b->d_nbr_atoms.as<uint4>();
     A.rho2 = b->d_rho2.as<float>();
     A.cand = b->d_cand.as<int>();
     A.ncand = b->d_ncand.as<int>();

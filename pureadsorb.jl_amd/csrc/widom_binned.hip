@@ -143,8 +143,34 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
     const int key = B.bin_base + bx + B.nb[0] * (by + B.nb[1] * bz);
     bool pending = false;
     if (B.reach[0] >= 0) {
-        const int n0 = min(B.reach[0], 1), n1 = min(B.reach[1], 1), n2 = min(B.reach[2], 1);
-        if (stencil_hit<NS>(a, F, s, sp, n0, n1, n2)) {
+        // the near stencil (home cell and its neighbours; reach >= 1 whenever phase 0 runs) from
+        // the home cell's precomputed atom list: one contiguous range per site
+        const int *nbs = a.nbr_start + F.cg_off;
+        const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
+        int j0[NS], j1[NS];
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const int h0 = (int)(((uint64_t)sp[t][0] * (uint64_t)F.cg_nc[0]) >> 32);
+            const int h1 = (int)(((uint64_t)sp[t][1] * (uint64_t)F.cg_nc[1]) >> 32);
+            const int h2 = (int)(((uint64_t)sp[t][2] * (uint64_t)F.cg_nc[2]) >> 32);
+            const int c = h0 + F.cg_nc[0] * (h1 + F.cg_nc[1] * h2);
+            j0[t] = nbs[c];
+            j1[t] = nbs[c + 1];
+        }
+        bool hit = false;
+#pragma unroll
+        for (int t = 0; t < NS; ++t)
+            for (int jj = j0[t]; jj < j1[t] && !hit; ++jj) {
+                const uint4 at = a.nbr_atoms[jj];
+                const float dx = (float)(int32_t)(sp[t][0] - at.x);
+                const float dy = (float)(int32_t)(sp[t][1] - at.y);
+                const float dz = (float)(int32_t)(sp[t][2] - at.z);
+                const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
+                const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
+                const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
+                if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
+            }
+        if (hit) {
             a.key[g - a.g0] = -1;
             a.dU[g - a.out_base] = INFINITY;
             return;
