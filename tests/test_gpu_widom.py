@@ -409,3 +409,21 @@ def test_bin_scale_autotune_and_pinning(c1p):
     assert abs(r3.ln_K_H - r1.ln_K_H) <= 1e-6 * max(1.0, abs(r1.ln_K_H))
     with pytest.raises(pa.ArgumentError):
         b.bin_scale = -1.0
+
+
+def test_repeated_calls_follow_changed_arguments(c1p):
+    """Per-call tables (system offsets, bins, tiles, phase-0 radii) are re-sent only when they
+    change: alternating temperatures, insertion counts and seeds on one batch give exactly
+    what a fresh batch gives for each call."""
+    fw, ff, g, ew, b0, o = c1p
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    b.bin_scale = 1.0
+    calls = [(298.15, 120_000, 4), (250.0, 120_000, 4), (298.15, 80_000, 5), (350.0, 80_000, 4), (298.15, 120_000, 4)]
+    got = [pa.widom(b, g, T=T, ninsert=n, seed=sd, nblocks=4)[0] for T, n, sd in calls]
+    for (T, n, sd), r in zip(calls, got):
+        fresh = pa.FrameworkBatch([fw], ff, g, ew)
+        fresh.bin_scale = 1.0
+        ref = pa.widom(fresh, g, T=T, ninsert=n, seed=sd, nblocks=4)[0]
+        print(T, n, sd, r.ln_K_H, ref.ln_K_H, flush=True)
+        assert r == ref
+    assert got[0] == got[4]
