@@ -111,7 +111,7 @@ typedef struct pa_timing {
     double reduce_ms;             /* Boltzmann-sum reduction kernels */
     int64_t insertions;
     double binning_ms;            /* mixed path: pose, bin count, scan and scatter launches */
-    double grad_ms;               /* pa_widom_grad: accumulator clears and the gradient kernel */
+    double grad_ms;               /* pa_widom_grad: weighted-insertion selection and the gradient kernels */
     int64_t grad_insertions;      /* pa_widom_grad: insertions with a weight above e^-40 of the best */
 } pa_timing;
 

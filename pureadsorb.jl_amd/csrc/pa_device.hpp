@@ -111,7 +111,8 @@ struct BinnedArgs {
     int *ncand;                // their count
 };
 
-// Gradient launch: one weighted insertion per wavefront (widom_grad.hip).
+// Gradient launches (widom_grad.hip): the weighted-insertion list, its per-insertion sites and
+// weights, and the accumulators of the gather and k-space kernels.
 struct GradArgs {
     const FwDesc *fw;
     const int *fw_of;
