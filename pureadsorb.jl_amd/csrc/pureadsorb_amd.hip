@@ -27,6 +27,7 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
+int binned_listcap_max(int ns);
 hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
@@ -961,7 +962,6 @@ static LseH lse_merge_h(LseH x, LseH y)
 // has few insertions for its volume.
 constexpr double BIN_EDGE_MIN = 2.0;
 constexpr double BIN_TARGET_INSERTIONS = 64.0;
-constexpr int LISTCAP_MAX = 3072;
 
 // Delta U of flattened insertions g in [gb, ge) (system s owns g in [seg[s], seg[s+1])) into
 // b->d_dU[g - out_base], in chunks of at most `chunk` insertions. Records a HIP event pair around
@@ -1071,7 +1071,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         }
     }
     const int nbins = base[nsys];
-    const int listcap = (int)std::min<double>(LISTCAP_MAX, std::ceil(cap_est / 64.0) * 64.0);
+    const int listcap = (int)std::min<double>(binned_listcap_max(b->nsites), std::ceil(cap_est / 64.0) * 64.0);
     HIP_TRY(upload_cached(b->d_bins, bins));
     HIP_TRY(upload_cached(b->d_bin_base, base));
     HIP_TRY(upload_cached(b->d_rho2, rho2));

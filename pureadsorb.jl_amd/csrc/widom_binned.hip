@@ -22,7 +22,7 @@
 
 namespace pa {
 
-constexpr int BIN_WG = 512;
+constexpr int BIN_WG = 768;
 constexpr int BIN_NW = BIN_WG / 64;
 constexpr int BRING = 128;  // per-wave ring; drained after every site sweep (< 64 + 64 pending)
 constexpr int BIN_MAX_TYPES = 64;
@@ -526,6 +526,17 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes)
 {
     *bytes = 0;
     return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (int *)nullptr, (int *)nullptr, nbins + 1);
+}
+
+// Largest neighbour-list capacity (entries, a multiple of 64) that keeps a bin workgroup's LDS
+// within BIN_LDS_BUDGET, so that two workgroups share a CU's 160 KB.
+constexpr size_t BIN_LDS_BUDGET = 80 * 1024;
+int binned_listcap_max(int ns)
+{
+    const size_t fixed = binned_lds_bytes(0, ns);
+    int cap = (int)((BIN_LDS_BUDGET - fixed) / 18) & ~63;
+    while (cap > 64 && binned_lds_bytes(cap, ns) > BIN_LDS_BUDGET) cap -= 64;
+    return cap;
 }
 
 // Steps 1-3 of the pipeline (pose + bin counts, scan, scatter), after clearing bin_count and
