@@ -240,11 +240,12 @@ template <bool CHARGED>
 __device__ inline float pair_term(float r2, float qj, int info, const float2 *pair, int ntc, const float *qke,
                                   float rc_lj2, float rc_ew2, float alpha, float alpha2)
 {
-    const int site = info >> 16, t = info & 0xffff;
+    // info: site << 16 | index of the (site, atom type) pair parameters
+    const int site = info >> 16, pidx = info & 0xffff;
     r2 = fmaxf(r2, 1.0e-30f);
     float e = 0.0f;
     if (r2 < rc_lj2) {
-        const float2 p = pair[site * ntc + t];
+        const float2 p = pair[pidx];
         const float x = p.x * __builtin_amdgcn_rcpf(r2);
         const float x3 = x * x * x;
         e = p.y * x3 * (x3 - 1.0f);
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                         const uint32_t slot = (tail + (uint32_t)__popcll(mask & below)) & (BRING - 1);
                         ring_r2[slot] = r2;
                         ring_q[slot] = E.w;
-                        ring_info[slot] = (t << 16) | ht;
+                        ring_info[slot] = (t << 16) | (t * ntc + ht);
                     }
                     tail += (uint32_t)__popcll(mask);
                     if (tail - head >= 64u) {
