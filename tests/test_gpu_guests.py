@@ -108,3 +108,25 @@ def test_guest_move_traces(ns):
         # an uncharged guest keeps no structure factor on the device
         assert np.allclose(st.sk(0), m.sk(), rtol=0, atol=1e-9)
     assert np.all(np.abs(pa.audit_energy(st)) < 1e-8)
+
+
+@pytest.mark.parametrize("ns", [1, 2, 5])
+def test_guest_gradients(ns):
+    """pa_widom_grad for other site counts against the oracle's FP64 gradient (1e-3 relative per
+    type entry; charge entries within 1e-3 of their largest)."""
+    ff = synth.synthetic_forcefield(11.0)
+    g = _guest(ff, ns)
+    fw = pa.replicate(synth.synthetic_framework(12.0, 12.0, 12.0, 90, 90, 90, 40, 5, charge_sd=0.5), (2, 2, 2))
+    ew = pa.EwaldParams(11.0)
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    n = 20000
+    gd = pa.widom_grad(b, g, T=T, ninsert=n, seed=3, nblocks=4, precision="mixed")[0]
+    ref = oracle_for(fw, ff, g, ew).widom_grad(T, n, 3, 0, nthreads=oracle_threads())
+    assert abs(gd.result.ln_K_H - ref["ln_K_H"]) <= 1e-4
+    for key in ("dlnkh_deps", "dlnkh_dsig", "dqst_deps", "dqst_dsig"):
+        an, rf = getattr(gd, key), ref[key]
+        print(f"{ns} sites {key}: max rel {np.max(np.abs(an - rf) / np.maximum(np.abs(rf), 1e-9)):.2e}", flush=True)
+        assert np.all(np.abs(an - rf) <= 1e-3 * np.abs(rf) + 1e-9), key
+    for key in ("dlnkh_dq", "dqst_dq"):
+        an, rf = getattr(gd, key), ref[key]
+        assert np.abs(an - rf).max() <= 1e-3 * np.abs(rf).max() + 1e-12, key
