@@ -133,9 +133,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000,
                     help="insertions timed on the CPU oracle, two-phase, all host threads (0: skip)")
     ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rubtak"), default="config1",
                     help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions; "
-                         "config3: gradient cost on 64 frameworks x 2e5 insertions; config4: GCMC move path")
+                         "config3: gradient cost on 64 frameworks x 2e5 insertions; config4: GCMC move path; "
+                         "rubtak: the reference's own benchmark system (RUBTAK 3x3x3 + CO2, TraPPE, 12 A)")
     ap.add_argument("--bin-scale", type=float, default=0.0,
                     help="mixed-path bin edge multiplier (0: the library tunes it on the first call)")
     ap.add_argument("--gcmc-warmup", type=int, default=20000)
@@ -166,6 +167,8 @@ def main():
         return bench_config3(args)
     if args.workload == "config4":
         return bench_config4(args)
+    if args.workload == "rubtak":
+        return bench_rubtak(args)
     fw, ff, g, ew = workload()
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
@@ -305,6 +308,33 @@ def bench_config2(args):
     out["binding_max_abs_d_ln_K_H"] = float(dln[bind].max()) if bind.any() else None
     out["nonbinding_max_rel_d_ln_K_H"] = float(rel[~bind].max()) if (~bind).any() else None
     log(f"config2 mixed: {ins_s:.4e} ins/s; f64 {ninsert / w64:.4e}; max |d ln K_H| {dln.max():.2e}")
+    print(json.dumps(out), flush=True)
+
+
+def bench_rubtak(args):
+    """The reference's published Widom benchmark system (RUBTAK 3x3x3, 3,078 atoms, CO2, TraPPE,
+    LJ 12 A / Ewald 12 A at precision 1e-6, 298.15 K, 1e6 insertions per call): mixed and FP64
+    plain throughput on the same streams, for comparison with the reference's own numbers on
+    other GPUs (docs/src/benchmarks.md)."""
+    golden = os.path.join(ROOT, "tests", "golden")
+    ff = pa.read_forcefield(os.path.join(golden, "trappe.yaml"))
+    g = pa.read_guest(os.path.join(golden, "co2.yaml"), ff)
+    fw = pa.replicate(pa.read_cif(os.path.join(golden, "RUBTAK.cif")), (3, 3, 3))
+    b = pa.FrameworkBatch([fw], ff, g, pa.EwaldParams(12.0, 1e-6))
+    wall, ems, nl, rm = time_steps(b, g, args.ninsert, args.steps, args.warmup, "mixed")
+    w64, e64, n64, r64 = time_steps(b, g, args.ninsert, max(1, args.f64_steps), 1, "f64")
+    rej = float(pa.hardcore_flags(b, 0, 42, T_K, 0, min(args.ninsert, 200_000)).mean())
+    out = {"metric": "Widom insertions/s on 1 GPU (RUBTAK + CO2)", "value": args.steps * args.ninsert / wall,
+           "unit": "insertions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "RUBTAK.cif (tests/golden), replicated 3x3x3",
+           "config": {"workload": "RUBTAK 3x3x3 (3,078 atoms) + CO2, TraPPE, LJ 12 A, Ewald 12 A / 1e-6, 298.15 K",
+                      "ninsert_per_step": args.ninsert},
+           "f64_plain": {"value": max(1, args.f64_steps) * args.ninsert / w64, "unit": "insertions/s"},
+           "energy_kernel_ms_per_launch": ems / max(nl, 1), "phase0_rejected_fraction": rej,
+           "ln_K_H_mixed": rm.ln_K_H, "ln_K_H_f64": r64.ln_K_H, "q_st_mixed_eV": rm.q_st, "q_st_f64_eV": r64.q_st}
+    log(f"rubtak: mixed {out['value']:.4e} ins/s, f64 {out['f64_plain']['value']:.4e} ins/s, "
+        f"d ln K_H {abs(rm.ln_K_H - r64.ln_K_H):.2e}")
     print(json.dumps(out), flush=True)
 
 
