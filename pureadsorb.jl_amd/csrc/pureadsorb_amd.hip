@@ -189,11 +189,12 @@ void build_ktable(const double A[9], double kmax, double alpha, double V, KTable
             }
 }
 
-// S(k) = sum_j q_j exp(i k.r_j) over the framework's atoms. The phase of atom j at k = B n is
-// 2 pi n.f_j; exp(i 2 pi n_d f_jd) is tabulated per atom and axis, so each (k, atom) term is
-// two complex products instead of a sincos.
+// S(k) = sum_j q_j exp(i k.r_j) over the framework's atoms, for the k-vectors with need[i] != 0
+// (the others stay 0). The phase of atom j at k = B n is 2 pi n.f_j; exp(i 2 pi n_d f_jd) is
+// tabulated per axis and power, atoms contiguous, so each (k, atom) term is two complex
+// products instead of a sincos.
 void structure_factor(const KTable &kt, const std::vector<double> &frac, const std::vector<double> &q,
-                      std::vector<double> &Sre, std::vector<double> &Sim)
+                      const std::vector<char> &need, std::vector<double> &Sre, std::vector<double> &Sim)
 {
     const int64_t nk = (int64_t)kt.kpref.size(), na = (int64_t)q.size();
     Sre.assign(nk, 0.0);
@@ -203,34 +204,39 @@ void structure_factor(const KTable &kt, const std::vector<double> &frac, const s
     for (int64_t i = 0; i < nk; ++i)
         for (int d = 0; d < 3; ++d) nmax[d] = std::max(nmax[d], std::abs(kt.n[3 * i + d]));
     const int w0 = nmax[0] + 1, w1 = 2 * nmax[1] + 1, w2 = 2 * nmax[2] + 1;
+    // tables [power][atom]
     std::vector<double> c0((size_t)na * w0), s0((size_t)na * w0), c1((size_t)na * w1), s1((size_t)na * w1),
         c2((size_t)na * w2), s2((size_t)na * w2);
     parallel_for(na, [&](int64_t b, int64_t e) {
         for (int64_t j = b; j < e; ++j) {
             for (int m = 0; m < w0; ++m) {
                 double ph = TWO_PI * (m * frac[3 * j]);
-                c0[j * w0 + m] = std::cos(ph); s0[j * w0 + m] = std::sin(ph);
+                c0[m * na + j] = std::cos(ph); s0[m * na + j] = std::sin(ph);
             }
             for (int m = 0; m < w1; ++m) {
                 double ph = TWO_PI * ((m - nmax[1]) * frac[3 * j + 1]);
-                c1[j * w1 + m] = std::cos(ph); s1[j * w1 + m] = std::sin(ph);
+                c1[m * na + j] = std::cos(ph); s1[m * na + j] = std::sin(ph);
             }
             for (int m = 0; m < w2; ++m) {
                 double ph = TWO_PI * ((m - nmax[2]) * frac[3 * j + 2]);
-                c2[j * w2 + m] = std::cos(ph); s2[j * w2 + m] = std::sin(ph);
+                c2[m * na + j] = std::cos(ph); s2[m * na + j] = std::sin(ph);
             }
         }
     });
-    parallel_for(nk, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
+    std::vector<int64_t> todo;
+    for (int64_t i = 0; i < nk; ++i)
+        if (need[i]) todo.push_back(i);
+    parallel_for((int64_t)todo.size(), [&](int64_t b, int64_t e) {
+        for (int64_t t = b; t < e; ++t) {
+            const int64_t i = todo[t];
             const int m0 = kt.n[3 * i], m1 = kt.n[3 * i + 1] + nmax[1], m2 = kt.n[3 * i + 2] + nmax[2];
+            const double *a_r = &c0[(size_t)m0 * na], *a_i = &s0[(size_t)m0 * na];
+            const double *b_r = &c1[(size_t)m1 * na], *b_i = &s1[(size_t)m1 * na];
+            const double *c_r = &c2[(size_t)m2 * na], *c_i = &s2[(size_t)m2 * na];
             double re = 0.0, im = 0.0;
             for (int64_t j = 0; j < na; ++j) {
-                double ar = c0[j * w0 + m0], ai = s0[j * w0 + m0];
-                double br = c1[j * w1 + m1], bi = s1[j * w1 + m1];
-                double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
-                double cr = c2[j * w2 + m2], ci = s2[j * w2 + m2];
-                double rr = pr * cr - pi * ci, ri = pr * ci + pi * cr;
+                const double pr = a_r[j] * b_r[j] - a_i[j] * b_i[j], pi = a_r[j] * b_i[j] + a_i[j] * b_r[j];
+                const double rr = pr * c_r[j] - pi * c_i[j], ri = pr * c_i[j] + pi * c_r[j];
                 re += q[j] * rr;
                 im += q[j] * ri;
             }
@@ -613,23 +619,33 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         KTable kt;
         double self_mean = 0.0, halfrange = 0.0;
         D.k_off = (int)k64.size();
-        if (charged || !rep1) {
-            build_ktable(D.A, kmax, alpha, V, kt);
-            structure_factor(kt, frac, q, kt.Sre, kt.Sim);
-        }
+        if (charged || !rep1) build_ktable(D.A, kmax, alpha, V, kt);
         const int nkf = (int)kt.kpref.size();
         auto coupled = [&](int i) {
             for (int d = 0; d < 3; ++d)
                 if (kt.n[3 * i + d] % F.replication[d] != 0) return false;
             return true;
         };
-        // replication check on up to 32 uncoupled k-vectors (src/batch.jl:173-195)
+        // the host structure factor is needed at the k-vectors the batch keeps (every one with
+        // fullk, the coupled ones otherwise) and at the replication check's sample of up to 32
+        // uncoupled ones (src/batch.jl:173-195)
+        std::vector<int> unc;
+        size_t unc_stride = 1;
         if (!rep1) {
-            std::vector<int> unc;
             for (int i = 0; i < nkf; ++i)
                 if (!coupled(i)) unc.push_back(i);
+            unc_stride = std::max<size_t>(1, (unc.size() + 31) / 32);
+        }
+        if (nkf > 0) {
+            std::vector<char> need(nkf, 0);
+            if (charged)
+                for (int i = 0; i < nkf; ++i) need[i] = (fullk || coupled(i)) ? 1 : 0;
+            for (size_t t = 0, taken = 0; t < unc.size() && taken < 32; t += unc_stride, ++taken) need[unc[t]] = 1;
+            structure_factor(kt, frac, q, need, kt.Sre, kt.Sim);
+        }
+        if (!rep1) {
             if (!unc.empty()) {
-                size_t stride = std::max<size_t>(1, (unc.size() + 31) / 32);
+                const size_t stride = unc_stride;
                 double maxS = 0.0, sabs = 0.0;
                 int arg = -1, taken = 0;
                 for (size_t t = 0; t < unc.size() && taken < 32; t += stride, ++taken) {
@@ -659,21 +675,23 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             }
             // orientation-averaged guest self term over the FULL k table (src/batch.jl:360-389)
             double samples[64];
-            for (int o = 0; o < 64; ++o) {
-                double gs[MAX_SITES][3];
-                for (int a = 0; a < NS; ++a) rotate(&self_q[4 * o], guest->sites + 3 * a, gs[a]);
-                double acc = 0.0;
-                for (int i = 0; i < nkf; ++i) {
-                    double re = 0.0, im = 0.0;
-                    for (int a = 0; a < NS; ++a) {
-                        double ph = kt.k[3 * i] * gs[a][0] + kt.k[3 * i + 1] * gs[a][1] + kt.k[3 * i + 2] * gs[a][2];
-                        re += guest->charges[a] * std::cos(ph);
-                        im += guest->charges[a] * std::sin(ph);
+            parallel_for(64, [&](int64_t ob, int64_t oe) {
+                for (int64_t o = ob; o < oe; ++o) {
+                    double gs[MAX_SITES][3];
+                    for (int a = 0; a < NS; ++a) rotate(&self_q[4 * o], guest->sites + 3 * a, gs[a]);
+                    double acc = 0.0;
+                    for (int i = 0; i < nkf; ++i) {
+                        double re = 0.0, im = 0.0;
+                        for (int a = 0; a < NS; ++a) {
+                            double ph = kt.k[3 * i] * gs[a][0] + kt.k[3 * i + 1] * gs[a][1] + kt.k[3 * i + 2] * gs[a][2];
+                            re += guest->charges[a] * std::cos(ph);
+                            im += guest->charges[a] * std::sin(ph);
+                        }
+                        acc += kt.kpref[i] * (re * re + im * im);
                     }
-                    acc += kt.kpref[i] * (re * re + im * im);
+                    samples[o] = KE * acc;
                 }
-                samples[o] = KE * acc;
-            }
+            });
             double sum = 0.0, mx = samples[0], mn = samples[0];
             for (double x : samples) { sum += x; mx = std::max(mx, x); mn = std::min(mn, x); }
             self_mean = sum / 64.0;
