@@ -1210,7 +1210,7 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     HIP_TRY(b->d_dU.ensure(sizeof(double) * G));
 
     // tiles: each block of each system split into runs of at most TILE insertions
-    const int64_t TILE = 8192;
+    const int64_t TILE = 2048;
     std::vector<int64_t> tile_g0;
     std::vector<int> tile_n;
     std::vector<int> tile_owner;  // s * nblocks + blk
