@@ -833,7 +833,9 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     int st = device_ready();
     if (st) return st;
     HIP_TRY(hipStreamCreate(&b->stream));
-    for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (no cache write-back and invalidation around the
+    // kernels they bracket); results reach the host through copies and stream synchronisation
+    for (auto &e : b->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     HIP_TRY(upload(b->d_fw, b->desc));
     HIP_TRY(upload(b->d_fw_of, b->fw_of));
     HIP_TRY(upload(b->d_atom64, atom64));
@@ -997,7 +999,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     const int64_t nchunks = (ge - gb + CH - 1) / CH;
     while ((int64_t)b->kev.size() < 2 * nchunks) {
         hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         b->kev.push_back(e);
     }
     *launches = 0;
@@ -1416,8 +1418,8 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
             if (b) (void)hipEventDestroy(b);
         }
     } evp;
-    HIP_TRY(hipEventCreate(&evp.a));
-    HIP_TRY(hipEventCreate(&evp.b));
+    HIP_TRY(hipEventCreateWithFlags(&evp.a, hipEventDisableSystemFence));
+    HIP_TRY(hipEventCreateWithFlags(&evp.b, hipEventDisableSystemFence));
     hipEvent_t g0 = evp.a, g1 = evp.b;
 
     // weighted insertions, selected on the device: w = exp(-beta U - m_s) relative to the system's
