@@ -395,7 +395,7 @@ extern "C" int pa_state_create(pa_batch *b, const pa_guest *guest, const int32_t
     HIP_TRY(upload(S->d_quat, quat));
     HIP_TRY(upload(S->d_host_e, host_e));
     HIP_TRY(upload(S->d_Sk, Sk));
-    HIP_TRY(upload(S->d_dS, Sk));
+    HIP_TRY(S->d_dS.ensure(sizeof(double2) * MC_SPEC * std::max<size_t>(Sk.size(), 1)));
     HIP_TRY(upload(S->d_skabs, std::vector<double>(Sk.size(), 0.0)));
     HIP_TRY(upload(S->d_energy, energy));
     HIP_TRY(upload(S->d_energy_abs, std::vector<double>(nsys, 0.0)));

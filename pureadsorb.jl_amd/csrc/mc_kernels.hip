@@ -21,6 +21,9 @@ namespace pa {
 
 constexpr int MC_WG = 512;
 constexpr int MC_NW = MC_WG / 64;
+constexpr int MC_GW = MC_WG / MC_SPEC;   // work-items per speculative attempt
+constexpr int MC_GNW = MC_GW / 64;
+static_assert(MC_GW % 64 == 0, "each attempt takes whole wavefronts");
 
 __device__ inline double wave_sum_d(double v)
 {
@@ -54,6 +57,9 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
 {
 #pragma clang fp contract(off)
     __shared__ double red[3][MC_NW];
+    // per group: the attempt's slot, acceptance uniform and proposed pose
+    __shared__ int sp_slot[MC_SPEC];
+    __shared__ double sp_u[MC_SPEC], sp_p[MC_SPEC][3], sp_q[MC_SPEC][4];
     // per-axis phase tables of the attempt's old and new sites: e^{2 pi i m f_d}, m = 0..kn_max[d]
     extern __shared__ double2 ktab[];
     const int n = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
     const int *types = a.atype + F.atom_off;
     const double4 *ks = a.k64 + F.k_off;
     double2 *Sk = a.Sk + a.sk_off[n];
-    double2 *dS = a.dS + a.sk_off[n];
+    double2 *dS = a.dS + (size_t)MC_SPEC * a.sk_off[n];   // MC_SPEC slices of nk
     double *skabs = a.sk_abs + a.sk_off[n];
     const int nk = F.nk, natoms = F.natoms;
     const GuestDev &G = a.guest;
@@ -89,8 +95,45 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
     const double fugV = a.fug ? a.fug[n] * F.volume : 0.0;
     const double cp = a.const_p ? a.const_p[n] : 0.0, cq = a.const_q ? a.const_q[n] : 0.0;
 
-    auto attempt = [&](int type) {
-        Chain rng(seed, ctr, 0);
+    // Speculative rounds: the workgroup splits into MC_SPEC groups; group gi evaluates attempt
+    // ctr + gi against the state at the start of the round. Every attempt's proposal and
+    // acceptance draw come from its own counter (ChainRNG(chain seed, attempt, 0)), so until the
+    // first accepted attempt of the round each one is exactly what a sequential chain would
+    // evaluate: the round commits its first accepted attempt (if any), keeps the decisions of the
+    // attempts before it and discards the rest, which the next round evaluates again.
+    const int gi = tid / MC_GW, gt = tid - gi * MC_GW;
+    const int m0 = F.kn_max[0] + 1, m1 = F.kn_max[1] + 1, tw = m0 + m1 + F.kn_max[2] + 1;
+    double2 *mytab = ktab + (size_t)gi * 2 * NS * tw;
+    double2 *mydS = dS + (size_t)gi * nk;
+
+    // cycle mode: a cycle of chain n is max(occupancy at its start, min_cycle) attempts, and its
+    // sample is taken at its end. Called before every attempt; false once the run is over.
+    int64_t c = 0, m = 0, len = 0;
+    bool finished = false;
+    auto cycle_boundary = [&]() -> bool {
+        while (m == len) {
+            if (c > 0) {
+                if (occ > maxocc) maxocc = occ;
+                if (a.sampling && tid == 0) {
+                    a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = E;
+                    a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = (double)occ;
+                }
+            }
+            if (c == a.ncycles) return false;
+            if (occ > maxocc) maxocc = occ;
+            len = occ > a.min_cycle ? occ : a.min_cycle;
+            m = 0;
+            ++c;
+        }
+        return true;
+    };
+
+    // processes up to nv attempts of types tys[0..nv) in order (cycle bookkeeping between
+    // them); returns how many were processed
+    auto round = [&](int nv, const int *tys) -> int {
+        const bool active = gi < nv;
+        const int type = active ? tys[gi] : MOVE_TRANSLATION;
+        Chain rng(seed, ctr + (uint64_t)gi, 0);
         int slot = 0;
         double op[3] = {0, 0, 0}, oq[4] = {0, 0, 0, 1}, np[3] = {0, 0, 0}, nq[4] = {0, 0, 0, 1};
         if (type != MOVE_INSERTION) {
@@ -128,12 +171,18 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         double os[NS][3], ns[NS][3];
         sites_at<NS>(G, op, oq, os);
         sites_at<NS>(G, np, nq, ns);
+        if (active && gt == 0) {
+            sp_slot[gi] = slot;
+            sp_u[gi] = u;
+            for (int i = 0; i < 3; ++i) sp_p[gi][i] = np[i];
+            for (int i = 0; i < 4; ++i) sp_q[gi][i] = nq[i];
+        }
 
         // (1) host-guest real space of the new pose
         double hsum = 0.0;
-        if (has_new)
-            #pragma unroll 1
-            for (int j = tid; j < natoms; j += MC_WG) {
+        if (active && has_new)
+#pragma unroll 1
+            for (int j = gt; j < natoms; j += MC_GW) {
                 const double4 h = atoms[j];
                 const double hp[3] = {h.x, h.y, h.z};
                 const int ht = types[j];
@@ -149,9 +198,9 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         // (2) guest-guest real space: the moving (or inserted, or deleted) guest against every
         // other live guest, one (other guest, site, site) triple per lane step
         double gsum = 0.0;
-        const int ntrip = occ * NS * NS;
-        #pragma unroll 1
-        for (int x = tid; x < ntrip; x += MC_WG) {
+        const int ntrip = active ? occ * NS * NS : 0;
+#pragma unroll 1
+        for (int x = gt; x < ntrip; x += MC_GW) {
             const int jj = x / (NS * NS), sa = (x / NS) % NS, sb = x % NS;
             const int j = gr0 + jj;
             if (has_old && j == slot) continue;
@@ -174,37 +223,38 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         // (own sincospi of 2 m f_d), then each k-vector takes two complex products per site.
         double rsum = 0.0;
         if (G.charged) {
-            const int m0 = F.kn_max[0] + 1, m1 = F.kn_max[1] + 1, tw = m0 + m1 + F.kn_max[2] + 1;
-            for (int e = tid; e < 2 * NS * tw; e += MC_WG) {
-                const int pose = e / (NS * tw), rem = e - pose * NS * tw, t = rem / tw, x = rem - t * tw;
-                const int d = x < m0 ? 0 : (x < m0 + m1 ? 1 : 2);
-                const int m = x - (d == 0 ? 0 : (d == 1 ? m0 : m0 + m1));
-                ktab[e] = axis_power(F.invA, pose ? ns[t] : os[t], d, m);
-            }
+            if (active)
+                for (int e = gt; e < 2 * NS * tw; e += MC_GW) {
+                    const int pose = e / (NS * tw), rem = e - pose * NS * tw, t = rem / tw, x = rem - t * tw;
+                    const int d = x < m0 ? 0 : (x < m0 + m1 ? 1 : 2);
+                    const int m = x - (d == 0 ? 0 : (d == 1 ? m0 : m0 + m1));
+                    mytab[e] = axis_power(F.invA, pose ? ns[t] : os[t], d, m);
+                }
             __syncthreads();
             const int4 *kn = a.kn + F.k_off;
+            if (active)
 #pragma unroll 1
-            for (int k = tid; k < nk; k += MC_WG) {
-                const double4 kv = ks[k];
-                const int4 n = kn[k];
-                const int ax = abs(n.x), ay = abs(n.y), az = abs(n.z);
-                double sre = 0.0, sim = 0.0;
+                for (int k = gt; k < nk; k += MC_GW) {
+                    const double4 kv = ks[k];
+                    const int4 nn = kn[k];
+                    const int ax = abs(nn.x), ay = abs(nn.y), az = abs(nn.z);
+                    double sre = 0.0, sim = 0.0;
 #pragma unroll
-                for (int pose = 0; pose < 2; ++pose) {
-                    if (pose == 1 ? !has_new : !has_old) continue;
-                    const double sg = pose == 1 ? 1.0 : -1.0;
+                    for (int pose = 0; pose < 2; ++pose) {
+                        if (pose == 1 ? !has_new : !has_old) continue;
+                        const double sg = pose == 1 ? 1.0 : -1.0;
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        const double2 *tb = ktab + (pose * NS + s) * tw;
-                        const double2 c = cis_from_axes(tb[ax], tb[m0 + ay], tb[m0 + m1 + az], n.x, n.y, n.z);
-                        sre += sg * G.q[s] * c.x;
-                        sim += sg * G.q[s] * c.y;
+                        for (int s = 0; s < NS; ++s) {
+                            const double2 *tb = mytab + (pose * NS + s) * tw;
+                            const double2 cc = cis_from_axes(tb[ax], tb[m0 + ay], tb[m0 + m1 + az], nn.x, nn.y, nn.z);
+                            sre += sg * G.q[s] * cc.x;
+                            sim += sg * G.q[s] * cc.y;
+                        }
                     }
+                    const double2 S = Sk[k];
+                    mydS[k] = make_double2(sre, sim);
+                    rsum += kv.w * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
                 }
-                const double2 S = Sk[k];
-                dS[k] = make_double2(sre, sim);
-                rsum += kv.w * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
-            }
         }
         hsum = wave_sum_d(hsum);
         gsum = wave_sum_d(gsum);
@@ -215,75 +265,99 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             red[2][wid] = rsum;
         }
         __syncthreads();
-        double e_new = 0.0, gg = 0.0, rec = 0.0;
-        for (int w = 0; w < MC_NW; ++w) {
-            e_new += red[0][w];
-            gg += red[1][w];
-            rec += red[2][w];
-        }
-        rec *= KE;
-        __syncthreads();  // red[] is reused by the next attempt
 
-        // Metropolis decision, identical in every work-item
-        double dU;
-        bool accept;
-        if (type <= MOVE_REINSERTION) {
-            dU = (e_new - a.host_e[slot]) + gg + rec;
-            accept = isfinite(dU) && (-dU / a.kT > log(u)) && occ > 0;
-        } else if (type == MOVE_INSERTION) {
-            dU = e_new + gg + rec + (cp + cq * (double)occ);
-            const double lp = log(fugV) - log((double)occ + 1.0) - log(a.kT);
-            const bool would = isfinite(dU) && (lp - dU / a.kT > log(u));
-            accept = would && occ < cap;
-            if (would && occ >= cap) caphit = 1;
-        } else {
-            const int occ_eff = occ > 0 ? occ : 1;
-            dU = -(a.host_e[slot] + gg + (cp + cq * (double)(occ - 1))) + rec;
-            const double lp = -(log(fugV) - log((double)(occ_eff - 1) + 1.0) - log(a.kT));
-            accept = isfinite(dU) && (lp - dU / a.kT > log(u)) && occ > 0;
+        // Metropolis decisions of the round's attempts in order, identical in every work-item;
+        // the first accepted one ends the round
+        int done = 0, first = -1;
+        double dU = 0.0, e_first = 0.0;
+        for (int g = 0; g < nv; ++g) {
+            if (a.mode != 0 && !cycle_boundary()) {
+                finished = true;
+                break;
+            }
+            ++m;
+            done = g + 1;
+            double e_new = 0.0, gg = 0.0, rec = 0.0;
+            for (int w = g * MC_GNW; w < (g + 1) * MC_GNW; ++w) {
+                e_new += red[0][w];
+                gg += red[1][w];
+                rec += red[2][w];
+            }
+            rec *= KE;
+            const int ty = tys[g], sl = sp_slot[g];
+            const double ug = sp_u[g];
+            double d;
+            bool accept;
+            if (ty <= MOVE_REINSERTION) {
+                d = (e_new - a.host_e[sl]) + gg + rec;
+                accept = isfinite(d) && (-d / a.kT > log(ug)) && occ > 0;
+            } else if (ty == MOVE_INSERTION) {
+                d = e_new + gg + rec + (cp + cq * (double)occ);
+                const double lp = log(fugV) - log((double)occ + 1.0) - log(a.kT);
+                const bool would = isfinite(d) && (lp - d / a.kT > log(ug));
+                accept = would && occ < cap;
+                if (would && occ >= cap) caphit = 1;
+            } else {
+                const int occ_eff = occ > 0 ? occ : 1;
+                d = -(a.host_e[sl] + gg + (cp + cq * (double)(occ - 1))) + rec;
+                const double lp = -(log(fugV) - log((double)(occ_eff - 1) + 1.0) - log(a.kT));
+                accept = isfinite(d) && (lp - d / a.kT > log(ug)) && occ > 0;
+            }
+            if (a.trace && ntrace < a.trace_cap && tid == 0)
+                a.trace[(int64_t)n * a.trace_cap + ntrace] =
+                    make_double4((double)ty, (double)(sl - gr0), d, accept ? 1.0 : 0.0);
+            ++ntrace;
+            ++att[ty - 1];
+            if (accept) {
+                first = g;
+                dU = d;
+                e_first = e_new;
+                break;
+            }
         }
-        if (a.trace && ntrace < a.trace_cap && tid == 0)
-            a.trace[(int64_t)n * a.trace_cap + ntrace] = make_double4((double)type, (double)(slot - gr0), dU, accept ? 1.0 : 0.0);
-        ++ntrace;
-        if (accept) {
+        if (first >= 0) {
+            const int ty = tys[first], sl = sp_slot[first];
             if (tid == 0) {
-                if (type <= MOVE_REINSERTION) {
-                    a.refpos[slot] = make_double4(np[0], np[1], np[2], 0.0);
-                    a.quat[slot] = make_double4(nq[0], nq[1], nq[2], nq[3]);
-                    a.host_e[slot] = e_new;
-                } else if (type == MOVE_INSERTION) {
+                const double *pp = sp_p[first], *qq = sp_q[first];
+                if (ty <= MOVE_REINSERTION) {
+                    a.refpos[sl] = make_double4(pp[0], pp[1], pp[2], 0.0);
+                    a.quat[sl] = make_double4(qq[0], qq[1], qq[2], qq[3]);
+                    a.host_e[sl] = e_first;
+                } else if (ty == MOVE_INSERTION) {
                     const int s2 = gr0 + occ;
-                    a.refpos[s2] = make_double4(np[0], np[1], np[2], 0.0);
-                    a.quat[s2] = make_double4(nq[0], nq[1], nq[2], nq[3]);
-                    a.host_e[s2] = e_new;
+                    a.refpos[s2] = make_double4(pp[0], pp[1], pp[2], 0.0);
+                    a.quat[s2] = make_double4(qq[0], qq[1], qq[2], qq[3]);
+                    a.host_e[s2] = e_first;
                 } else {
                     const int hi = gr0 + occ - 1;
-                    if (slot != hi) {
-                        a.refpos[slot] = a.refpos[hi];
-                        a.quat[slot] = a.quat[hi];
-                        a.host_e[slot] = a.host_e[hi];
+                    if (sl != hi) {
+                        a.refpos[sl] = a.refpos[hi];
+                        a.quat[sl] = a.quat[hi];
+                        a.host_e[sl] = a.host_e[hi];
                     }
                 }
             }
-            if (type == MOVE_INSERTION) ++occ;
-            if (type == MOVE_DELETION) --occ;
+            if (ty == MOVE_INSERTION) ++occ;
+            if (ty == MOVE_DELETION) --occ;
             E += dU;
             Eabs += fabs(dU);
-            ++acc[type - 1];
-            if (G.charged)
+            ++acc[ty - 1];
+            if (G.charged) {
+                const double2 *fdS = dS + (size_t)first * nk;
                 for (int k = tid; k < nk; k += MC_WG) {
-                    const double2 d = dS[k], S = Sk[k];
-                    Sk[k] = make_double2(S.x + d.x, S.y + d.y);
+                    const double2 dd = fdS[k], S = Sk[k];
+                    Sk[k] = make_double2(S.x + dd.x, S.y + dd.y);
                     skabs[k] += ds_bound;
                 }
+            }
         }
-        ++att[type - 1];
-        ++ctr;
-        __syncthreads();  // committed poses visible to the next attempt
+        ctr += (uint64_t)done;
+        __syncthreads();  // committed poses, red[] and the group records reused by the next round
+        return done;
     };
 
-    auto draw_type = [&]() {
-        Chain r1(seed, ctr, 1);
+    auto draw_type = [&](uint64_t c) {
+        Chain r1(seed, c, 1);
         const double v = unit_f64(r1.next());
         int t = NMOVES;
         for (int i = 0; i < NMOVES; ++i)
@@ -291,35 +365,23 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         return t;
     };
 
-    // one attempt site for both modes; in cycle mode a cycle of chain n is
-    // max(occupancy at its start, min_cycle) attempts, and its sample is taken at its end
-    int64_t c = 0, m = 0, len = 0;
+    // both modes. In cycle mode a round may span cycle boundaries: the move types are drawn per
+    // attempt counter, and a boundary only samples the state and, while no attempt has been
+    // accepted, fixes the next cycle's length from the unchanged occupancy.
+    int tys[MC_SPEC];
     for (;;) {
-        int type;
+        int nv;
         if (a.mode == 0) {
             if (m >= a.nmoves) break;
-            ++m;
-            type = a.movetype;
+            nv = (int)(a.nmoves - m < MC_SPEC ? a.nmoves - m : MC_SPEC);
+            for (int g = 0; g < nv; ++g) tys[g] = a.movetype;
         } else {
-            if (m == len) {
-                if (c > 0) {
-                    if (occ > maxocc) maxocc = occ;
-                    if (a.sampling && tid == 0) {
-                        a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = E;
-                        a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = (double)occ;
-                    }
-                }
-                if (c == a.ncycles) break;
-                if (occ > maxocc) maxocc = occ;
-                len = occ > a.min_cycle ? occ : a.min_cycle;
-                m = 0;
-                ++c;
-                continue;
-            }
-            ++m;
-            type = draw_type();
+            if (!cycle_boundary()) break;
+            nv = MC_SPEC;
+            for (int g = 0; g < nv; ++g) tys[g] = draw_type(ctr + (uint64_t)g);
         }
-        attempt(type);
+        round(nv, tys);
+        if (finished) break;
     }
     if (tid == 0) {
         a.occupancy[n] = occ;
@@ -340,7 +402,7 @@ hipError_t launch_mc(const McArgs &a, hipStream_t st)
     const dim3 grid((unsigned)a.nsys), block(MC_WG);
     const int ns = a.guest.nsites;
     // the per-axis phase tables: old and new pose, every site, kn_max[d] + 1 powers per axis
-    const size_t lds = a.guest.charged ? (size_t)2 * ns * a.ktab_w * sizeof(double2) : 0;
+    const size_t lds = a.guest.charged ? (size_t)MC_SPEC * 2 * ns * a.ktab_w * sizeof(double2) : 0;
     if (lds > 96 * 1024) return hipErrorInvalidValue;
     switch (ns) {
     case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, lds, st, a); break;

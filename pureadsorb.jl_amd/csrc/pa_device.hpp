@@ -190,6 +190,9 @@ __host__ __device__ inline double2 cis_from_axes(double2 e0, double2 e1, double2
 // exchange moves.
 constexpr int MOVE_TRANSLATION = 1, MOVE_ROTATION = 2, MOVE_REINSERTION = 3, MOVE_INSERTION = 4, MOVE_DELETION = 5;
 constexpr int NMOVES = 5;
+// attempts of one chain the move kernel evaluates together (speculatively, against the state at
+// the start of the round); the pending structure-factor changes take MC_SPEC slices per chain
+constexpr int MC_SPEC = 2;
 
 // One launch of the move path: every chain (system) is one workgroup that runs its move attempts
 // back to back (mc_kernels.hip).
