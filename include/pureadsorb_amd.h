@@ -252,6 +252,52 @@ int pa_gcmc_run(pa_state *state, const pa_gcmc_params *params, pa_gcmc_result *o
 /* Per-cycle production samples of the last pa_gcmc_run: total energy (eV) and loading. */
 int pa_gcmc_samples(const pa_state *state, int32_t sys, int64_t n, double *energy, double *loading);
 
+/* NVT driver with Widom insertion along the chain (src/nvt.jl).
+ *
+ * widom_chain_kernel! (src/nvt.jl:85-119): Delta U (eV) of test insertions at the given poses
+ * (pos n x 3 in Å, quat n x 4 as x y z w) into system `sys` as it currently stands: host-guest
+ * real space, the reciprocal cross term against the running structure factor, guest-guest real
+ * space against every live guest, plus insertion_constant_term(sys, occupancy) (src/nvt.jl:44-66).
+ * The state is not changed. FP64. */
+int pa_widom_chain_energies(pa_state *state, int32_t sys, int64_t n, const double *pos, const double *quat,
+                            double *dU);
+
+/* run_nvt! (src/nvt.jl:192-358) on every chain of `state`. n_warmup cycles are discarded, then
+ * n_production cycles are recorded. A cycle is nsteps = max(max occupancy, min_cycle_length)
+ * attempts on every chain (the cycle length is one value for the batch); attempt c of chain n
+ * draws its move type uniformly among translation, rotation and reinsertion from
+ * ChainRNG(chain seed, c, 1) word 0 (the reference draws one type per step for all chains from
+ * a host Xoshiro(seed)), its proposal as pa_mc_step. With no guest slot in the batch no move is
+ * attempted. After every production cycle, n_widom_per_cycle test insertions per system into the
+ * configuration as it then stands; insertion t of production cycle c of system s takes the pose
+ * of pa_widom's insertion j = c * n_widom_per_cycle + t of system s under widom_seed, so a batch
+ * with no guests reproduces pa_widom(seed = widom_seed, ninsert = nsys * n_widom_per_cycle *
+ * n_production, run = 1). The energy audit (pa_audit_energy) runs every n_audit cycles, warmup
+ * and production sharing one count; a production cycle that ends on an audit records the audited
+ * energy. Acceptance counters restart at the start of production. */
+typedef struct pa_nvt_params {
+    double T;
+    int64_t n_warmup, n_production, n_widom_per_cycle, n_audit, min_cycle_length;
+    const double *step_trans, *step_rot; /* per system */
+    uint64_t widom_seed;
+    int32_t nblocks;
+} pa_nvt_params;
+
+/* NVTResult (src/nvt.jl:22-33): energy block statistics over nblocks blocks of cycles, the Widom
+ * estimators with every production cycle as one block (_reduce, src/widom.jl:451-492), and the
+ * production acceptance rate of translation, rotation and reinsertion. */
+typedef struct pa_nvt_result {
+    double energy, energy_err;
+    double mu_ex, mu_ex_err, K_H, K_H_err, q_st, q_st_err;
+    double acceptance[3];
+    int64_t ncycles;
+} pa_nvt_result;
+
+int pa_nvt_run(pa_state *state, const pa_guest *guest, const pa_nvt_params *params, pa_nvt_result *out);
+/* Per-production-cycle records of the last pa_nvt_run on system `sys` (each nullable, n entries):
+ * total energy (eV), and the cycle's Widom sums of exp(-dU/kT) and dU exp(-dU/kT). */
+int pa_nvt_samples(const pa_state *state, int32_t sys, int64_t n, double *energy, double *sum_w, double *sum_uw);
+
 /* Mixed-path bin edge multiplier of `batch` (no counterpart in the reference): every system's
  * spatial bin edge is kappa * max(2 Å, (64 V / insertions)^(1/3)). 0 (the initial value) lets the
  * first large enough pa_widom call time a few values of kappa and keep the fastest; a fixed

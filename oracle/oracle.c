@@ -1675,3 +1675,78 @@ void or_mc_sk(const or_mc *m, double *re, double *im)
 {
     for (int k = 0; k < m->s->nkfull; ++k) { re[k] = m->Sre[k]; im[k] = m->Sim[k]; }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * NVT driver pieces (src/nvt.jl): test insertions into a chain's current configuration and
+ * the energy audit's reset, for the serial restatement of run_nvt!.
+ * ---------------------------------------------------------------------------------------- */
+
+/* insertion_constant_term (src/nvt.jl:44-66): constant_offset at Ng = 0; otherwise
+ * constant_offset with its own 0 -> 1 tail and net-charge terms taken out and the Ng -> Ng+1
+ * ones put in. */
+double or_nvt_const_term(const or_system *s, int64_t Ng)
+{
+    if (Ng == 0) return s->constant_offset;
+    double base = s->constant_offset - or_guest_tail(s, 1) - or_net(s, 1);
+    return base + or_guest_tail(s, Ng + 1) - or_guest_tail(s, Ng) + or_net(s, Ng + 1) - or_net(s, Ng);
+}
+
+/* widom_chain_kernel! (src/nvt.jl:85-119) for one pose: host-guest real space, the reciprocal
+ * cross term 2 Re(conj(S) S_g) against the chain's RUNNING structure factor over the full k
+ * table (insertion_energy, src/energy.jl:69-80), guest-guest real space against every live
+ * guest (guest_pair_realspace_energy), plus const_term. Minimum image per site pair. */
+double or_chain_insertion(const or_mc *m, const double pos[3], const double q[4], double const_term)
+{
+    const or_system *s = m->s;
+    double ns[OR_MAXSITES][3], sj[OR_MAXSITES][3];
+    or_sites_at(s, pos, q, ns);
+    double e = or_host_guest(s, pos, q);
+    double lr = 0.0;
+    if (s->charged)
+        for (int k = 0; k < s->nkfull; ++k) {
+            double re = 0.0, im = 0.0;
+            for (int a = 0; a < s->nsites; ++a) {
+                double ph = or_dot3(s->kfull + 3 * k, ns[a]);
+                re += s->gq[a] * cos(ph);
+                im += s->gq[a] * sin(ph);
+            }
+            lr += s->kpref_full[k] * 2.0 * (m->Sre[k] * re + m->Sim[k] * im);
+        }
+    double gg = 0.0;
+    for (int j = 0; j < m->occ; ++j) {
+        or_sites_at(s, m->pos + 3 * j, m->quat + 4 * j, sj);
+        gg += or_guest_pair(s, ns, sj);
+    }
+    return e + OR_KE * lr + gg + const_term;
+}
+
+/* Test insertions j0 .. j0+n-1 of system `sysidx` into the chain as it stands, poses from the
+ * Widom stream or_pose(widom_seed, sysidx, j). */
+void or_chain_energies(const or_mc *m, uint64_t widom_seed, int64_t sysidx, int64_t j0, int64_t n, double const_term,
+                       double *dU, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t t = 0; t < n; ++t) {
+        double pos[3], q[4];
+        or_pose(widom_seed, sysidx, j0 + t, m->s->A, pos, q);
+        dU[t] = or_chain_insertion(m, pos, q, const_term);
+    }
+}
+
+/* audit_energy!'s reset on success (src/audit.jl:97-165): the running structure factor and
+ * energy become the values rebuilt from the poses, the |dU| accumulator restarts. Returns the
+ * discrepancy recomputed - running. */
+double or_mc_audit(or_mc *m)
+{
+    double E = or_mc_total_energy(m);
+    double d = E - m->energy;
+    or_mc_rebuild_sk(m, m->Sre, m->Sim);
+    m->energy = E;
+    m->energy_abs = 0.0;
+    return d;
+}

@@ -165,6 +165,14 @@ def lib():
     L.or_mc_guests.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_mc_sk.restype = None
     L.or_mc_sk.argtypes = [C.c_void_p, _dp, _dp]
+    L.or_nvt_const_term.restype = C.c_double
+    L.or_nvt_const_term.argtypes = [C.c_void_p, C.c_int64]
+    L.or_chain_insertion.restype = C.c_double
+    L.or_chain_insertion.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
+    L.or_chain_energies.restype = None
+    L.or_chain_energies.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_double, _dp, C.c_int]
+    L.or_mc_audit.restype = C.c_double
+    L.or_mc_audit.argtypes = [C.c_void_p]
     L.or_widom_grad.restype = C.c_int
     L.or_widom_grad.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp]
@@ -402,6 +410,10 @@ class OracleSystem:
                                       np.ascontiguousarray(q, dtype=np.float64), C.byref(npair))
         return e, npair.value
 
+    def nvt_const_term(self, Ng):
+        """insertion_constant_term (src/nvt.jl:44-66) at Ng guests."""
+        return lib().or_nvt_const_term(self._h, int(Ng))
+
     def insertion_energy_reference(self, pos, q):
         """src/reference.jl:31-67"""
         return lib().or_insertion_energy_reference(self._h, np.ascontiguousarray(pos, dtype=np.float64),
@@ -537,6 +549,118 @@ class OracleMC:
 
     def total_energy(self):
         return lib().or_mc_total_energy(self._h)
+
+    def audit(self):
+        """audit_energy!'s reset (src/audit.jl:97-165); returns recomputed - running."""
+        return lib().or_mc_audit(self._h)
+
+    def const_term(self):
+        """insertion_constant_term at the chain's occupancy (src/nvt.jl:44-66)."""
+        return lib().or_nvt_const_term(self.system._h, self.scalars()["occupancy"])
+
+    def chain_insertion(self, pos, q, const_term=None):
+        """widom_chain_kernel! (src/nvt.jl:85-119) at one Cartesian pose."""
+        ct = self.const_term() if const_term is None else const_term
+        return lib().or_chain_insertion(self._h, np.ascontiguousarray(pos, dtype=np.float64),
+                                        np.ascontiguousarray(q, dtype=np.float64), float(ct))
+
+    def chain_energies(self, widom_seed, sysidx, j0, n, const_term=None, nthreads=0):
+        """Test insertions j0 .. j0+n-1 of the Widom stream of system `sysidx` into the chain."""
+        ct = self.const_term() if const_term is None else const_term
+        out = np.zeros(max(int(n), 1))
+        lib().or_chain_energies(self._h, int(widom_seed) & (2**64 - 1), int(sysidx), int(j0), int(n), float(ct), out,
+                                int(nthreads))
+        return out[:int(n)]
+
+
+def block_mean_sem(x, nblocks):
+    """block_mean_sem (src/nvt.jl:126-140): contiguous blocks, the remainder folded into the last."""
+    x = np.asarray(x, dtype=np.float64)
+    ns = len(x)
+    if nblocks < 2 or ns < 2 * nblocks:
+        raise ValueError("block_mean_sem needs nblocks >= 2 and at least 2*nblocks samples")
+    bl = ns // nblocks
+    means = []
+    for b in range(nblocks):
+        lo, hi = b * bl, (ns if b == nblocks - 1 else (b + 1) * bl)
+        s = 0.0
+        for v in x[lo:hi]:
+            s += v
+        means.append(s / (hi - lo))
+    m = sum(means) / nblocks
+    v = sum((y - m) ** 2 for y in means) / (nblocks - 1)
+    return m, float(np.sqrt(v / nblocks))
+
+
+def nvt_run(chains, *, T, n_warmup, n_production, n_widom_per_cycle, n_audit, step_trans, step_rot,
+            min_cycle_length=1, widom_seed=0, nblocks=10, nslots=None):
+    """run_nvt! (src/nvt.jl:192-358) restated over OracleMC chains (one per system, system
+    index = list position): nsteps = max(max occupancy, min_cycle_length) attempts per cycle,
+    each chain drawing its move type uniformly among translation, rotation and reinsertion from
+    its own stream (or_mc_cycle with exchange off); the audit reset every n_audit cycles; after
+    every production cycle n_widom_per_cycle insertions per system from the Widom stream. Moves
+    are skipped when the batch has no guest slot (nslots = total capacity, default from the
+    chains). Returns per system a dict: energies, sW, sUW per cycle, dU (cycles x insertions),
+    acceptance and the NVTResult fields."""
+    ns = len(chains)
+    kT = KB * T
+    occ = [c.scalars()["occupancy"] for c in chains]
+    nsteps = max(max(occ), int(min_cycle_length))
+    if nslots is None:
+        nslots = sum(c.capacity for c in chains)
+    probs = (1.0, 1.0, 1.0, 0.0)
+    ct = [c.const_term() for c in chains]
+    st = np.broadcast_to(np.asarray(step_trans, dtype=np.float64), (ns,))
+    sr = np.broadcast_to(np.asarray(step_rot, dtype=np.float64), (ns,))
+    since = 0
+    base = [None] * ns
+    recs = [{"energies": [], "sW": [], "sUW": [], "dU": []} for _ in range(ns)]
+
+    def cycle():
+        nonlocal since
+        if nslots > 0:
+            for i, c in enumerate(chains):
+                c.cycle(probs, nsteps, st[i], sr[i], 1.0, T)
+        since += 1
+        if since >= n_audit:
+            for c in chains:
+                c.audit()
+            since = 0
+
+    for _ in range(n_warmup):
+        cycle()
+    base = [c.counts() for c in chains]
+    for cyc in range(n_production):
+        cycle()
+        for i, c in enumerate(chains):
+            recs[i]["energies"].append(c.scalars()["energy"])
+            dU = c.chain_energies(widom_seed, i, cyc * n_widom_per_cycle, n_widom_per_cycle, ct[i])
+            sw = suw = 0.0
+            for d in dU:
+                w, uw = boltzmann_weight(d, kT)
+                sw += w
+                suw += uw
+            recs[i]["sW"].append(sw)
+            recs[i]["sUW"].append(suw)
+            recs[i]["dU"].append(dU)
+    out = []
+    for i, c in enumerate(chains):
+        r = recs[i]
+        acc, att = c.counts()
+        acc = acc - base[i][0]
+        att = att - base[i][1]
+        V = lib_scalars(c.system)[0]
+        status, res = reduce(np.array(r["sW"]), np.array(r["sUW"]),
+                             np.full(n_production, n_widom_per_cycle, dtype=np.int64), kT, V)
+        if status != 0:
+            raise ValueError(f"system {i + 1}: _reduce status {status}")
+        e, ee = block_mean_sem(r["energies"], nblocks)
+        out.append({"energies": np.array(r["energies"]), "sW": np.array(r["sW"]), "sUW": np.array(r["sUW"]),
+                    "dU": np.array(r["dU"]), "energy": e, "energy_err": ee, "mu_ex": res[0], "mu_ex_err": res[1],
+                    "K_H": res[2], "K_H_err": res[3], "q_st": res[4], "q_st_err": res[5],
+                    "acceptance": tuple(float(acc[t]) / att[t] if att[t] else 0.0 for t in range(3)),
+                    "attempted": att, "accepted": acc})
+    return out
 
 
 def lib_scalars(system):

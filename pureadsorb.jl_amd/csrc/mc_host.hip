@@ -11,6 +11,7 @@
 
 namespace pa {
 hipError_t launch_mc(const McArgs &a, hipStream_t st);
+hipError_t launch_chain_widom(const ChainWidomArgs &a, int ngroups, int nsplit, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -27,6 +28,12 @@ struct pa_state {
     DevAlloc d_guest_off, d_occ, d_refpos, d_quat, d_host_e, d_Sk, d_dS, d_skabs, d_sk_off, d_energy, d_energy_abs,
         d_seed, d_counter, d_acc, d_att, d_step_t, d_step_r, d_fug, d_cp, d_cq, d_samp_e, d_samp_n, d_maxocc,
         d_caphit, d_trace;
+    // NVT driver: fractional atom table, insertion constant per system, move-launch snapshots,
+    // test-insertion scratch, and the per-cycle records of the last run (nsys x nvt_np)
+    DevAlloc d_atomf64, d_ct, d_snap_pos, d_snap_quat, d_snap_sk, d_snap_occ, d_item_off, d_groups, d_part,
+        d_pose_pos, d_pose_quat;
+    std::vector<double> nvt_E, nvt_sW, nvt_sUW;
+    int64_t nvt_np = 0;
 };
 
 namespace {
@@ -845,5 +852,392 @@ extern "C" int pa_gcmc_samples(const pa_state *st, int32_t sys, int64_t n, doubl
         HIP_TRY(hipMemcpy(energy, st->d_samp_e.as<double>() + (size_t)sys * have, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (loading)
         HIP_TRY(hipMemcpy(loading, st->d_samp_n.as<double>() + (size_t)sys * have, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+// ---- NVT driver with Widom insertion along the chain (src/nvt.jl) ---------------------------
+
+namespace {
+
+bool same_guest(const pa_batch *b, const pa_guest *g)
+{
+    if (g->nsites != b->nsites) return false;
+    for (int a = 0; a < b->nsites; ++a) {
+        if (g->types[a] != b->gtypes[a] || g->charges[a] != b->gq[a]) return false;
+        for (int d = 0; d < 3; ++d)
+            if (g->sites[3 * a + d] != b->gsites[3 * a + d]) return false;
+    }
+    return true;
+}
+
+// insertion_constant_term (src/nvt.jl:44-66): constant_offset at Ng = 0, else constant_offset
+// with its own 0 -> 1 tail and net-charge terms replaced by the Ng -> Ng+1 ones.
+double insertion_constant_term(const pa_batch *b, int u, int Ng)
+{
+    const double c0 = b->info[u].const_off;
+    if (Ng == 0) return c0;
+    const double base = c0 - guest_tail_h(b, u, 1.0) - net_h(b, u, 1.0);
+    const double N = (double)Ng;
+    return base + guest_tail_h(b, u, N + 1) - guest_tail_h(b, u, N) + net_h(b, u, N + 1) - net_h(b, u, N);
+}
+
+// Fractional coordinates and charge of every framework atom (the chain-insertion kernel takes
+// minimum images in fractional coordinates).
+int ensure_atomf64(pa_state *st)
+{
+    pa_batch *b = st->b;
+    if (st->d_atomf64.p) return PA_OK;
+    std::vector<double4> af(std::max<size_t>(b->atom64_h.size(), 1), make_double4(0, 0, 0, 0));
+    for (int u = 0; u < b->nfw; ++u) {
+        const FwDesc &F = b->desc[u];
+        const std::vector<double> &fr = b->fgh[u].frac;
+        if ((int)fr.size() != 3 * F.natoms) return fail(PA_ERR_INTERNAL, "framework %d: fractional table missing", u + 1);
+        for (int j = 0; j < F.natoms; ++j)
+            af[F.atom_off + j] = make_double4(fr[3 * j], fr[3 * j + 1], fr[3 * j + 2], b->atom64_h[F.atom_off + j].w);
+    }
+    HIP_TRY(upload(st->d_atomf64, af));
+    return PA_OK;
+}
+
+ChainWidomArgs base_chain_widom(pa_state *st)
+{
+    pa_batch *b = st->b;
+    ChainWidomArgs a{};
+    a.fw = b->d_fw.as<FwDesc>();
+    a.fw_of = b->d_fw_of.as<int>();
+    a.atomf64 = st->d_atomf64.as<double4>();
+    a.atype = b->d_atype.as<int>();
+    a.k64 = b->d_k64.as<double4>();
+    a.kn = b->d_kn.as<int4>();
+    a.pair64 = b->d_pair64.as<double2>();
+    a.ntypes_c = b->ntypes_c;
+    a.rc_lj2 = b->rc_lj * b->rc_lj;
+    a.rc_ew2 = b->gdev.charged ? b->rc_ew * b->rc_ew : 0.0;
+    a.guest = b->gdev;
+    a.nsys = st->nsys;
+    a.guest_off = st->d_guest_off.as<int>();
+    a.sk_off = st->d_sk_off.as<int>();
+    a.occupancy = st->d_occ.as<int>();
+    a.refpos = st->d_refpos.as<double4>();
+    a.quat = st->d_quat.as<double4>();
+    a.Sk = st->d_Sk.as<double2>();
+    a.nslots = st->guest_off[st->nsys];
+    a.nk_total = st->sk_off[st->nsys];
+    return a;
+}
+
+// Work split of one chain-insertion launch: 64 insertions of one system per workgroup; the
+// grid's second dimension splits every system's atoms, k-vectors and guests further while the
+// workgroups are too few to fill the device.
+int launch_chain(pa_state *st, ChainWidomArgs &a, const std::vector<int64_t> &item_off, int *nsplit_out)
+{
+    pa_batch *b = st->b;
+    std::vector<int2> groups;
+    int maxwork = 1;
+    for (int s = 0; s < st->nsys; ++s) {
+        for (int64_t i = item_off[s]; i < item_off[s + 1]; i += 64) groups.push_back(make_int2(s, (int)i));
+        if (item_off[s + 1] > item_off[s]) {
+            const FwDesc &F = b->desc[b->fw_of[s]];
+            maxwork = std::max(maxwork, std::max(F.natoms, F.nk));
+        }
+    }
+    const int64_t nitems = item_off[st->nsys];
+    if (nitems > INT32_MAX) return fail(PA_ERR_ARG, "%lld test insertions in one launch exceed 2^31", (long long)nitems);
+    const int ng = (int)groups.size();
+    int nsplit = 1;
+    if (ng > 0) {
+        const int want = (2048 + ng - 1) / ng, cap = std::max(1, std::min(64, maxwork / 256));
+        nsplit = std::max(1, std::min(want, cap));
+    }
+    HIP_TRY(upload(st->d_item_off, item_off));
+    HIP_TRY(upload(st->d_groups, groups));
+    HIP_TRY(st->d_part.ensure(sizeof(double) * std::max<int64_t>(1, nitems * nsplit)));
+    a.item_off = st->d_item_off.as<int64_t>();
+    a.groups = st->d_groups.as<int2>();
+    a.nitems = nitems;
+    a.part = st->d_part.as<double>();
+    HIP_TRY(launch_chain_widom(a, ng, nsplit, b->stream));
+    *nsplit_out = nsplit;
+    return PA_OK;
+}
+
+// Insertion constant of every system at its current occupancy.
+int upload_const_terms(pa_state *st, std::vector<double> &ct)
+{
+    std::vector<int> occ;
+    HIP_TRY(download(occ, st->d_occ, st->nsys));
+    ct.resize(st->nsys);
+    for (int n = 0; n < st->nsys; ++n) ct[n] = insertion_constant_term(st->b, st->b->fw_of[n], occ[n]);
+    HIP_TRY(upload(st->d_ct, ct));
+    return PA_OK;
+}
+
+}  // namespace
+
+extern "C" int pa_widom_chain_energies(pa_state *st, int32_t sys, int64_t n, const double *pos, const double *quat,
+                                       double *dU)
+{
+    if (!st) return fail(PA_ERR_ARG, "NULL state");
+    if (sys < 0 || sys >= st->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, st->nsys - 1);
+    if (n < 0) return fail(PA_ERR_ARG, "n=%lld must be >= 0", (long long)n);
+    if (n == 0) return ok();
+    if (!pos || !quat || !dU) return fail(PA_ERR_ARG, "NULL argument");
+    for (int64_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(pos[i])) return fail(PA_ERR_ARG, "pos[%lld] = %g is not finite", (long long)i, pos[i]);
+    for (int64_t i = 0; i < 4 * n; ++i)
+        if (!std::isfinite(quat[i])) return fail(PA_ERR_ARG, "quat[%lld] = %g is not finite", (long long)i, quat[i]);
+    int s0 = ensure_atomf64(st);
+    if (s0) return s0;
+    std::vector<double> ct;
+    s0 = upload_const_terms(st, ct);
+    if (s0) return s0;
+    HIP_TRY(upload(st->d_pose_pos, std::vector<double>(pos, pos + 3 * n)));
+    HIP_TRY(upload(st->d_pose_quat, std::vector<double>(quat, quat + 4 * n)));
+    ChainWidomArgs a = base_chain_widom(st);
+    a.snap_n = 0;
+    a.nw = n;
+    a.pose_pos = st->d_pose_pos.as<double>();
+    a.pose_quat = st->d_pose_quat.as<double>();
+    std::vector<int64_t> item_off(st->nsys + 1);
+    for (int s = 0; s <= st->nsys; ++s) item_off[s] = s <= sys ? 0 : n;
+    int nsplit = 1;
+    s0 = launch_chain(st, a, item_off, &nsplit);
+    if (s0) return s0;
+    HIP_TRY(hipStreamSynchronize(st->b->stream));
+    std::vector<double> part;
+    HIP_TRY(download(part, st->d_part, (size_t)nsplit * n));
+    for (int64_t i = 0; i < n; ++i) {
+        double e = 0.0;
+        for (int p = 0; p < nsplit; ++p) e += part[(size_t)p * n + i];
+        dU[i] = e + ct[sys];
+    }
+    return ok();
+}
+
+extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_params *p, pa_nvt_result *out)
+{
+    if (!st || !guest || !p || !out) return fail(PA_ERR_ARG, "NULL argument");
+    pa_batch *b = st->b;
+    const int nsys = st->nsys;
+    if (!same_guest(b, guest))
+        return fail(PA_ERR_ARG, "guest passed to run_nvt! does not match the guest FrameworkBatch/SystemState were built with");
+    if (p->n_warmup < 0) return fail(PA_ERR_ARG, "n_warmup=%lld must be >= 0", (long long)p->n_warmup);
+    if (p->n_production < 1) return fail(PA_ERR_ARG, "n_production=%lld must be >= 1", (long long)p->n_production);
+    if (p->n_widom_per_cycle < 1)
+        return fail(PA_ERR_ARG, "n_widom_per_cycle=%lld must be >= 1", (long long)p->n_widom_per_cycle);
+    if (p->n_audit < 1) return fail(PA_ERR_ARG, "n_audit=%lld must be >= 1", (long long)p->n_audit);
+    if (p->nblocks < 2) return fail(PA_ERR_ARG, "nblocks=%d: at least two blocks are needed for a standard error", p->nblocks);
+    if (p->n_production < 2 * (int64_t)p->nblocks)
+        return fail(PA_ERR_ARG, "%lld cycles: need at least 2*nblocks=%d production cycles", (long long)p->n_production,
+                    2 * p->nblocks);
+    int s0 = check_temperature(p->T);
+    if (s0) return s0;
+    s0 = upload_per_chain(st, st->d_step_t, p->step_trans, "step_trans");
+    if (s0) return s0;
+    s0 = upload_per_chain(st, st->d_step_r, p->step_rot, "step_rot");
+    if (s0) return s0;
+    s0 = ensure_atomf64(st);
+    if (s0) return s0;
+    const double kT = KB * p->T;
+    const int64_t NP = p->n_production, NW = p->n_widom_per_cycle;
+    std::vector<int> occ;
+    HIP_TRY(download(occ, st->d_occ, nsys));
+    int maxocc = 0;
+    for (int x : occ) maxocc = std::max(maxocc, x);
+    const int64_t nsteps = std::max<int64_t>(maxocc, p->min_cycle_length);
+    const bool any_guests = st->guest_off[nsys] > 0;
+    if (any_guests && nsteps < 1)
+        return fail(PA_ERR_ARG, "min_cycle_length=%lld: a cycle needs at least one attempt", (long long)p->min_cycle_length);
+    std::vector<double> ct;
+    s0 = upload_const_terms(st, ct);
+    if (s0) return s0;
+
+    McArgs a = base_mc(st);
+    a.mode = 1;
+    a.kT = kT;
+    const double pr[NMOVES] = {1.0, 1.0, 1.0, 0.0, 0.0};
+    double c = 0.0;
+    for (int t = 0; t < NMOVES; ++t) {
+        c += pr[t] / 3.0;
+        a.pcum[t] = c;
+    }
+    a.pcum[NMOVES - 1] = 2.0;
+    a.min_cycle = nsteps;
+    HIP_TRY(st->d_samp_e.ensure(sizeof(double) * nsys * NP));
+    HIP_TRY(st->d_samp_n.ensure(sizeof(double) * nsys * NP));
+    a.samp_e = st->d_samp_e.as<double>();
+    a.samp_n = st->d_samp_n.as<double>();
+    a.samp_stride = NP;
+    st->samp_stride = NP;
+    HIP_TRY(upload(st->d_maxocc, occ));
+    a.nslots = st->guest_off[nsys];
+    a.nk_total = st->sk_off[nsys];
+
+    // cycles per launch: bounded by the audit interval, by 256 MiB of snapshots and by 2^24
+    // test insertions
+    const double snap_bytes = (double)a.nslots * 2 * sizeof(double4) + (double)a.nk_total * sizeof(double2) + 4.0 * nsys;
+    int64_t seg_cap = std::max<int64_t>(1, (int64_t)(256.0 * 1024 * 1024 / snap_bytes) + 1);
+    seg_cap = std::min<int64_t>(seg_cap, std::max<int64_t>(1, (int64_t(1) << 24) / ((int64_t)nsys * NW)));
+    if (any_guests) {
+        const int64_t ns = std::max<int64_t>(1, std::min(seg_cap, NP) - 1);
+        HIP_TRY(st->d_snap_pos.ensure(sizeof(double4) * std::max(1, a.nslots) * ns));
+        HIP_TRY(st->d_snap_quat.ensure(sizeof(double4) * std::max(1, a.nslots) * ns));
+        HIP_TRY(st->d_snap_sk.ensure(sizeof(double2) * std::max(1, a.nk_total) * ns));
+        HIP_TRY(st->d_snap_occ.ensure(sizeof(int) * nsys * ns));
+        a.snap_pos = st->d_snap_pos.as<double4>();
+        a.snap_quat = st->d_snap_quat.as<double4>();
+        a.snap_sk = st->d_snap_sk.as<double2>();
+        a.snap_occ = st->d_snap_occ.as<int>();
+    }
+    st->nvt_np = 0;
+    st->nvt_E.assign((size_t)nsys * NP, 0.0);
+    st->nvt_sW.assign((size_t)nsys * NP, 0.0);
+    st->nvt_sUW.assign((size_t)nsys * NP, 0.0);
+
+    int64_t since_audit = 0;
+    std::vector<double> Ecur, part, seg_e;
+    for (int phase = 0; phase < 2; ++phase) {
+        const int64_t ncyc = phase == 0 ? p->n_warmup : NP;
+        if (phase == 1) {
+            std::vector<int> z((size_t)nsys * NMOVES, 0);
+            HIP_TRY(upload(st->d_acc, z));
+            HIP_TRY(upload(st->d_att, z));
+            st->acc_at_audit.assign(nsys, 0);
+        }
+        for (int64_t c0 = 0; c0 < ncyc;) {
+            int64_t len = std::min(ncyc - c0, p->n_audit - since_audit);
+            if (phase == 1) len = std::min(len, seg_cap);
+            if (any_guests) {
+                a.ncycles = len;
+                a.sampling = phase == 1;
+                a.samp_base = c0;
+                a.snap_n = phase == 1 ? len - 1 : 0;
+                HIP_TRY(launch_mc(a, b->stream));
+                HIP_TRY(hipStreamSynchronize(b->stream));
+            }
+            since_audit += len;
+            bool audited = false;
+            if (since_audit >= p->n_audit) {
+                int s = pa_audit_energy(st, nullptr);
+                if (s) return s;
+                since_audit = 0;
+                audited = true;
+            }
+            if (phase == 1) {
+                // energy samples of the segment (the audited energy on a cycle that ends on an audit)
+                HIP_TRY(download(Ecur, st->d_energy, nsys));
+                for (int n = 0; n < nsys; ++n) {
+                    double *dst = st->nvt_E.data() + (size_t)n * NP + c0;
+                    if (any_guests) {
+                        HIP_TRY(hipMemcpy(dst, st->d_samp_e.as<double>() + (size_t)n * NP + c0, sizeof(double) * len,
+                                          hipMemcpyDeviceToHost));
+                        if (audited) dst[len - 1] = Ecur[n];
+                    } else {
+                        for (int64_t i = 0; i < len; ++i) dst[i] = Ecur[n];
+                    }
+                }
+                // test insertions after every cycle of the segment
+                ChainWidomArgs w = base_chain_widom(st);
+                w.snap_n = any_guests ? len - 1 : 0;
+                w.snap_pos = st->d_snap_pos.as<double4>();
+                w.snap_quat = st->d_snap_quat.as<double4>();
+                w.snap_sk = st->d_snap_sk.as<double2>();
+                w.snap_occ = st->d_snap_occ.as<int>();
+                w.nw = NW;
+                w.cyc0 = c0;
+                w.widom_seed = p->widom_seed;
+                std::vector<int64_t> item_off(nsys + 1);
+                for (int s = 0; s <= nsys; ++s) item_off[s] = (int64_t)s * len * NW;
+                int nsplit = 1;
+                int s = launch_chain(st, w, item_off, &nsplit);
+                if (s) return s;
+                HIP_TRY(hipStreamSynchronize(b->stream));
+                const int64_t nit = item_off[nsys];
+                HIP_TRY(download(part, st->d_part, (size_t)nsplit * nit));
+                // boltzmann_weight (src/widom.jl:80-84) summed per system and cycle in insertion order
+                for (int n = 0; n < nsys; ++n)
+                    for (int64_t cl = 0; cl < len; ++cl) {
+                        double sw = 0.0, suw = 0.0;
+                        for (int64_t t = 0; t < NW; ++t) {
+                            const int64_t i = item_off[n] + cl * NW + t;
+                            double e = 0.0;
+                            for (int q = 0; q < nsplit; ++q) e += part[(size_t)q * nit + i];
+                            const double dU = e + ct[n];
+                            const double wgt = std::exp(-dU / kT);
+                            sw += wgt;
+                            suw += wgt == 0.0 ? 0.0 : dU * wgt;
+                        }
+                        st->nvt_sW[(size_t)n * NP + c0 + cl] = sw;
+                        st->nvt_sUW[(size_t)n * NP + c0 + cl] = suw;
+                    }
+                st->nvt_np = c0 + len;
+            }
+            c0 += len;
+        }
+    }
+    std::vector<int> acc, att;
+    HIP_TRY(download(acc, st->d_acc, (size_t)nsys * NMOVES));
+    HIP_TRY(download(att, st->d_att, (size_t)nsys * NMOVES));
+    for (int n = 0; n < nsys; ++n) {
+        pa_nvt_result &r = out[n];
+        const double *E = st->nvt_E.data() + (size_t)n * NP;
+        block_mean_sem(E, NP, p->nblocks, &r.energy, &r.energy_err);
+        // _reduce (src/widom.jl:451-492) with one block per production cycle
+        const double *sW = st->nvt_sW.data() + (size_t)n * NP, *sUW = st->nvt_sUW.data() + (size_t)n * NP;
+        double tW = 0.0, tUW = 0.0;
+        for (int64_t k = 0; k < NP; ++k) {
+            tW += sW[k];
+            tUW += sUW[k];
+        }
+        const double ntot = (double)(NP * NW), W = tW / ntot, UW = tUW / ntot;
+        if (!std::isfinite(W) || !std::isfinite(UW))
+            return fail(PA_ERR_ARG,
+                        "system %d: Boltzmann accumulation overflowed Float64 (W=%g, UW=%g); an insertion energy below "
+                        "about -709·kT was sampled",
+                        n + 1, W, UW);
+        double varW = 0.0, varUW = 0.0, cov = 0.0;
+        for (int64_t k = 0; k < NP; ++k) {
+            const double mW = sW[k] / (double)NW, mUW = sUW[k] / (double)NW;
+            varW += (mW - W) * (mW - W);
+            varUW += (mUW - UW) * (mUW - UW);
+            cov += (mW - W) * (mUW - UW);
+        }
+        varW /= (double)(NP - 1);
+        varUW /= (double)(NP - 1);
+        cov /= (double)(NP - 1);
+        const double semW = std::sqrt(varW / (double)NP);
+        const double ratio = UW / W;
+        const double var_ratio =
+            UW == 0.0 ? 0.0 : ratio * ratio * (varUW / (UW * UW) + varW / (W * W) - 2.0 * cov / (UW * W)) / (double)NP;
+        r.mu_ex = -kT * std::log(W);
+        r.q_st = kT - ratio;
+        if (!std::isfinite(r.mu_ex)) return fail(PA_ERR_ARG, "system %d: mu_ex = %g is not finite", n + 1, r.mu_ex);
+        if (!std::isfinite(r.q_st)) return fail(PA_ERR_ARG, "system %d: q_st = %g is not finite", n + 1, r.q_st);
+        const double V = b->info[b->fw_of[n]].volume;
+        r.mu_ex_err = kT * semW / W;
+        r.K_H = V * W / kT;
+        r.K_H_err = V * semW / kT;
+        r.q_st_err = std::sqrt(std::max(var_ratio, 0.0));
+        for (int t = 0; t < 3; ++t) {
+            const int at = att[n * NMOVES + t];
+            r.acceptance[t] = at == 0 ? 0.0 : (double)acc[n * NMOVES + t] / (double)at;
+        }
+        r.ncycles = NP;
+    }
+    return ok();
+}
+
+extern "C" int pa_nvt_samples(const pa_state *st, int32_t sys, int64_t n, double *energy, double *sum_w, double *sum_uw)
+{
+    if (!st) return fail(PA_ERR_ARG, "NULL state");
+    if (sys < 0 || sys >= st->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, st->nsys - 1);
+    if (n < 0 || n > st->nvt_np)
+        return fail(PA_ERR_ARG, "n=%lld production cycles requested, %lld recorded", (long long)n, (long long)st->nvt_np);
+    const size_t off = (size_t)sys * (st->nvt_E.size() / std::max(1, st->nsys));
+    for (int64_t i = 0; i < n; ++i) {
+        if (energy) energy[i] = st->nvt_E[off + i];
+        if (sum_w) sum_w[i] = st->nvt_sW[off + i];
+        if (sum_uw) sum_uw[i] = st->nvt_sUW[off + i];
+    }
     return ok();
 }

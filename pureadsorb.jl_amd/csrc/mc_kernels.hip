@@ -110,6 +110,10 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
     // sample is taken at its end. Called before every attempt; false once the run is over.
     int64_t c = 0, m = 0, len = 0;
     bool finished = false;
+    // cycles whose end-of-cycle snapshot is still to be written: a boundary only records the
+    // cycle, and the copy runs before the next commit (the state cannot change before then)
+    int64_t pend[MC_SPEC + 1];
+    int npend = 0;
     auto cycle_boundary = [&]() -> bool {
         while (m == len) {
             if (c > 0) {
@@ -118,6 +122,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                     a.samp_e[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = E;
                     a.samp_n[(int64_t)n * a.samp_stride + a.samp_base + c - 1] = (double)occ;
                 }
+                if (c - 1 < a.snap_n) pend[npend++] = c - 1;
             }
             if (c == a.ncycles) return false;
             if (occ > maxocc) maxocc = occ;
@@ -126,6 +131,24 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             ++c;
         }
         return true;
+    };
+
+    // writes the pending snapshots: every work-item copies the structure-factor entries it
+    // updates itself on a commit (k = tid mod MC_WG) and a share of the guests, which work-item 0
+    // last wrote before a barrier
+    auto flush_snapshots = [&]() {
+        for (int p = 0; p < npend; ++p) {
+            const int64_t cs = pend[p];
+            double2 *dsk = a.snap_sk + cs * a.nk_total + a.sk_off[n];
+            for (int k = tid; k < nk; k += MC_WG) dsk[k] = Sk[k];
+            double4 *dp = a.snap_pos + cs * a.nslots + gr0, *dq = a.snap_quat + cs * a.nslots + gr0;
+            for (int i = tid; i < occ; i += MC_WG) {
+                dp[i] = a.refpos[gr0 + i];
+                dq[i] = a.quat[gr0 + i];
+            }
+            if (tid == 0) a.snap_occ[cs * a.nsys + n] = occ;
+        }
+        npend = 0;
     };
 
     // processes up to nv attempts of types tys[0..nv) in order (cycle bookkeeping between
@@ -315,6 +338,10 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                 break;
             }
         }
+        if (npend > 0) {
+            flush_snapshots();
+            __syncthreads();  // the commit below overwrites guests the copy read
+        }
         if (first >= 0) {
             const int ty = tys[first], sl = sp_slot[first];
             if (tid == 0) {
@@ -383,6 +410,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
         round(nv, tys);
         if (finished) break;
     }
+    flush_snapshots();
     if (tid == 0) {
         a.occupancy[n] = occ;
         a.energy[n] = E;

@@ -242,6 +242,48 @@ struct McArgs {
     int *capacity_hits;         // per chain: accepted-by-Metropolis insertions refused at capacity
     double4 *trace;             // optional, per chain trace_cap entries: type, slot, dU, accepted
     int64_t trace_cap;
+    // mode 1 snapshots: the state at the end of cycle c (0-based within the launch) for c < snap_n,
+    // guests at snap_pos/snap_quat[c * nslots + slot], structure factor at
+    // snap_sk[c * nk_total + sk_off[n] + k], occupancy at snap_occ[c * nsys + n]
+    int64_t snap_n;
+    int nslots, nk_total;
+    double4 *snap_pos, *snap_quat;
+    double2 *snap_sk;
+    int *snap_occ;
+};
+
+// Test insertions into the chains' current configurations (nvt_kernels.hip). Insertion i of
+// system s lies in [item_off[s], item_off[s+1]); local index l = i - item_off[s] is cycle slot
+// l / nw, insertion l % nw of that cycle. Its pose is pose_pos/pose_quat[i] when given, else the
+// Widom stream's pose of per-system index (cyc0 + slot) * nw + l % nw. Slot c < snap_n sees the
+// mode-1 snapshot c of the move launch before it, later slots the live state.
+struct ChainWidomArgs {
+    const FwDesc *fw;
+    const int *fw_of;
+    const double4 *atomf64;     // fractional x, y, z and charge per atom
+    const int *atype;
+    const double4 *k64;         // kx, ky, kz, kprefactor (every k-vector)
+    const int4 *kn;
+    const double2 *pair64;
+    int ntypes_c;
+    double rc_lj2, rc_ew2;
+    GuestDev guest;
+    int nsys;
+    const int *guest_off, *sk_off;
+    const int *occupancy;
+    const double4 *refpos, *quat;
+    const double2 *Sk;
+    int64_t snap_n;
+    int nslots, nk_total;
+    const double4 *snap_pos, *snap_quat;
+    const double2 *snap_sk;
+    const int *snap_occ;
+    const int64_t *item_off;    // nsys + 1
+    const int2 *groups;         // per workgroup: system, first insertion (64 per group)
+    int64_t nitems, nw, cyc0;
+    uint64_t widom_seed;
+    const double *pose_pos, *pose_quat;
+    double *part;               // gridDim.y x nitems partial energies
 };
 
 }  // namespace pa

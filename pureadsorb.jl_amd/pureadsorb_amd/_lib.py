@@ -31,6 +31,7 @@ EXPORTS = (
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
     "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
+    "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples",
 )
 
 
@@ -124,6 +125,22 @@ class PaGcmcResult(C.Structure):
     ]
 
 
+class PaNvtParams(C.Structure):
+    _fields_ = [
+        ("T", C.c_double), ("n_warmup", C.c_int64), ("n_production", C.c_int64), ("n_widom_per_cycle", C.c_int64),
+        ("n_audit", C.c_int64), ("min_cycle_length", C.c_int64), ("step_trans", C.c_void_p), ("step_rot", C.c_void_p),
+        ("widom_seed", C.c_uint64), ("nblocks", C.c_int32),
+    ]
+
+
+class PaNvtResult(C.Structure):
+    _fields_ = [
+        ("energy", C.c_double), ("energy_err", C.c_double), ("mu_ex", C.c_double), ("mu_ex_err", C.c_double),
+        ("K_H", C.c_double), ("K_H_err", C.c_double), ("q_st", C.c_double), ("q_st_err", C.c_double),
+        ("acceptance", C.c_double * 3), ("ncycles", C.c_int64),
+    ]
+
+
 class PaTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
@@ -193,6 +210,12 @@ def lib():
     L.pa_gcmc_run.argtypes = [vp, C.POINTER(PaGcmcParams), vp]
     L.pa_gcmc_samples.restype = C.c_int
     L.pa_gcmc_samples.argtypes = [vp, C.c_int32, C.c_int64, vp, vp]
+    L.pa_widom_chain_energies.restype = C.c_int
+    L.pa_widom_chain_energies.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp]
+    L.pa_nvt_run.restype = C.c_int
+    L.pa_nvt_run.argtypes = [vp, C.POINTER(PaGuest), C.POINTER(PaNvtParams), vp]
+    L.pa_nvt_samples.restype = C.c_int
+    L.pa_nvt_samples.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp]
     L.pa_hardcore_flags.restype = C.c_int
     L.pa_hardcore_flags.argtypes = [vp, C.c_int32, C.c_uint64, C.c_double, C.c_int64, C.c_int64, vp]
     L.pa_widom_energies.restype = C.c_int

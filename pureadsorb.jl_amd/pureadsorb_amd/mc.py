@@ -1,6 +1,7 @@
 """The move-energy path on the MI355X backend: `SystemState` (src/state.jl:89-446), the NVT
 and μVT moves (`mc_step!`, `mc_insert!`, `mc_delete!`, `mc_exchange!`, src/moves.jl), the
-energy audit (`audit_energy!`, src/audit.jl) and the GCMC driver (`run_gcmc!`, src/gcmc.jl).
+energy audit (`audit_energy!`, src/audit.jl), the GCMC driver (`run_gcmc!`, src/gcmc.jl) and the
+NVT driver with Widom insertion along the chain (`run_nvt!`, `widom_chain_kernel!`, src/nvt.jl).
 
 Every chain lives on the device; the moves run in FP64 in one launch per call."""
 from __future__ import annotations
@@ -195,3 +196,71 @@ def gcmc_samples(state: SystemState, n: int, ncycles: int):
     e, l = np.zeros(max(ncycles, 1)), np.zeros(max(ncycles, 1))
     _lib.check(_lib.lib().pa_gcmc_samples(state.handle, int(n), int(ncycles), e.ctypes.data, l.ctypes.data))
     return e[:ncycles], l[:ncycles]
+
+
+def widom_chain_energies(state: SystemState, n: int, pos, quat) -> np.ndarray:
+    """`widom_chain_kernel!` (src/nvt.jl:85-119) for system n (0-based): Delta U (eV) of test
+    insertions at the given poses (Cartesian positions in Å, quaternions x y z w) into the
+    system as it currently stands, including `insertion_constant_term` at its occupancy. The
+    state is not changed."""
+    p = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    q = np.ascontiguousarray(np.asarray(quat, dtype=np.float64).reshape(-1, 4))
+    if len(p) != len(q):
+        raise _lib.DimensionMismatch(f"{len(p)} positions but {len(q)} orientations")
+    out = np.zeros(max(len(p), 1))
+    _lib.check(_lib.lib().pa_widom_chain_energies(state.handle, int(n), len(p), p.ctypes.data, q.ctypes.data,
+                                                  out.ctypes.data))
+    return out[:len(p)]
+
+
+@dataclass(frozen=True)
+class NVTResult:
+    """`NVTResult` (src/nvt.jl:22-33): production energy and its cycle-block error, the Widom
+    estimators from insertions into the chain's configuration (each production cycle one block),
+    the production acceptance rate of translation, rotation and reinsertion."""
+    energy: float
+    energy_err: float
+    mu_ex: float
+    mu_ex_err: float
+    K_H: float
+    K_H_err: float
+    q_st: float
+    q_st_err: float
+    acceptance: tuple
+    ncycles: int
+
+
+def run_nvt(batch: FrameworkBatch, state: SystemState, guest: Guest, ff=None, *, T, n_warmup: int,
+            n_production: int, n_widom_per_cycle: int, n_audit: int, step_trans, step_rot,
+            min_cycle_length: int = 1, widom_seed: int = 0, nblocks: int = 10) -> list[NVTResult]:
+    """`run_nvt!(batch, state, guest, ff; T, n_warmup, n_production, n_widom_per_cycle, n_audit,
+    step_trans, step_rot, min_cycle_length, widom_seed, nblocks)` (src/nvt.jl:192-358).
+
+    A cycle is max(max occupancy, min_cycle_length) attempts on every chain, each a translation,
+    rotation or reinsertion drawn uniformly from the chain's own counter-based stream (the
+    reference's host `Xoshiro(seed)` move-type draw has no counterpart, so there is no `seed`
+    argument); after every production cycle, n_widom_per_cycle test insertions per system whose
+    poses are pa_widom's poses under `widom_seed`. `ff` is accepted for signature parity: the
+    batch already holds the force field."""
+    if state.batch is not batch:
+        raise _lib.ArgumentError("state was not built on this batch")
+    n = state.nsys
+    g = _GuestArgs(guest)
+    st = _vec(step_trans, n, "step_trans")
+    sr = _vec(step_rot, n, "step_rot")
+    p = _lib.PaNvtParams(_T(T), int(n_warmup), int(n_production), int(n_widom_per_cycle), int(n_audit),
+                         int(min_cycle_length), st.ctypes.data, sr.ctypes.data, int(widom_seed) & (2**64 - 1),
+                         int(nblocks))
+    out = (_lib.PaNvtResult * n)()
+    _lib.check(_lib.lib().pa_nvt_run(state.handle, C.byref(g.s), C.byref(p), out))
+    return [NVTResult(r.energy, r.energy_err, r.mu_ex, r.mu_ex_err, r.K_H, r.K_H_err, r.q_st, r.q_st_err,
+                      tuple(float(x) for x in r.acceptance), int(r.ncycles)) for r in out]
+
+
+def nvt_samples(state: SystemState, n: int, ncycles: int):
+    """Per-production-cycle (energy, sum of exp(-dU/kT), sum of dU exp(-dU/kT)) of system n from
+    the last run_nvt."""
+    e, w, uw = np.zeros(max(ncycles, 1)), np.zeros(max(ncycles, 1)), np.zeros(max(ncycles, 1))
+    _lib.check(_lib.lib().pa_nvt_samples(state.handle, int(n), int(ncycles), e.ctypes.data, w.ctypes.data,
+                                         uw.ctypes.data))
+    return e[:ncycles], w[:ncycles], uw[:ncycles]
