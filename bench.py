@@ -35,6 +35,7 @@ from pureadsorb_amd import synth  # noqa: E402
 
 METRIC = "Widom insertions/s on 1 GPU, mixed vs FP64 at |d lnK_H|<=1e-4; grad cost"
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak
 PEAK_HBM_GBS = 8000.0
 T_K = 298.15
 
@@ -133,14 +134,20 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000,
                     help="insertions timed on the CPU oracle, two-phase, all host threads (0: skip)")
     ap.add_argument("--no-parity", action="store_true", help="skip the porous-variant parity line")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rubtak"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rubtak", "nvt"),
+                    default="config1",
                     help="config1: the 8,100-atom CO2 case (the metric); config2: 256 frameworks x 1e5 insertions; "
                          "config3: gradient cost on 64 frameworks x 2e5 insertions; config4: GCMC move path; "
-                         "rubtak: the reference's own benchmark system (RUBTAK 3x3x3 + CO2, TraPPE, 12 A)")
+                         "rubtak: the reference's own benchmark system (RUBTAK 3x3x3 + CO2, TraPPE, 12 A); "
+                         "nvt: NVT move cost and Widom insertion along the chain (RUBTAK 3x3x3 + 50 CO2)")
     ap.add_argument("--bin-scale", type=float, default=0.0,
                     help="mixed-path bin edge multiplier (0: the library tunes it on the first call)")
     ap.add_argument("--gcmc-warmup", type=int, default=20000)
     ap.add_argument("--gcmc-production", type=int, default=50000)
+    ap.add_argument("--nvt-moves", type=int, default=20000, help="nvt: translation attempts per chain count, in total")
+    ap.add_argument("--nvt-widom", type=int, default=256, help="nvt: test insertions per chain per cycle")
+    ap.add_argument("--nvt-cycles", type=int, default=20, help="nvt: production cycles per timed run_nvt")
+    ap.add_argument("--nvt-cpu-sample", type=int, default=20000, help="nvt: test insertions timed on the CPU oracle")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -169,6 +176,8 @@ def main():
         return bench_config4(args)
     if args.workload == "rubtak":
         return bench_rubtak(args)
+    if args.workload == "nvt":
+        return bench_nvt(args)
     fw, ff, g, ew = workload()
     t0 = time.perf_counter()
     b = pa.FrameworkBatch([fw], ff, g, ew)
@@ -335,6 +344,101 @@ def bench_rubtak(args):
            "ln_K_H_mixed": rm.ln_K_H, "ln_K_H_f64": r64.ln_K_H, "q_st_mixed_eV": rm.q_st, "q_st_f64_eV": r64.q_st}
     log(f"rubtak: mixed {out['value']:.4e} ins/s, f64 {out['f64_plain']['value']:.4e} ins/s, "
         f"d ln K_H {abs(rm.ln_K_H - r64.ln_K_H):.2e}")
+    print(json.dumps(out), flush=True)
+
+
+def bench_nvt(args):
+    """The reference's NVT benchmark system (RUBTAK 3x3x3, 3,078 atoms, + 50 CO2, TraPPE, LJ 12 A,
+    Ewald 12 A / 1e-6, 298.15 K, FP64; docs/src/benchmarks.md:55-75): translation-move cost per
+    attempt at 1, 64, 256 and 1,024 chains (the reference times one mc_step! launch per attempt,
+    this library runs every chain's attempts back to back in one launch), then run_nvt with
+    Widom insertion along the chain on 64 chains. `value` is the test insertions/s of the
+    chain-insertion kernel (HIP events); its roofline prices FP64 FLOPs (29 per (site, atom)
+    distance test, 88 per in-cutoff charged (site, atom) pair, 10 per (k-vector, site) and 5 per
+    k-vector, 29 per guest-guest site-pair test) against the FP64 peak."""
+    import oracle as O
+
+    golden = os.path.join(ROOT, "tests", "golden")
+    ff = pa.read_forcefield(os.path.join(golden, "trappe.yaml"))
+    g = pa.read_guest(os.path.join(golden, "co2.yaml"), ff)
+    fw = pa.replicate(pa.read_cif(os.path.join(golden, "RUBTAK.cif")), (3, 3, 3))
+    ew = pa.EwaldParams(12.0, 1e-6)
+    ref_us = {1: 187.2, 64: 18.2, 256: 15.8, 1024: 10.8}  # RTX 4070, FP64, docs/src/benchmarks.md:64-68
+    moves = []
+    for nsys in (1, 64, 256, 1024):
+        b = pa.FrameworkBatch([fw] * nsys, ff, g, ew, fullk=True)
+        t0 = time.perf_counter()
+        st = pa.SystemState(b, g, [50] * nsys, T=T_K, seed=1)
+        t_state = time.perf_counter() - t0
+        nm = max(200, args.nvt_moves // nsys)
+        pa.mc_moves(st, pa.MOVE_TRANSLATION, 10, T=T_K, step_trans=0.3)
+        pa.synchronize()
+        t0 = time.perf_counter()
+        pa.mc_moves(st, pa.MOVE_TRANSLATION, nm, T=T_K, step_trans=0.3)
+        pa.synchronize()
+        wall = time.perf_counter() - t0
+        acc = sum(st.info(n)["accepted"][0] for n in range(nsys)) / (nsys * (nm + 10))
+        moves.append({"chains": nsys, "attempts_per_chain": nm, "us_per_move": wall / (nm * nsys) * 1e6,
+                      "wall_s": wall, "translation_acceptance": acc, "state_build_s": t_state,
+                      "reference_rtx4070_f64_us_per_move": ref_us[nsys]})
+        log(f"nvt moves: {nsys} chains, {nm} attempts each: {wall / (nm * nsys) * 1e6:.3f} us/move "
+            f"(reference RTX 4070 FP64 {ref_us[nsys]} us), acceptance {acc:.3f}")
+        st.close()
+        b.close()
+    # run_nvt with Widom insertion along the chain
+    nsys, nw = 64, args.nvt_widom
+    b = pa.FrameworkBatch([fw] * nsys, ff, g, ew, fullk=True)
+    st = pa.SystemState(b, g, [50] * nsys, T=T_K, seed=3)
+    kw = dict(T=T_K, n_warmup=2, n_widom_per_cycle=nw, n_audit=1_000_000, step_trans=0.3, step_rot=0.3,
+              widom_seed=1003, nblocks=5)
+    pa.run_nvt(b, st, g, ff, n_production=10, **kw)
+    ins_ms, ins_n, mv_ms, tot_ms, ncyc = 0.0, 0, 0.0, 0.0, 0
+    res = None
+    for _ in range(args.steps):
+        res = pa.run_nvt(b, st, g, ff, n_production=args.nvt_cycles, **kw)
+        t = pa.nvt_last_timing(st)
+        ins_ms += t["insertion_ms"]
+        ins_n += t["insertions"]
+        mv_ms += t["move_ms"]
+        tot_ms += t["total_ms"]
+        ncyc += args.nvt_cycles
+    ins_s = ins_n / (ins_ms * 1e-3)
+    # algorithmic FP64 FLOPs per test insertion; in-cutoff (site, atom) pairs from the oracle on a
+    # sample of the same insertion stream (system 0)
+    types = [pa.typeindex(ff, s_ + "_") for s_ in fw.symbols]
+    osys = O.OracleSystem(fw.cell, fw.frac, types, fw.charges, fw.replication, ff.sigma, ff.epsilon, ff.cutoff,
+                          ff.tail, g.sites, g.types, g.charges, ew.cutoff, ew.precision)
+    _, npairs = osys.energies(1003, 0, 0, 2000, want_pairs=True, nthreads=16)
+    p_in = float(np.mean(npairs))
+    ns_, na, nk = len(g.charges), fw.natoms, int(b.nk[0])
+    flops = 29.0 * ns_ * na + 88.0 * p_in + (10.0 * ns_ + 5.0) * nk + 29.0 * 50 * ns_ * ns_
+    achieved = flops * ins_s / 1e12
+    # CPU baseline: the oracle's chain insertion (FP64 port) on the host cores, same workload
+    chain = O.OracleMC(osys, 50, 1, *st.guests(0)[:2])
+    ncpu = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    chain.chain_energies(1003, 0, 0, args.nvt_cpu_sample, nthreads=ncpu)
+    cpu_s = time.perf_counter() - t0
+    out = {"metric": "NVT test insertions/s into loaded configurations on 1 GPU (FP64)", "value": ins_s,
+           "unit": "insertions/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": tot_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "RUBTAK.cif (tests/golden), replicated 3x3x3; 50 CO2 per chain",
+           "config": {"workload": f"run_nvt on {nsys} chains of RUBTAK 3x3x3 + 50 CO2, TraPPE, LJ 12 A, Ewald 12 A "
+                                  f"/ 1e-6, 298.15 K, {args.nvt_cycles} production cycles of 50 attempts, {nw} test "
+                                  "insertions per chain per cycle", "chains": nsys, "n_widom_per_cycle": nw},
+           "roofline": {"bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                        "flops_per_insertion": flops, "in_cutoff_pairs_per_insertion": p_in, "k_vectors": nk},
+           "cpu_baseline": {"value": args.nvt_cpu_sample / cpu_s, "unit": "insertions/s", "cores": ncpu,
+                            "kind": "port", "sample": f"{args.nvt_cpu_sample} test insertions into chain 0's initial "
+                                                     f"50-guest configuration, FP64 oracle, {cpu_s:.1f} s"},
+           "move_cost": moves,
+           "nvt_run": {"cycles": ncyc, "move_kernel_ms": mv_ms, "insertion_kernel_ms": ins_ms, "wall_ms": tot_ms,
+                       "moves_per_s": ncyc * 50 * nsys / (mv_ms * 1e-3),
+                       "acceptance": list(res[0].acceptance), "mu_ex_eV": res[0].mu_ex, "q_st_eV": res[0].q_st,
+                       "energy_eV": res[0].energy}}
+    log(f"nvt: {ins_s:.4e} test insertions/s ({achieved:.2f} TFLOP/s FP64, {achieved / PEAK_FP64_TFLOPS:.1%}), "
+        f"moves {ncyc * 50 * nsys / (mv_ms * 1e-3):.3e}/s, cpu {args.nvt_cpu_sample / cpu_s:.3e}/s")
     print(json.dumps(out), flush=True)
 
 

@@ -298,6 +298,20 @@ int pa_nvt_run(pa_state *state, const pa_guest *guest, const pa_nvt_params *para
  * total energy (eV), and the cycle's Widom sums of exp(-dU/kT) and dU exp(-dU/kT). */
 int pa_nvt_samples(const pa_state *state, int32_t sys, int64_t n, double *energy, double *sum_w, double *sum_uw);
 
+/* Timing of the last pa_nvt_run or pa_widom_chain_energies on a state: device time of the move
+ * launches and of the test-insertion launches (HIP events on the library's stream), and the
+ * host wall time of the whole pa_nvt_run (audits, copies and the host reductions included). */
+typedef struct pa_nvt_timing {
+    double move_ms;
+    int64_t move_launches;
+    double insertion_ms;
+    int64_t insertion_launches;
+    int64_t insertions;
+    double total_ms;
+} pa_nvt_timing;
+
+int pa_nvt_last_timing(const pa_state *state, pa_nvt_timing *out);
+
 /* Mixed-path bin edge multiplier of `batch` (no counterpart in the reference): every system's
  * spatial bin edge is kappa * max(2 Å, (64 V / insertions)^(1/3)). 0 (the initial value) lets the
  * first large enough pa_widom call time a few values of kappa and keep the fastest; a fixed

@@ -2,6 +2,7 @@
 // points, the FP64 from-scratch energy and audit, and the GCMC cycle driver (C ABI in
 // include/pureadsorb_amd.h).
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -34,6 +35,14 @@ struct pa_state {
         d_pose_pos, d_pose_quat;
     std::vector<double> nvt_E, nvt_sW, nvt_sUW;
     int64_t nvt_np = 0;
+    // device time of the last pa_nvt_run / pa_widom_chain_energies, from events around each launch
+    hipEvent_t tev[2] = {nullptr, nullptr};
+    pa_nvt_timing timing{};
+    ~pa_state()
+    {
+        for (auto &e : tev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -961,6 +970,25 @@ int launch_chain(pa_state *st, ChainWidomArgs &a, const std::vector<int64_t> &it
     return PA_OK;
 }
 
+// Runs `launch` between two events on the library's stream, waits for it and adds its duration.
+template <class F>
+int timed(pa_state *st, double *ms, int64_t *count, F launch)
+{
+    hipStream_t s = st->b->stream;
+    for (auto &e : st->tev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(st->tev[0], s));
+    int r = launch();
+    if (r) return r;
+    HIP_TRY(hipEventRecord(st->tev[1], s));
+    HIP_TRY(hipEventSynchronize(st->tev[1]));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, st->tev[0], st->tev[1]));
+    *ms += t;
+    ++*count;
+    return PA_OK;
+}
+
 // Insertion constant of every system at its current occupancy.
 int upload_const_terms(pa_state *st, std::vector<double> &ct)
 {
@@ -1001,9 +1029,11 @@ extern "C" int pa_widom_chain_energies(pa_state *st, int32_t sys, int64_t n, con
     std::vector<int64_t> item_off(st->nsys + 1);
     for (int s = 0; s <= st->nsys; ++s) item_off[s] = s <= sys ? 0 : n;
     int nsplit = 1;
-    s0 = launch_chain(st, a, item_off, &nsplit);
+    st->timing = pa_nvt_timing{};
+    s0 = timed(st, &st->timing.insertion_ms, &st->timing.insertion_launches,
+               [&] { return launch_chain(st, a, item_off, &nsplit); });
     if (s0) return s0;
-    HIP_TRY(hipStreamSynchronize(st->b->stream));
+    st->timing.insertions = n;
     std::vector<double> part;
     HIP_TRY(download(part, st->d_part, (size_t)nsplit * n));
     for (int64_t i = 0; i < n; ++i) {
@@ -1096,6 +1126,8 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
 
     int64_t since_audit = 0;
     std::vector<double> Ecur, part, seg_e;
+    st->timing = pa_nvt_timing{};
+    const auto t_start = std::chrono::steady_clock::now();
     for (int phase = 0; phase < 2; ++phase) {
         const int64_t ncyc = phase == 0 ? p->n_warmup : NP;
         if (phase == 1) {
@@ -1112,8 +1144,11 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
                 a.sampling = phase == 1;
                 a.samp_base = c0;
                 a.snap_n = phase == 1 ? len - 1 : 0;
-                HIP_TRY(launch_mc(a, b->stream));
-                HIP_TRY(hipStreamSynchronize(b->stream));
+                int s = timed(st, &st->timing.move_ms, &st->timing.move_launches, [&]() -> int {
+                    HIP_TRY(launch_mc(a, b->stream));
+                    return PA_OK;
+                });
+                if (s) return s;
             }
             since_audit += len;
             bool audited = false;
@@ -1149,10 +1184,11 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
                 std::vector<int64_t> item_off(nsys + 1);
                 for (int s = 0; s <= nsys; ++s) item_off[s] = (int64_t)s * len * NW;
                 int nsplit = 1;
-                int s = launch_chain(st, w, item_off, &nsplit);
+                int s = timed(st, &st->timing.insertion_ms, &st->timing.insertion_launches,
+                              [&] { return launch_chain(st, w, item_off, &nsplit); });
                 if (s) return s;
-                HIP_TRY(hipStreamSynchronize(b->stream));
                 const int64_t nit = item_off[nsys];
+                st->timing.insertions += nit;
                 HIP_TRY(download(part, st->d_part, (size_t)nsplit * nit));
                 // boltzmann_weight (src/widom.jl:80-84) summed per system and cycle in insertion order
                 for (int n = 0; n < nsys; ++n)
@@ -1175,6 +1211,8 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
             c0 += len;
         }
     }
+    st->timing.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     std::vector<int> acc, att;
     HIP_TRY(download(acc, st->d_acc, (size_t)nsys * NMOVES));
     HIP_TRY(download(att, st->d_att, (size_t)nsys * NMOVES));
@@ -1239,5 +1277,12 @@ extern "C" int pa_nvt_samples(const pa_state *st, int32_t sys, int64_t n, double
         if (sum_w) sum_w[i] = st->nvt_sW[off + i];
         if (sum_uw) sum_uw[i] = st->nvt_sUW[off + i];
     }
+    return ok();
+}
+
+extern "C" int pa_nvt_last_timing(const pa_state *st, pa_nvt_timing *out)
+{
+    if (!st || !out) return fail(PA_ERR_ARG, "NULL argument");
+    *out = st->timing;
     return ok();
 }

@@ -10,7 +10,7 @@ from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
 from .mc import (MOVE_DELETION, MOVE_INSERTION, MOVE_REINSERTION, MOVE_ROTATION, MOVE_TRANSLATION, GCMCResult,
                  NVTResult, SystemState, audit_energy, gcmc_samples, mc_delete, mc_exchange, mc_insert, mc_moves, mc_step,
-                 nvt_samples, run_gcmc, run_nvt, total_energy, widom_chain_energies)
+                 nvt_last_timing, nvt_samples, run_gcmc, run_nvt, total_energy, widom_chain_energies)
 from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
                         reciprocal_basis, replicate, total_charge, volume)
 from .widom import (KB, KE, PASCAL, WidomGrad, WidomResult, default_run, hardcore_flags, last_timing, sys_of_index,
@@ -24,5 +24,5 @@ __all__ = [
     "sys_of_index", "system_counts", "widom", "widom_energies", "widom_grad", "widom_poses", "WidomGrad", "hardcore_flags",
     "MOVE_TRANSLATION", "MOVE_ROTATION", "MOVE_REINSERTION", "MOVE_INSERTION", "MOVE_DELETION", "SystemState",
     "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "run_gcmc",
-    "GCMCResult", "gcmc_samples", "NVTResult", "run_nvt", "nvt_samples", "widom_chain_energies",
+    "GCMCResult", "gcmc_samples", "NVTResult", "run_nvt", "nvt_samples", "widom_chain_energies", "nvt_last_timing",
 ]

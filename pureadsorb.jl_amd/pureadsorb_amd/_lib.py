@@ -31,7 +31,7 @@ EXPORTS = (
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
     "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
-    "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples",
+    "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples", "pa_nvt_last_timing",
 )
 
 
@@ -141,6 +141,13 @@ class PaNvtResult(C.Structure):
     ]
 
 
+class PaNvtTiming(C.Structure):
+    _fields_ = [
+        ("move_ms", C.c_double), ("move_launches", C.c_int64), ("insertion_ms", C.c_double),
+        ("insertion_launches", C.c_int64), ("insertions", C.c_int64), ("total_ms", C.c_double),
+    ]
+
+
 class PaTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("energy_ms", C.c_double), ("energy_launches", C.c_int64),
@@ -216,6 +223,8 @@ def lib():
     L.pa_nvt_run.argtypes = [vp, C.POINTER(PaGuest), C.POINTER(PaNvtParams), vp]
     L.pa_nvt_samples.restype = C.c_int
     L.pa_nvt_samples.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp]
+    L.pa_nvt_last_timing.restype = C.c_int
+    L.pa_nvt_last_timing.argtypes = [vp, C.POINTER(PaNvtTiming)]
     L.pa_hardcore_flags.restype = C.c_int
     L.pa_hardcore_flags.argtypes = [vp, C.c_int32, C.c_uint64, C.c_double, C.c_int64, C.c_int64, vp]
     L.pa_widom_energies.restype = C.c_int

@@ -264,3 +264,11 @@ def nvt_samples(state: SystemState, n: int, ncycles: int):
     _lib.check(_lib.lib().pa_nvt_samples(state.handle, int(n), int(ncycles), e.ctypes.data, w.ctypes.data,
                                          uw.ctypes.data))
     return e[:ncycles], w[:ncycles], uw[:ncycles]
+
+
+def nvt_last_timing(state: SystemState) -> dict:
+    """Device time of the move and test-insertion launches of the last run_nvt or
+    widom_chain_energies on `state`, and the host wall time of the last run_nvt."""
+    t = _lib.PaNvtTiming()
+    _lib.check(_lib.lib().pa_nvt_last_timing(state.handle, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in t._fields_}
