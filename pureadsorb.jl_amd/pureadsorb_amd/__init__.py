@@ -8,6 +8,8 @@ libpureadsorb_amd.so (HIP kernels for gfx950) through its C ABI; there is no CPU
 from ._lib import ArgumentError, DeviceError, DimensionMismatch, device_name, synchronize
 from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
+from .fugacity import Fugacity, cubic_roots, peng_robinson_fugacity
+from .isotherm import IsothermResult, combine_replicas, run_isotherm
 from .mc import (MOVE_DELETION, MOVE_INSERTION, MOVE_REINSERTION, MOVE_ROTATION, MOVE_TRANSLATION, GCMCResult,
                  NVTResult, SystemState, audit_energy, gcmc_samples, mc_delete, mc_exchange, mc_insert, mc_moves, mc_step,
                  nvt_last_timing, nvt_samples, run_gcmc, run_nvt, total_energy, widom_chain_energies)
@@ -25,4 +27,5 @@ __all__ = [
     "MOVE_TRANSLATION", "MOVE_ROTATION", "MOVE_REINSERTION", "MOVE_INSERTION", "MOVE_DELETION", "SystemState",
     "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "run_gcmc",
     "GCMCResult", "gcmc_samples", "NVTResult", "run_nvt", "nvt_samples", "widom_chain_energies", "nvt_last_timing",
+    "Fugacity", "cubic_roots", "peng_robinson_fugacity", "IsothermResult", "combine_replicas", "run_isotherm",
 ]
