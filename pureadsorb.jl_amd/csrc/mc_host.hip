@@ -12,7 +12,8 @@
 
 namespace pa {
 hipError_t launch_mc(const McArgs &a, hipStream_t st);
-hipError_t launch_chain_widom(const ChainWidomArgs &a, int ngroups, int nsplit, hipStream_t st);
+hipError_t launch_chain_widom(const ChainWidomArgs &a, hipStream_t st);
+hipError_t launch_chain_snapshot(const ChainSnap &S, int64_t slot, const McArgs &m, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -31,8 +32,12 @@ struct pa_state {
         d_caphit, d_trace;
     // NVT driver: fractional atom table, insertion constant per system, move-launch snapshots,
     // test-insertion scratch, and the per-cycle records of the last run (nsys x nvt_np)
-    DevAlloc d_atomf64, d_ct, d_snap_pos, d_snap_quat, d_snap_sk, d_snap_occ, d_item_off, d_groups, d_part,
-        d_pose_pos, d_pose_quat;
+    DevAlloc d_atomf64, d_atype_s, d_chunk, d_chunk_off, d_lines, d_rounds, d_round_off, d_perm, d_coff,
+        d_half_perp, d_ct, d_snap_pos, d_snap_quat, d_snap_sk, d_snap_occ, d_item_off, d_cw_dU, d_pose_pos,
+        d_pose_quat;
+    double cull_extra = 0.0;
+    int64_t ctot = 0;       // entries of one snapshot slot's weighted structure factors
+    int64_t snap_slots = 0; // slots allocated
     std::vector<double> nvt_E, nvt_sW, nvt_sUW;
     int64_t nvt_np = 0;
     // device time of the last pa_nvt_run / pa_widom_chain_energies, from events around each launch
@@ -890,22 +895,147 @@ double insertion_constant_term(const pa_batch *b, int u, int Ng)
     return base + guest_tail_h(b, u, N + 1) - guest_tail_h(b, u, N) + net_h(b, u, N + 1) - net_h(b, u, N);
 }
 
-// Fractional coordinates and charge of every framework atom (the chain-insertion kernel takes
-// minimum images in fractional coordinates).
-int ensure_atomf64(pa_state *st)
+// Spreads the low 10 bits of v to every third bit (Morton interleave).
+uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FF;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+// Tables of the chain-insertion kernel, built once per state: every framework's atoms in
+// Morton order of a ~4 Å cell grid, cut into chunks of 64 with a centre and a radius that
+// bounds every atom's nearest-image distance to the centre; the k-vectors cut into runs of
+// consecutive n3 at fixed (n1, n2), longest first; half the smallest perpendicular width.
+int ensure_chain_tables(pa_state *st)
 {
     pa_batch *b = st->b;
     if (st->d_atomf64.p) return PA_OK;
-    std::vector<double4> af(std::max<size_t>(b->atom64_h.size(), 1), make_double4(0, 0, 0, 0));
+    const size_t na = std::max<size_t>(b->atom64_h.size(), 1);
+    std::vector<double4> af(na, make_double4(0, 0, 0, 0));
+    std::vector<int> at(na, 0), chunk_off(1, 0), round_off(1, 0), perm(std::max<size_t>(b->k64_h.size(), 1), 0),
+        fsize(b->nfw, 0);
+    std::vector<double4> chunks;
+    std::vector<int2> lines, rounds;
+    std::vector<double> halfw(b->nfw);
     for (int u = 0; u < b->nfw; ++u) {
         const FwDesc &F = b->desc[u];
+        const FwHost &h = b->info[u];
         const std::vector<double> &fr = b->fgh[u].frac;
-        if ((int)fr.size() != 3 * F.natoms) return fail(PA_ERR_INTERNAL, "framework %d: fractional table missing", u + 1);
-        for (int j = 0; j < F.natoms; ++j)
-            af[F.atom_off + j] = make_double4(fr[3 * j], fr[3 * j + 1], fr[3 * j + 2], b->atom64_h[F.atom_off + j].w);
+        const int n = F.natoms;
+        if ((int)fr.size() != 3 * n) return fail(PA_ERR_INTERNAL, "framework %d: fractional table missing", u + 1);
+        int nc[3];
+        for (int d = 0; d < 3; ++d) nc[d] = std::max(1, std::min(1024, (int)std::floor(h.perp[d] / 4.0)));
+        std::vector<uint64_t> key(n);
+        std::vector<double> f(3 * (size_t)n);
+        for (int j = 0; j < n; ++j) {
+            uint32_t c[3];
+            for (int d = 0; d < 3; ++d) {
+                const double x = fr[3 * j + d] - std::floor(fr[3 * j + d]);
+                f[3 * j + d] = x;
+                c[d] = (uint32_t)std::min(nc[d] - 1, (int)(x * nc[d]));
+            }
+            const uint64_t m = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+            key[j] = (m << 32) | (uint32_t)j;
+        }
+        std::sort(key.begin(), key.end());
+        for (int i = 0; i < n; ++i) {
+            const int j = (int)(key[i] & 0xFFFFFFFFu);
+            af[F.atom_off + i] = make_double4(f[3 * j], f[3 * j + 1], f[3 * j + 2], b->atom64_h[F.atom_off + j].w);
+            at[F.atom_off + i] = b->atype_h[F.atom_off + j];
+        }
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int c1 = std::min(n, c0 + 64);
+            const double4 r0 = af[F.atom_off + c0];
+            std::vector<double> uw(3 * (size_t)(c1 - c0));
+            double cen[3] = {0, 0, 0};
+            for (int i = c0; i < c1; ++i) {
+                const double4 p = af[F.atom_off + i];
+                const double d[3] = {p.x - r0.x, p.y - r0.y, p.z - r0.z};
+                for (int e = 0; e < 3; ++e) {
+                    uw[3 * (i - c0) + e] = (e == 0 ? r0.x : e == 1 ? r0.y : r0.z) + (d[e] - std::rint(d[e]));
+                    cen[e] += uw[3 * (i - c0) + e];
+                }
+            }
+            for (int e = 0; e < 3; ++e) cen[e] /= (double)(c1 - c0);
+            double rad = 0.0;
+            for (int i = 0; i < c1 - c0; ++i) {
+                const double d[3] = {uw[3 * i] - cen[0], uw[3 * i + 1] - cen[1], uw[3 * i + 2] - cen[2]};
+                double m[3];
+                matvec(F.A, d, m);
+                rad = std::max(rad, std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+            }
+            chunks.push_back(make_double4(cen[0], cen[1], cen[2], rad * (1.0 + 1e-12) + 1e-9));
+        }
+        chunk_off.push_back((int)chunks.size());
+        // runs of consecutive n3 at fixed (n1, n2), in k-table order, then longest first
+        std::vector<int2> fl;
+        for (int k = 0; k < F.nk; ++k) {
+            const int4 nn = b->kn_h[F.k_off + k];
+            if (k > 0) {
+                const int4 pn = b->kn_h[F.k_off + k - 1];
+                if (nn.x == pn.x && nn.y == pn.y && nn.z == pn.z + 1) {
+                    ++fl.back().y;
+                    continue;
+                }
+            }
+            fl.push_back(make_int2(k, 1));
+        }
+        std::stable_sort(fl.begin(), fl.end(), [](const int2 &x, const int2 &y) { return x.y > y.y; });
+        // rounds of 64 runs (one per lane); step i of lane l in a round sits at base + 64 i + l
+        int base = 0;
+        for (size_t r0 = 0; r0 < fl.size(); r0 += 64) {
+            rounds.push_back(make_int2(base, 0));
+            const int len0 = fl[r0].y;  // longest run of the round
+            for (int l = 0; l < 64; ++l) {
+                const size_t r = r0 + l;
+                if (r < fl.size()) {
+                    lines.push_back(fl[r]);
+                    for (int i = 0; i < fl[r].y; ++i) perm[F.k_off + fl[r].x + i] = base + 64 * i + l;
+                } else {
+                    lines.push_back(make_int2(0, 0));
+                }
+            }
+            base += 64 * len0;
+        }
+        round_off.push_back((int)rounds.size());
+        fsize[u] = base;
+        halfw[u] = 0.5 * std::min({h.perp[0], h.perp[1], h.perp[2]});
     }
+    const double rcmax = std::max(b->rc_lj, b->gdev.charged ? b->rc_ew : 0.0);
+    st->cull_extra = rcmax + b->r_guest + 1e-6;
+    HIP_TRY(upload(st->d_atype_s, at));
+    HIP_TRY(upload(st->d_chunk, chunks));
+    HIP_TRY(upload(st->d_chunk_off, chunk_off));
+    HIP_TRY(upload(st->d_lines, lines));
+    HIP_TRY(upload(st->d_rounds, rounds));
+    HIP_TRY(upload(st->d_round_off, round_off));
+    HIP_TRY(upload(st->d_perm, perm));
+    std::vector<int> coff(st->nsys + 1, 0);
+    for (int n = 0; n < st->nsys; ++n) coff[n + 1] = coff[n] + fsize[b->fw_of[n]];
+    st->ctot = coff[st->nsys];
+    HIP_TRY(upload(st->d_coff, coff));
+    HIP_TRY(upload(st->d_half_perp, halfw));
     HIP_TRY(upload(st->d_atomf64, af));
     return PA_OK;
+}
+
+ChainSnap chain_snap(pa_state *st)
+{
+    ChainSnap S{};
+    S.nsys = st->nsys;
+    S.nslots = st->guest_off[st->nsys];
+    S.ctot = st->ctot;
+    S.coff = st->d_coff.as<int>();
+    S.perm = st->d_perm.as<int>();
+    S.occ = st->d_snap_occ.as<int>();
+    S.pos = st->d_snap_pos.as<double4>();
+    S.quat = st->d_snap_quat.as<double4>();
+    S.csk = st->d_snap_sk.as<double2>();
+    return S;
 }
 
 ChainWidomArgs base_chain_widom(pa_state *st)
@@ -915,7 +1045,14 @@ ChainWidomArgs base_chain_widom(pa_state *st)
     a.fw = b->d_fw.as<FwDesc>();
     a.fw_of = b->d_fw_of.as<int>();
     a.atomf64 = st->d_atomf64.as<double4>();
-    a.atype = b->d_atype.as<int>();
+    a.atype = st->d_atype_s.as<int>();
+    a.chunk = st->d_chunk.as<double4>();
+    a.chunk_off = st->d_chunk_off.as<int>();
+    a.lines = st->d_lines.as<int2>();
+    a.rounds = st->d_rounds.as<int2>();
+    a.round_off = st->d_round_off.as<int>();
+    a.half_perp = st->d_half_perp.as<double>();
+    a.cull_extra = st->cull_extra;
     a.k64 = b->d_k64.as<double4>();
     a.kn = b->d_kn.as<int4>();
     a.pair64 = b->d_pair64.as<double2>();
@@ -925,48 +1062,41 @@ ChainWidomArgs base_chain_widom(pa_state *st)
     a.guest = b->gdev;
     a.nsys = st->nsys;
     a.guest_off = st->d_guest_off.as<int>();
-    a.sk_off = st->d_sk_off.as<int>();
-    a.occupancy = st->d_occ.as<int>();
-    a.refpos = st->d_refpos.as<double4>();
-    a.quat = st->d_quat.as<double4>();
-    a.Sk = st->d_Sk.as<double2>();
-    a.nslots = st->guest_off[st->nsys];
-    a.nk_total = st->sk_off[st->nsys];
+    a.snap = chain_snap(st);
     return a;
 }
 
-// Work split of one chain-insertion launch: 64 insertions of one system per workgroup; the
-// grid's second dimension splits every system's atoms, k-vectors and guests further while the
-// workgroups are too few to fill the device.
-int launch_chain(pa_state *st, ChainWidomArgs &a, const std::vector<int64_t> &item_off, int *nsplit_out)
+// Makes room for `slots` snapshot slots.
+int ensure_snapshots(pa_state *st, int64_t slots)
 {
-    pa_batch *b = st->b;
-    std::vector<int2> groups;
-    int maxwork = 1;
-    for (int s = 0; s < st->nsys; ++s) {
-        for (int64_t i = item_off[s]; i < item_off[s + 1]; i += 64) groups.push_back(make_int2(s, (int)i));
-        if (item_off[s + 1] > item_off[s]) {
-            const FwDesc &F = b->desc[b->fw_of[s]];
-            maxwork = std::max(maxwork, std::max(F.natoms, F.nk));
-        }
-    }
+    if (slots <= st->snap_slots) return PA_OK;
+    const size_t ns = std::max(1, st->guest_off[st->nsys]);
+    HIP_TRY(st->d_snap_pos.ensure(sizeof(double4) * ns * slots));
+    HIP_TRY(st->d_snap_quat.ensure(sizeof(double4) * ns * slots));
+    HIP_TRY(st->d_snap_sk.ensure(sizeof(double2) * std::max<int64_t>(1, st->ctot) * slots));
+    HIP_TRY(st->d_snap_occ.ensure(sizeof(int) * st->nsys * slots));
+    st->snap_slots = slots;
+    return PA_OK;
+}
+
+// Snapshot slot `slot` of the live state.
+int snapshot_live(pa_state *st, int64_t slot)
+{
+    HIP_TRY(launch_chain_snapshot(chain_snap(st), slot, base_mc(st), st->b->stream));
+    return PA_OK;
+}
+
+// One chain-insertion launch over the insertions item_off describes (one wavefront each).
+int launch_chain(pa_state *st, ChainWidomArgs &a, const std::vector<int64_t> &item_off)
+{
     const int64_t nitems = item_off[st->nsys];
     if (nitems > INT32_MAX) return fail(PA_ERR_ARG, "%lld test insertions in one launch exceed 2^31", (long long)nitems);
-    const int ng = (int)groups.size();
-    int nsplit = 1;
-    if (ng > 0) {
-        const int want = (2048 + ng - 1) / ng, cap = std::max(1, std::min(64, maxwork / 256));
-        nsplit = std::max(1, std::min(want, cap));
-    }
     HIP_TRY(upload(st->d_item_off, item_off));
-    HIP_TRY(upload(st->d_groups, groups));
-    HIP_TRY(st->d_part.ensure(sizeof(double) * std::max<int64_t>(1, nitems * nsplit)));
+    HIP_TRY(st->d_cw_dU.ensure(sizeof(double) * std::max<int64_t>(1, nitems)));
     a.item_off = st->d_item_off.as<int64_t>();
-    a.groups = st->d_groups.as<int2>();
     a.nitems = nitems;
-    a.part = st->d_part.as<double>();
-    HIP_TRY(launch_chain_widom(a, ng, nsplit, b->stream));
-    *nsplit_out = nsplit;
+    a.dU = st->d_cw_dU.as<double>();
+    HIP_TRY(launch_chain_widom(a, st->b->stream));
     return PA_OK;
 }
 
@@ -1014,33 +1144,32 @@ extern "C" int pa_widom_chain_energies(pa_state *st, int32_t sys, int64_t n, con
         if (!std::isfinite(pos[i])) return fail(PA_ERR_ARG, "pos[%lld] = %g is not finite", (long long)i, pos[i]);
     for (int64_t i = 0; i < 4 * n; ++i)
         if (!std::isfinite(quat[i])) return fail(PA_ERR_ARG, "quat[%lld] = %g is not finite", (long long)i, quat[i]);
-    int s0 = ensure_atomf64(st);
+    int s0 = ensure_chain_tables(st);
     if (s0) return s0;
     std::vector<double> ct;
     s0 = upload_const_terms(st, ct);
     if (s0) return s0;
     HIP_TRY(upload(st->d_pose_pos, std::vector<double>(pos, pos + 3 * n)));
     HIP_TRY(upload(st->d_pose_quat, std::vector<double>(quat, quat + 4 * n)));
+    s0 = ensure_snapshots(st, 1);
+    if (s0) return s0;
+    s0 = snapshot_live(st, 0);
+    if (s0) return s0;
     ChainWidomArgs a = base_chain_widom(st);
-    a.snap_n = 0;
+    a.snap_n = 1;
     a.nw = n;
     a.pose_pos = st->d_pose_pos.as<double>();
     a.pose_quat = st->d_pose_quat.as<double>();
     std::vector<int64_t> item_off(st->nsys + 1);
     for (int s = 0; s <= st->nsys; ++s) item_off[s] = s <= sys ? 0 : n;
-    int nsplit = 1;
     st->timing = pa_nvt_timing{};
     s0 = timed(st, &st->timing.insertion_ms, &st->timing.insertion_launches,
-               [&] { return launch_chain(st, a, item_off, &nsplit); });
+               [&] { return launch_chain(st, a, item_off); });
     if (s0) return s0;
     st->timing.insertions = n;
-    std::vector<double> part;
-    HIP_TRY(download(part, st->d_part, (size_t)nsplit * n));
-    for (int64_t i = 0; i < n; ++i) {
-        double e = 0.0;
-        for (int p = 0; p < nsplit; ++p) e += part[(size_t)p * n + i];
-        dU[i] = e + ct[sys];
-    }
+    std::vector<double> e;
+    HIP_TRY(download(e, st->d_cw_dU, (size_t)n));
+    for (int64_t i = 0; i < n; ++i) dU[i] = e[i] + ct[sys];
     return ok();
 }
 
@@ -1066,7 +1195,7 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
     if (s0) return s0;
     s0 = upload_per_chain(st, st->d_step_r, p->step_rot, "step_rot");
     if (s0) return s0;
-    s0 = ensure_atomf64(st);
+    s0 = ensure_chain_tables(st);
     if (s0) return s0;
     const double kT = KB * p->T;
     const int64_t NP = p->n_production, NW = p->n_widom_per_cycle;
@@ -1100,25 +1229,15 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
     a.samp_stride = NP;
     st->samp_stride = NP;
     HIP_TRY(upload(st->d_maxocc, occ));
-    a.nslots = st->guest_off[nsys];
-    a.nk_total = st->sk_off[nsys];
-
     // cycles per launch: bounded by the audit interval, by 256 MiB of snapshots and by 2^24
     // test insertions
-    const double snap_bytes = (double)a.nslots * 2 * sizeof(double4) + (double)a.nk_total * sizeof(double2) + 4.0 * nsys;
-    int64_t seg_cap = std::max<int64_t>(1, (int64_t)(256.0 * 1024 * 1024 / snap_bytes) + 1);
+    const double snap_bytes =
+        (double)st->guest_off[nsys] * 2 * sizeof(double4) + (double)st->ctot * sizeof(double2) + 4.0 * nsys;
+    int64_t seg_cap = std::max<int64_t>(1, (int64_t)(256.0 * 1024 * 1024 / snap_bytes));
     seg_cap = std::min<int64_t>(seg_cap, std::max<int64_t>(1, (int64_t(1) << 24) / ((int64_t)nsys * NW)));
-    if (any_guests) {
-        const int64_t ns = std::max<int64_t>(1, std::min(seg_cap, NP) - 1);
-        HIP_TRY(st->d_snap_pos.ensure(sizeof(double4) * std::max(1, a.nslots) * ns));
-        HIP_TRY(st->d_snap_quat.ensure(sizeof(double4) * std::max(1, a.nslots) * ns));
-        HIP_TRY(st->d_snap_sk.ensure(sizeof(double2) * std::max(1, a.nk_total) * ns));
-        HIP_TRY(st->d_snap_occ.ensure(sizeof(int) * nsys * ns));
-        a.snap_pos = st->d_snap_pos.as<double4>();
-        a.snap_quat = st->d_snap_quat.as<double4>();
-        a.snap_sk = st->d_snap_sk.as<double2>();
-        a.snap_occ = st->d_snap_occ.as<int>();
-    }
+    s0 = ensure_snapshots(st, std::max<int64_t>(1, std::min(seg_cap, NP)));
+    if (s0) return s0;
+    a.snap = chain_snap(st);
     st->nvt_np = 0;
     st->nvt_E.assign((size_t)nsys * NP, 0.0);
     st->nvt_sW.assign((size_t)nsys * NP, 0.0);
@@ -1143,7 +1262,7 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
                 a.ncycles = len;
                 a.sampling = phase == 1;
                 a.samp_base = c0;
-                a.snap_n = phase == 1 ? len - 1 : 0;
+                a.snap_n = phase == 1 ? len - 1 : 0;  // the segment's last cycle: snapshot_live below
                 int s = timed(st, &st->timing.move_ms, &st->timing.move_launches, [&]() -> int {
                     HIP_TRY(launch_mc(a, b->stream));
                     return PA_OK;
@@ -1171,34 +1290,30 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
                         for (int64_t i = 0; i < len; ++i) dst[i] = Ecur[n];
                     }
                 }
-                // test insertions after every cycle of the segment
+                // test insertions after every cycle of the segment; the last cycle (or, with no
+                // guest slot, every cycle) sees the live, possibly audited, state
+                const int64_t last = any_guests ? len - 1 : 0;
+                int s1 = snapshot_live(st, last);
+                if (s1) return s1;
                 ChainWidomArgs w = base_chain_widom(st);
-                w.snap_n = any_guests ? len - 1 : 0;
-                w.snap_pos = st->d_snap_pos.as<double4>();
-                w.snap_quat = st->d_snap_quat.as<double4>();
-                w.snap_sk = st->d_snap_sk.as<double2>();
-                w.snap_occ = st->d_snap_occ.as<int>();
+                w.snap_n = last + 1;
                 w.nw = NW;
                 w.cyc0 = c0;
                 w.widom_seed = p->widom_seed;
                 std::vector<int64_t> item_off(nsys + 1);
                 for (int s = 0; s <= nsys; ++s) item_off[s] = (int64_t)s * len * NW;
-                int nsplit = 1;
                 int s = timed(st, &st->timing.insertion_ms, &st->timing.insertion_launches,
-                              [&] { return launch_chain(st, w, item_off, &nsplit); });
+                              [&] { return launch_chain(st, w, item_off); });
                 if (s) return s;
                 const int64_t nit = item_off[nsys];
                 st->timing.insertions += nit;
-                HIP_TRY(download(part, st->d_part, (size_t)nsplit * nit));
+                HIP_TRY(download(part, st->d_cw_dU, (size_t)nit));
                 // boltzmann_weight (src/widom.jl:80-84) summed per system and cycle in insertion order
                 for (int n = 0; n < nsys; ++n)
                     for (int64_t cl = 0; cl < len; ++cl) {
                         double sw = 0.0, suw = 0.0;
                         for (int64_t t = 0; t < NW; ++t) {
-                            const int64_t i = item_off[n] + cl * NW + t;
-                            double e = 0.0;
-                            for (int q = 0; q < nsplit; ++q) e += part[(size_t)q * nit + i];
-                            const double dU = e + ct[n];
+                            const double dU = part[item_off[n] + cl * NW + t] + ct[n];
                             const double wgt = std::exp(-dU / kT);
                             sw += wgt;
                             suw += wgt == 0.0 ? 0.0 : dU * wgt;

@@ -139,14 +139,20 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
     auto flush_snapshots = [&]() {
         for (int p = 0; p < npend; ++p) {
             const int64_t cs = pend[p];
-            double2 *dsk = a.snap_sk + cs * a.nk_total + a.sk_off[n];
-            for (int k = tid; k < nk; k += MC_WG) dsk[k] = Sk[k];
-            double4 *dp = a.snap_pos + cs * a.nslots + gr0, *dq = a.snap_quat + cs * a.nslots + gr0;
+            const ChainSnap &S = a.snap;
+            double2 *dsk = S.csk + cs * S.ctot + S.coff[n];
+            const int *perm = S.perm + F.k_off;
+            for (int k = tid; k < nk; k += MC_WG) {
+                const double w = ks[k].w;
+                const double2 v = Sk[k];
+                dsk[perm[k]] = make_double2(w * v.x, w * v.y);
+            }
+            double4 *dp = S.pos + cs * S.nslots + gr0, *dq = S.quat + cs * S.nslots + gr0;
             for (int i = tid; i < occ; i += MC_WG) {
                 dp[i] = a.refpos[gr0 + i];
                 dq[i] = a.quat[gr0 + i];
             }
-            if (tid == 0) a.snap_occ[cs * a.nsys + n] = occ;
+            if (tid == 0) S.occ[cs * S.nsys + n] = occ;
         }
         npend = 0;
     };

@@ -5,20 +5,26 @@
 //   host-guest real space (LJ + KE q q' erfc(alpha r)/r, minimum image per site pair)
 //   + KE * 2 sum_k kpref_k Re(conj(S_k) S_g,k)    against the chain's running S (host + G)
 //   + guest-guest real space against every live guest of G
-//   + const_term[s]                                (insertion_constant_term, src/nvt.jl:44-66)
+// (the host adds insertion_constant_term, src/nvt.jl:44-66).
 //
-// Layout of a launch: the insertions of one system are contiguous (cycle slot major, then the
-// insertion within the cycle), and a workgroup takes 64 consecutive insertions of one system,
-// one per lane, so every framework atom and k-vector is the same for all lanes of a wavefront
-// (scalar loads, no divergence in the loops). The workgroup's four wavefronts and the grid's
-// second dimension split the atoms, k-vectors and guests of the system into P = 4 * gridDim.y
-// parts; each workgroup writes the sum of its four parts per insertion, and the host adds the
-// gridDim.y partial sums in a fixed order.
-//
-// Reciprocal term: k = 2 pi A^-T n with integer n enumerated n1 outer, n2, n3 inner ascending
-// (kvectors, src/ewald.jl:142-157), so consecutive k-vectors of one (n1, n2) line differ by
-// one in n3. A lane evaluates e^{i k.r} = e^{2 pi i n.f} (f = A^-1 r) with one sincospi at the
-// start of each line and one complex multiply by e^{2 pi i f3} per further k-vector.
+// One wavefront per insertion:
+//   * host-guest: the framework's atoms are stored in spatially compact chunks of 64 (one atom
+//     per lane); pairs inside the cutoff are compacted through LDS before the LJ + erfc
+//     evaluation. A chunk whose centre lies farther than its radius + guest radius + cutoff from
+//     the insertion's reference point is skipped whole. The distance is a lower bound of the
+//     true nearest-image distance: the rint image is the nearest one whenever it is shorter
+//     than half the smallest perpendicular width, so min(rint distance, half width) never
+//     exceeds the true distance; and with widths >= 2 * cutoff every pair within the cutoff
+//     is found by the per-pair rint image the energy uses.
+//   * reciprocal: k = 2 pi A^-T n is enumerated n1 outer, n2, n3 inner ascending (kvectors,
+//     src/ewald.jl:142-157), so the k-vectors fall into runs of consecutive n3 at fixed
+//     (n1, n2). Each lane takes whole runs (longest first, so the lanes' trip counts match):
+//     one sincospi of 2 pi n.f per site at the start of a run, then one complex multiply by
+//     e^{2 pi i f3} per further k-vector (f = A^-1 r). The snapshot stores kprefactor * S in
+//     the lanes' reading order, so each step is one coalesced load.
+//   * guest-guest: lanes take (guest, site, site) triples.
+// The lanes' partial sums are reduced across the wavefront in a fixed order. The pose is drawn
+// exactly as the Widom path draws it; the energy terms use fused multiply-adds.
 #include "pa_device.hpp"
 
 namespace pa {
@@ -26,42 +32,87 @@ namespace pa {
 constexpr int CW_WG = 256;
 constexpr int CW_NW = CW_WG / 64;
 
+__device__ inline double cw_wave_sum(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The insertion energies need FP64 accuracy (~1e-14 relative), not bit-for-bit agreement with
+// another code path, so the helpers below fuse multiply-adds (the shared helpers keep every
+// rounding step of the reference's formulas for the paths that need exact agreement).
+// r^2 of the cell vector A (f - rint(f)) for a fractional offset f.
+__device__ inline double cw_r2(const double A[9], double f0, double f1, double f2)
+{
+    f0 -= __builtin_rint(f0);
+    f1 -= __builtin_rint(f1);
+    f2 -= __builtin_rint(f2);
+    const double m0 = __builtin_fma(A[0], f0, __builtin_fma(A[3], f1, A[6] * f2));
+    const double m1 = __builtin_fma(A[1], f0, __builtin_fma(A[4], f1, A[7] * f2));
+    const double m2 = __builtin_fma(A[2], f0, __builtin_fma(A[5], f1, A[8] * f2));
+    return __builtin_fma(m0, m0, __builtin_fma(m1, m1, m2 * m2));
+}
+
+// LJ inside rc_lj plus qq_ke erfc(alpha r)/r inside rc_ew (site_pair_energy with one division
+// for 1/r shared by both terms).
+__device__ inline double cw_pair(double r2, double sg, double ep, double qq_ke, double alpha, double rc_lj2,
+                                 double rc_ew2)
+{
+    const double r = __builtin_sqrt(r2);
+    const double ir = 1.0 / r;
+    double e = 0.0;
+    if (r2 < rc_lj2) {
+        const double s = sg * ir;
+        double x = s * s;
+        x = x * x * x;
+        e = 4.0 * ep * __builtin_fma(x, x, -x);
+    }
+    if (r2 < rc_ew2) e = __builtin_fma(qq_ke * pair_erfc_f64_t<true>(alpha * r), ir, e);
+    return e;
+}
+
+constexpr int CW_RING = 128;  // per-wavefront ring of in-cutoff pairs (< 64 pending + one append)
+
 template <int NS>
 __global__ __launch_bounds__(CW_WG) void chain_widom_kernel(ChainWidomArgs a)
 {
-#pragma clang fp contract(off)
-    __shared__ double red[CW_NW][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int2 grp = a.groups[blockIdx.x];  // system, first insertion
-    const int s = grp.x;
-    const int64_t item = (int64_t)grp.y + lane;
-    const int64_t iend = a.item_off[s + 1];
-    const bool valid = item < iend;
-    const int64_t local = valid ? item - a.item_off[s] : 0;
+    __shared__ double ring_r2[CW_NW][CW_RING], ring_qq[CW_NW][CW_RING];
+    __shared__ int ring_code[CW_NW][CW_RING];
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * CW_NW + (threadIdx.x >> 6);
+    if (item >= a.nitems) return;  // whole wavefront; the kernel has no barrier
+    int lo = 0, hi = a.nsys - 1;
+    while (lo < hi) {  // system of the item: item_off[s] <= item < item_off[s+1]
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.item_off[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int s = lo;
+    const int64_t local = item - a.item_off[s];
     const int64_t cl = local / a.nw, t = local - cl * a.nw;
-    const FwDesc &F = a.fw[a.fw_of[s]];
+    const int u = a.fw_of[s];
+    const FwDesc &F = a.fw[u];
     const GuestDev &G = a.guest;
 
-    // the state this insertion sees: the snapshot of its cycle slot, or the live state
-    const bool snap = cl < a.snap_n;
+    // the state this insertion sees: the snapshot of its cycle slot
+    const ChainSnap &S = a.snap;
+    const int64_t slot = cl < a.snap_n ? cl : a.snap_n - 1;
     const int gr0 = a.guest_off[s];
-    const int occ = snap ? a.snap_occ[cl * a.nsys + s] : a.occupancy[s];
-    const double4 *gpos = snap ? a.snap_pos + cl * a.nslots + gr0 : a.refpos + gr0;
-    const double4 *gquat = snap ? a.snap_quat + cl * a.nslots + gr0 : a.quat + gr0;
-    const double2 *Sk = snap ? a.snap_sk + cl * a.nk_total + a.sk_off[s] : a.Sk + a.sk_off[s];
+    const int occ = S.occ[slot * S.nsys + s];
+    const double4 *gpos = S.pos + slot * S.nslots + gr0;
+    const double4 *gquat = S.quat + slot * S.nslots + gr0;
+    const double2 *csk = S.csk + slot * S.ctot + S.coff[s];
 
-    // pose
     double pos[3], q[4];
     if (a.pose_pos) {
-        const int64_t i = valid ? item : a.item_off[s];
-        for (int d = 0; d < 3; ++d) pos[d] = a.pose_pos[3 * i + d];
-        for (int d = 0; d < 4; ++d) q[d] = a.pose_quat[4 * i + d];
+        for (int d = 0; d < 3; ++d) pos[d] = a.pose_pos[3 * item + d];
+        for (int d = 0; d < 4; ++d) q[d] = a.pose_quat[4 * item + d];
     } else {
         double f[3];
         pose_uniforms(a.widom_seed, s, (a.cyc0 + cl) * a.nw + t, f, q);
         matvec(F.A, f, pos);
     }
-    double sc[NS][3], sf[NS][3];
+    double pf[3], sc[NS][3], sf[NS][3];
+    matvec(F.invA, pos, pf);
 #pragma unroll
     for (int x = 0; x < NS; ++x) {
         double r[3];
@@ -69,113 +120,167 @@ __global__ __launch_bounds__(CW_WG) void chain_widom_kernel(ChainWidomArgs a)
         for (int d = 0; d < 3; ++d) sc[x][d] = pos[d] + r[d];
         matvec(F.invA, sc[x], sf[x]);
     }
-
-    const int P = CW_NW * gridDim.y, part = blockIdx.y * CW_NW + w;
     double e = 0.0;
 
-    // (1) host-guest real space over this part's atoms
+    // (1) host-guest real space, chunk by chunk. The (site, atom) pairs inside the cutoff are
+    // appended to the wavefront's ring in LDS and evaluated 64 at a time, one per lane, so the
+    // LJ + erfc work runs on full wavefronts whatever the lanes' own pairs do.
     {
-        const int j0 = (int)((int64_t)F.natoms * part / P), j1 = (int)((int64_t)F.natoms * (part + 1) / P);
+        const int w = threadIdx.x >> 6;
+        double *qr2 = ring_r2[w], *qqq = ring_qq[w];
+        int *qcode = ring_code[w];
+        unsigned head = 0, tail = 0;
+        auto drain = [&](unsigned n) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if ((unsigned)lane < n) {
+                const unsigned i = (head + lane) & (CW_RING - 1);
+                const int code = qcode[i];
+                const double2 p = a.pair64[code];
+                e += cw_pair(qr2[i], p.x, p.y, qqq[i], F.alpha, a.rc_lj2, a.rc_ew2);
+            }
+            head += n;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        const int c0 = a.chunk_off[u], c1 = a.chunk_off[u + 1];
+        const double halfw = a.half_perp[u];
         const double4 *af = a.atomf64 + F.atom_off;
         const int *types = a.atype + F.atom_off;
 #pragma unroll 1
-        for (int j = j0; j < j1; ++j) {
-            const double4 h = af[j];
-            const int ht = types[j];
+        for (int c = c0; c < c1; ++c) {
+            const double4 cc = a.chunk[c];
+            const double dist = fmin(__builtin_sqrt(cw_r2(F.A, pf[0] - cc.x, pf[1] - cc.y, pf[2] - cc.z)), halfw);
+            if (dist > cc.w + a.cull_extra) continue;
+            const int j = (c - c0) * 64 + lane;
+            const bool valid = j < F.natoms;
+            const double4 h = valid ? af[j] : make_double4(0, 0, 0, 0);
+            const int ht = valid ? types[j] : 0;
 #pragma unroll
             for (int x = 0; x < NS; ++x) {
-                double df[3] = {sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z}, m[3];
-                for (int d = 0; d < 3; ++d) df[d] = df[d] - __builtin_rint(df[d]);
-                matvec(F.A, df, m);
-                const double r2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-                if (r2 < a.rc_lj2 || r2 < a.rc_ew2) {
-                    const double2 p = a.pair64[x * a.ntypes_c + ht];
-                    e += site_pair_energy(r2, p.x, p.y, KE * G.q[x] * h.w, F.alpha, a.rc_lj2, a.rc_ew2);
+                const double r2 = cw_r2(F.A, sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z);
+                const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
+                const uint64_t mask = __ballot(in);
+                if (in) {
+                    const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    const unsigned i = (tail + below) & (CW_RING - 1);
+                    qr2[i] = r2;
+                    qqq[i] = KE * G.q[x] * h.w;
+                    qcode[i] = x * a.ntypes_c + ht;
                 }
+                tail += (unsigned)__popcll(mask);
+                if (tail - head >= 64) drain(64);
             }
         }
+        drain(tail - head);
     }
-    // (2) reciprocal cross term over this part's k-vectors
+    // (2) reciprocal cross term, whole runs of consecutive n3 per lane; step i of round R reads
+    // the weighted structure factor at round base + 64 i + lane (consecutive across the lanes)
     if (G.charged) {
-        const int k0 = (int)((int64_t)F.nk * part / P), k1 = (int)((int64_t)F.nk * (part + 1) / P);
-        const double4 *ks = a.k64 + F.k_off;
         const int4 *kn = a.kn + F.k_off;
-        double zr[NS], zi[NS], str[NS], sti[NS];
+        const int r0 = a.round_off[u], r1 = a.round_off[u + 1];
+        double str[NS], sti[NS];
 #pragma unroll
-        for (int x = 0; x < NS; ++x) {
-            sincospi(2.0 * sf[x][2], &sti[x], &str[x]);
-            zr[x] = 1.0;
-            zi[x] = 0.0;
-        }
-        int px = 0, py = 0, pz = 0;
-        bool have = false;
+        for (int x = 0; x < NS; ++x) sincospi(2.0 * sf[x][2], &sti[x], &str[x]);
         double acc = 0.0;
 #pragma unroll 1
-        for (int k = k0; k < k1; ++k) {
-            const int4 nn = kn[k];
-            if (have && nn.x == px && nn.y == py && nn.z == pz + 1) {
+        for (int R = r0; R < r1; ++R) {
+            const int2 ln = a.lines[(int64_t)R * 64 + lane];
+            if (ln.y == 0) continue;
+            const double2 *cr = csk + a.rounds[R].x + lane;
+            const int4 nn = kn[ln.x];
+            double zr[NS], zi[NS];
+#pragma unroll
+            for (int x = 0; x < NS; ++x)
+                sincospi(2.0 * (((double)nn.x * sf[x][0] + (double)nn.y * sf[x][1]) + (double)nn.z * sf[x][2]), &zi[x],
+                         &zr[x]);
+#pragma unroll 1
+            for (int i = 0; i < ln.y; ++i) {
+                if (i > 0) {
+#pragma unroll
+                    for (int x = 0; x < NS; ++x) {
+                        const double r = __builtin_fma(zr[x], str[x], -(zi[x] * sti[x]));
+                        const double im = __builtin_fma(zr[x], sti[x], zi[x] * str[x]);
+                        zr[x] = r;
+                        zi[x] = im;
+                    }
+                }
+                double gre = 0.0, gim = 0.0;
 #pragma unroll
                 for (int x = 0; x < NS; ++x) {
-                    const double r = zr[x] * str[x] - zi[x] * sti[x];
-                    const double i = zr[x] * sti[x] + zi[x] * str[x];
-                    zr[x] = r;
-                    zi[x] = i;
+                    gre = __builtin_fma(G.q[x], zr[x], gre);
+                    gim = __builtin_fma(G.q[x], zi[x], gim);
                 }
-            } else {
-#pragma unroll
-                for (int x = 0; x < NS; ++x)
-                    sincospi(2.0 * (((double)nn.x * sf[x][0] + (double)nn.y * sf[x][1]) + (double)nn.z * sf[x][2]), &zi[x],
-                             &zr[x]);
+                const double2 c = cr[(int64_t)i * 64];
+                acc = __builtin_fma(c.x, gre, __builtin_fma(c.y, gim, acc));
             }
-            px = nn.x;
-            py = nn.y;
-            pz = nn.z;
-            have = true;
-            double gre = 0.0, gim = 0.0;
-#pragma unroll
-            for (int x = 0; x < NS; ++x) {
-                gre += G.q[x] * zr[x];
-                gim += G.q[x] * zi[x];
-            }
-            const double2 S = Sk[k];
-            acc += ks[k].w * (S.x * gre + S.y * gim);
         }
         e += KE * (2.0 * acc);
     }
-    // (3) guest-guest real space against this part's share of the live guests
+    // (3) guest-guest real space, one (guest, site, site) triple per lane step
+    const int ntrip = occ * NS * NS;
 #pragma unroll 1
-    for (int j = part; j < occ; j += P) {
-        const double4 p4 = gpos[j], q4 = gquat[j];
+    for (int xi = lane; xi < ntrip; xi += 64) {
+        const int jj = xi / (NS * NS), sa = (xi / NS) % NS, sb = xi % NS;
+        const double4 p4 = gpos[jj], q4 = gquat[jj];
         const double qj[4] = {q4.x, q4.y, q4.z, q4.w};
-#pragma unroll
-        for (int y = 0; y < NS; ++y) {
-            double r[3];
-            rotate(qj, G.site[y], r);
-            const double pb[3] = {p4.x + r[0], p4.y + r[1], p4.z + r[2]};
-#pragma unroll
-            for (int x = 0; x < NS; ++x) {
-                double d[3] = {sc[x][0] - pb[0], sc[x][1] - pb[1], sc[x][2] - pb[2]}, m[3];
-                minimum_image(F.A, F.invA, d, m);
-                const double r2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-                const double2 p = a.pair64[x * a.ntypes_c + G.type[y]];
-                e += site_pair_energy(r2, p.x, p.y, KE * G.q[x] * G.q[y], F.alpha, a.rc_lj2, a.rc_ew2);
-            }
+        double r[3];
+        rotate(qj, G.site[sb], r);
+        const double pb[3] = {p4.x + r[0], p4.y + r[1], p4.z + r[2]};
+        double fb[3];
+        matvec(F.invA, pb, fb);
+        const double r2 = cw_r2(F.A, sf[sa][0] - fb[0], sf[sa][1] - fb[1], sf[sa][2] - fb[2]);
+        if (r2 < a.rc_lj2 || r2 < a.rc_ew2) {
+            const double2 p = a.pair64[sa * a.ntypes_c + G.type[sb]];
+            e += cw_pair(r2, p.x, p.y, KE * G.q[sa] * G.q[sb], F.alpha, a.rc_lj2, a.rc_ew2);
         }
     }
 
-    red[w][lane] = e;
-    __syncthreads();
-    if (w == 0 && valid) {
-        double sum = red[0][lane];
-        for (int v = 1; v < CW_NW; ++v) sum += red[v][lane];
-        a.part[(int64_t)blockIdx.y * a.nitems + item] = sum;
-    }
+    e = cw_wave_sum(e);
+    if (lane == 0) a.dU[item] = e;
 }
 
-hipError_t launch_chain_widom(const ChainWidomArgs &a, int ngroups, int nsplit, hipStream_t st)
+// Snapshot slot `slot` of every system's live state (after an audit, or for insertions at
+// caller poses): one workgroup per system.
+__global__ __launch_bounds__(256) void chain_snapshot_kernel(ChainSnap S, int64_t slot, const FwDesc *fw,
+                                                            const int *fw_of, const double4 *k64, const int *guest_off,
+                                                            const int *sk_off, const int *occupancy,
+                                                            const double4 *refpos, const double4 *quat,
+                                                            const double2 *Sk)
 {
-    if (ngroups <= 0) return hipSuccess;
-    const dim3 grid((unsigned)ngroups, (unsigned)nsplit), block(CW_WG);
+    const int n = blockIdx.x, tid = threadIdx.x;
+    const FwDesc &F = fw[fw_of[n]];
+    const double4 *ks = k64 + F.k_off;
+    const double2 *sk = Sk + sk_off[n];
+    const int *perm = S.perm + F.k_off;
+    double2 *dsk = S.csk + slot * S.ctot + S.coff[n];
+    for (int k = tid; k < F.nk; k += 256) {
+        const double w = ks[k].w;
+        const double2 v = sk[k];
+        dsk[perm[k]] = make_double2(w * v.x, w * v.y);
+    }
+    const int gr0 = guest_off[n], occ = occupancy[n];
+    for (int i = tid; i < occ; i += 256) {
+        S.pos[slot * S.nslots + gr0 + i] = refpos[gr0 + i];
+        S.quat[slot * S.nslots + gr0 + i] = quat[gr0 + i];
+    }
+    if (tid == 0) S.occ[slot * S.nsys + n] = occ;
+}
+
+hipError_t launch_chain_snapshot(const ChainSnap &S, int64_t slot, const McArgs &m, hipStream_t st)
+{
+    hipLaunchKernelGGL(chain_snapshot_kernel, dim3((unsigned)S.nsys), dim3(256), 0, st, S, slot, m.fw, m.fw_of, m.k64,
+                       m.guest_off, m.sk_off, m.occupancy, m.refpos, m.quat, m.Sk);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_widom(const ChainWidomArgs &a, hipStream_t st)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((a.nitems + CW_NW - 1) / CW_NW)), block(CW_WG);
     switch (a.guest.nsites) {
     case 1: hipLaunchKernelGGL(chain_widom_kernel<1>, grid, block, 0, st, a); break;
     case 2: hipLaunchKernelGGL(chain_widom_kernel<2>, grid, block, 0, st, a); break;

@@ -194,6 +194,21 @@ constexpr int NMOVES = 5;
 // the start of the round); the pending structure-factor changes take MC_SPEC slices per chain
 constexpr int MC_SPEC = 2;
 
+// Snapshots of the chains' states for the test insertions (nvt_kernels.hip). Slot c holds, per
+// system n: occupancy at occ[c * nsys + n]; guests at pos/quat[c * nslots + guest_off[n] + i];
+// the weighted structure factor c_k = kprefactor_k * S_k of every k-vector at
+// csk[c * ctot + coff[n] + perm[F.k_off + k]], the order in which the insertion kernel's lanes
+// read it (runs of consecutive n3 interleaved across the 64 lanes).
+struct ChainSnap {
+    int nsys, nslots;
+    int64_t ctot;
+    const int *coff;           // per system: first entry of its slice of csk
+    const int *perm;           // per k-vector of the batch: position within its framework's slice
+    int *occ;
+    double4 *pos, *quat;
+    double2 *csk;
+};
+
 // One launch of the move path: every chain (system) is one workgroup that runs its move attempts
 // back to back (mc_kernels.hip).
 struct McArgs {
@@ -242,26 +257,33 @@ struct McArgs {
     int *capacity_hits;         // per chain: accepted-by-Metropolis insertions refused at capacity
     double4 *trace;             // optional, per chain trace_cap entries: type, slot, dU, accepted
     int64_t trace_cap;
-    // mode 1 snapshots: the state at the end of cycle c (0-based within the launch) for c < snap_n,
-    // guests at snap_pos/snap_quat[c * nslots + slot], structure factor at
-    // snap_sk[c * nk_total + sk_off[n] + k], occupancy at snap_occ[c * nsys + n]
+    // mode 1 snapshots: the state at the end of cycle c (0-based within the launch) for c < snap_n
+    // (layout of ChainSnap)
     int64_t snap_n;
-    int nslots, nk_total;
-    double4 *snap_pos, *snap_quat;
-    double2 *snap_sk;
-    int *snap_occ;
+    ChainSnap snap;
 };
 
 // Test insertions into the chains' current configurations (nvt_kernels.hip). Insertion i of
 // system s lies in [item_off[s], item_off[s+1]); local index l = i - item_off[s] is cycle slot
 // l / nw, insertion l % nw of that cycle. Its pose is pose_pos/pose_quat[i] when given, else the
-// Widom stream's pose of per-system index (cyc0 + slot) * nw + l % nw. Slot c < snap_n sees the
-// mode-1 snapshot c of the move launch before it, later slots the live state.
+// Widom stream's pose of per-system index (cyc0 + slot) * nw + l % nw. Cycle slot c sees snapshot
+// min(c, snap_n - 1).
 struct ChainWidomArgs {
     const FwDesc *fw;
     const int *fw_of;
-    const double4 *atomf64;     // fractional x, y, z and charge per atom
-    const int *atype;
+    // framework atoms of every framework re-ordered into spatially compact chunks of 64
+    // (same per-framework offsets as the batch's atom tables)
+    const double4 *atomf64;     // fractional x, y, z and charge
+    const int *atype;           // compact type
+    const double4 *chunk;       // per chunk: centre (fractional x, y, z), radius (Å)
+    const int *chunk_off;       // nfw + 1
+    // k-vectors of every framework grouped into runs of consecutive n3 at fixed (n1, n2),
+    // longest first, 64 runs per round (one per lane; short rounds padded with empty runs)
+    const int2 *lines;          // per (round, lane): first k-vector (framework-relative), length
+    const int2 *rounds;         // per round: first entry of the round in the snapshot slice, 0
+    const int *round_off;       // nfw + 1
+    const double *half_perp;    // per framework: half the smallest perpendicular width (Å)
+    double cull_extra;          // largest cutoff + guest radius + rounding margin (Å)
     const double4 *k64;         // kx, ky, kz, kprefactor (every k-vector)
     const int4 *kn;
     const double2 *pair64;
@@ -269,21 +291,14 @@ struct ChainWidomArgs {
     double rc_lj2, rc_ew2;
     GuestDev guest;
     int nsys;
-    const int *guest_off, *sk_off;
-    const int *occupancy;
-    const double4 *refpos, *quat;
-    const double2 *Sk;
+    const int *guest_off;
     int64_t snap_n;
-    int nslots, nk_total;
-    const double4 *snap_pos, *snap_quat;
-    const double2 *snap_sk;
-    const int *snap_occ;
+    ChainSnap snap;
     const int64_t *item_off;    // nsys + 1
-    const int2 *groups;         // per workgroup: system, first insertion (64 per group)
     int64_t nitems, nw, cyc0;
     uint64_t widom_seed;
     const double *pose_pos, *pose_quat;
-    double *part;               // gridDim.y x nitems partial energies
+    double *dU;                 // nitems: energy without the insertion constant
 };
 
 }  // namespace pa

@@ -232,7 +232,10 @@ __host__ __device__ inline uint32_t to_fixed(double f)
 constexpr double PAIR_ERFC_XMAX = 4.0;
 constexpr double PAIR_ERFC_XLO = 2.0 * (2.0 / (2.0 + PAIR_ERFC_XMAX)) - 1.0;
 
-__host__ __device__ inline double pair_erfc_f64(double z)
+// FUSED = true evaluates the recurrence with fma (one rounding per step instead of two); the
+// move path and the Widom FP64 path use FUSED = false.
+template <bool FUSED>
+__host__ __device__ inline double pair_erfc_f64_t(double z)
 {
 #pragma clang fp contract(off)
     const double c[17] = {-0.888894912038317, 0.4479385778277769, -0.00021608983881297707,
@@ -244,6 +247,16 @@ __host__ __device__ inline double pair_erfc_f64(double z)
     double t = 2.0 / (2.0 + z);
     double xi = 2.0 * (2.0 * t - 1.0 - PAIR_ERFC_XLO) / (1.0 - PAIR_ERFC_XLO) - 1.0;
     double d = 0.0, dd = 0.0;
+    if (FUSED) {
+        const double x2 = 2.0 * xi;
+#pragma unroll
+        for (int j = 16; j >= 1; --j) {
+            double tmp = d;
+            d = __builtin_fma(x2, d, c[j] - dd);
+            dd = tmp;
+        }
+        return t * exp(__builtin_fma(xi, d, c[0] / 2.0 - dd) - z * z);
+    }
     for (int j = 16; j >= 1; --j) {
         double tmp = d;
         d = 2.0 * xi * d - dd + c[j];
@@ -251,6 +264,8 @@ __host__ __device__ inline double pair_erfc_f64(double z)
     }
     return t * exp(xi * d - dd + c[0] / 2.0 - z * z);
 }
+
+__host__ __device__ inline double pair_erfc_f64(double z) { return pair_erfc_f64_t<false>(z); }
 
 // The reference's Float32 coefficient set (src/ewald.jl:120-123).
 __host__ __device__ constexpr float pair_erfc_c32(int j)

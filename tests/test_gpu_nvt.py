@@ -24,11 +24,11 @@ T = 298.15
 SEED = 5
 
 
-def _porous_system(nsys=1, natoms=40):
+def _porous_system(nsys=1, natoms=40, rep=2, gamma=90):
     ff = synth.synthetic_forcefield(11.0)
     g = synth.co2_guest(ff)
-    uc = synth.synthetic_framework(12.0, 12.0, 12.0, 90, 90, 90, natoms, 5, charge_sd=0.5)
-    fw = pa.replicate(uc, (2, 2, 2))
+    uc = synth.synthetic_framework(12.0, 12.0, 12.0, 90, 90, gamma, natoms, 5, charge_sd=0.5)
+    fw = pa.replicate(uc, (rep, rep, rep))
     ew = pa.EwaldParams(11.0)
     b = pa.FrameworkBatch([fw] * nsys, ff, g, ew, fullk=True)
     return fw, ff, g, ew, b
@@ -57,8 +57,11 @@ def _chains(st, caps, fw, ff, g, ew):
     return o, out
 
 
-def test_chain_energies_match_oracle_in_a_loaded_configuration():
-    fw, ff, g, ew, b = _porous_system()
+@pytest.mark.parametrize("rep,gamma", [(2, 90), (3, 90), (3, 110)])
+def test_chain_energies_match_oracle_in_a_loaded_configuration(rep, gamma):
+    """(3, 3, 3) cells are wide enough that most atom chunks lie beyond the cutoff of an
+    insertion and are skipped whole; the triclinic case checks the distance bound there."""
+    fw, ff, g, ew, b = _porous_system(rep=rep, gamma=gamma)
     st = pa.SystemState(b, g, [8], T=T, seed=SEED, capacities=[20])
     o, (m,) = _chains(st, [20], fw, ff, g, ew)
     tr = pa.mc_moves(st, pa.MOVE_TRANSLATION, 60, T=T, step_trans=0.5, trace=True)[0]

@@ -28,7 +28,7 @@ def test_product_coefficients_equal_the_pinned_literals():
     """The device series (pa_numeric.hpp) uses exactly the literals the oracle's regeneration
     test pins."""
     src = open(os.path.join(ROOT, "pureadsorb.jl_amd", "csrc", "pa_numeric.hpp")).read()
-    block = src[src.index("inline double pair_erfc_f64"):]
+    block = src[src.index("inline double pair_erfc_f64_t"):]
     block = block[block.index("{", block.index("c[17]")) + 1: block.index("};")]
     vals = np.array([float(x) for x in re.findall(r"[-+]?\d[\d.]*e?[-+]?\d*", block.replace("\n", " "))])
     assert np.array_equal(vals, O.erfc_coefficients("pair64"))
