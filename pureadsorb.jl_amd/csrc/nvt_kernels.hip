@@ -73,8 +73,10 @@ __device__ inline double cw_pair(double r2, double sg, double ep, double qq_ke, 
 
 constexpr int CW_RING = 128;  // per-wavefront ring of in-cutoff pairs (< 64 pending + one append)
 
+// Guests of up to three sites fit 4 wavefronts per SIMD without spilling; larger guests keep
+// the compiler's own register budget.
 template <int NS>
-__global__ __launch_bounds__(CW_WG) void chain_widom_kernel(ChainWidomArgs a)
+__global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(ChainWidomArgs a)
 {
     __shared__ double ring_r2[CW_NW][CW_RING], ring_qq[CW_NW][CW_RING];
     __shared__ int ring_code[CW_NW][CW_RING];
