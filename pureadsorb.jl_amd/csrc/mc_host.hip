@@ -683,6 +683,25 @@ extern "C" int pa_audit_energy(pa_state *st, double *discrepancy)
     return ok();
 }
 
+// Zeroes the move counters at the start of production. The audit's "accepted moves since the
+// last audit" (which sizes its rounding tolerance) keeps counting the moves accepted before the
+// reset: the running energy and structure factor carry their rounding until the next audit.
+static int reset_move_counters(pa_state *st)
+{
+    const int nsys = st->nsys;
+    std::vector<int> acc;
+    HIP_TRY(download(acc, st->d_acc, (size_t)nsys * NMOVES));
+    for (int n = 0; n < nsys; ++n) {
+        int nacc = 0;
+        for (int t = 0; t < NMOVES; ++t) nacc += acc[n * NMOVES + t];
+        st->acc_at_audit[n] -= nacc;  // pending = nacc - acc_at_audit carries over as -acc_at_audit
+    }
+    std::vector<int> z((size_t)nsys * NMOVES, 0);
+    HIP_TRY(upload(st->d_acc, z));
+    HIP_TRY(upload(st->d_att, z));
+    return PA_OK;
+}
+
 // block_mean_sem (src/nvt.jl:126-140).
 static void block_mean_sem(const double *x, int64_t ns, int nb, double *mean, double *sem)
 {
@@ -802,10 +821,8 @@ extern "C" int pa_gcmc_run(pa_state *st, const pa_gcmc_params *p, pa_gcmc_result
     for (int phase = 0; phase < 2; ++phase) {
         const int64_t ncyc = phase == 0 ? p->n_warmup : NP;
         if (phase == 1) {
-            std::vector<int> z((size_t)nsys * NMOVES, 0);
-            HIP_TRY(upload(st->d_acc, z));
-            HIP_TRY(upload(st->d_att, z));
-            st->acc_at_audit.assign(nsys, 0);
+            int s = reset_move_counters(st);
+            if (s) return s;
         }
         for (int64_t c0 = 0; c0 < ncyc;) {
             const int64_t len = std::min(ncyc - c0, p->n_audit - since_audit);
@@ -1250,10 +1267,8 @@ extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_para
     for (int phase = 0; phase < 2; ++phase) {
         const int64_t ncyc = phase == 0 ? p->n_warmup : NP;
         if (phase == 1) {
-            std::vector<int> z((size_t)nsys * NMOVES, 0);
-            HIP_TRY(upload(st->d_acc, z));
-            HIP_TRY(upload(st->d_att, z));
-            st->acc_at_audit.assign(nsys, 0);
+            int s = reset_move_counters(st);
+            if (s) return s;
         }
         for (int64_t c0 = 0; c0 < ncyc;) {
             int64_t len = std::min(ncyc - c0, p->n_audit - since_audit);

@@ -194,3 +194,29 @@ def test_nvt_rejects_mismatched_guest_and_bad_arguments():
     with pytest.raises(pa.ArgumentError, match="2\\*nblocks"):
         pa.run_nvt(b, st, g, ff, T=T, n_warmup=0, n_production=3, n_widom_per_cycle=1, n_audit=1,
                    step_trans=0.3, step_rot=0.3, nblocks=2)
+
+
+def test_audit_after_production_start_counts_earlier_moves():
+    """Moves accepted before a driver starts production stay in the running structure factor
+    until the next audit; an audit right after the production counters restart must size its
+    tolerance with them (regression: the audit saw 1 accepted move for a 14-move drift)."""
+    golden = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+    tr = pa.read_forcefield(golden + "/trappe.yaml")
+    co2 = pa.read_guest(golden + "/co2.yaml", tr)
+    sc = pa.replicate(pa.read_cif(golden + "/RUBTAK.cif"), (3, 3, 3))
+    b = pa.FrameworkBatch([sc], tr, co2, pa.EwaldParams(12.0, 1e-6), fullk=True)
+    for driver in ("nvt", "gcmc"):
+        st = pa.SystemState(b, co2, [4], T=T, seed=9, capacities=[16])
+        for mt in (pa.MOVE_TRANSLATION, pa.MOVE_ROTATION, pa.MOVE_REINSERTION):
+            pa.mc_moves(st, mt, 40, T=T, step_trans=0.5, step_rot=0.3)
+        nacc = sum(st.info(0)["accepted"])
+        print(f"{driver}: {nacc} moves accepted before the driver", flush=True)
+        assert nacc >= 10
+        if driver == "nvt":
+            pa.run_nvt(b, st, co2, tr, T=T, n_warmup=1, n_production=4, n_widom_per_cycle=4, n_audit=2,
+                       step_trans=0.5, step_rot=0.3, widom_seed=77, nblocks=2)
+        else:
+            pa.run_gcmc(st, T=T, n_warmup=1, n_production=4, n_audit=2, step_trans=0.5, step_rot=0.3,
+                        fugacity=1e5, nblocks=2)
+        d = pa.audit_energy(st)
+        assert abs(d[0]) < 1e-9
