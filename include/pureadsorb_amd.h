@@ -118,9 +118,10 @@ typedef struct pa_timing {
 typedef struct pa_batch pa_batch;
 
 /* Replaces `FrameworkBatch(fws, ff, guest, ewald; fullk)` (src/batch.jl:237-432): validates,
- * deduplicates frameworks by value, builds every table in FP64 on the host (k-vectors,
- * prefactors, host structure factors, replication check, orientation-averaged guest self
- * term, constant_offset) and uploads them once. Input arrays are copied. */
+ * deduplicates frameworks by value, builds every table in FP64 (k-vectors, prefactors, host
+ * structure factors, replication check, orientation-averaged guest self term,
+ * constant_offset; the structure factors and the self term on the device, or on the host with
+ * PA_BATCH_HOST_ONLY) and uploads them once. Input arrays are copied. */
 int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_forcefield *ff, const pa_guest *guest,
                     const pa_ewald *ewald, uint32_t flags, pa_batch **out);
 void pa_batch_destroy(pa_batch *batch);

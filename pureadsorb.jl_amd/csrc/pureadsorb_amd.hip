@@ -32,6 +32,9 @@ hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, 
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
 hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st);
+hipError_t launch_sk(const double4 *atoms, int natoms, const int4 *kreq, int nreq, double2 *out, hipStream_t st);
+hipError_t launch_self_term(const int4 *kn, const double *kpref, int nk, const double *gsf, const double *q, int ns,
+                            double *partial, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -244,6 +247,81 @@ void structure_factor(const KTable &kt, const std::vector<double> &frac, const s
             Sim[i] = im;
         }
     });
+}
+
+
+// Device scratch of the table builder (reused across frameworks).
+struct BuildScratch {
+    DevAlloc atoms, kreq, sk, kn, kpref, gsf, q, partial;
+};
+
+// structure_factor on the device (batch_kernels.hip): FP64, one work-item per needed k-vector.
+int device_structure_factor(BuildScratch &bs, const KTable &kt, const std::vector<double> &frac,
+                            const std::vector<double> &q, const std::vector<char> &need, std::vector<double> &Sre,
+                            std::vector<double> &Sim)
+{
+    const int64_t nk = (int64_t)kt.kpref.size(), na = (int64_t)q.size();
+    Sre.assign(nk, 0.0);
+    Sim.assign(nk, 0.0);
+    if (nk == 0 || na == 0) return PA_OK;
+    std::vector<int4> kr;
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < nk; ++i)
+        if (need[i]) {
+            kr.push_back(make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0));
+            idx.push_back(i);
+        }
+    if (kr.empty()) return PA_OK;
+    int st = device_ready();
+    if (st) return st;
+    std::vector<double4> at(na);
+    for (int64_t j = 0; j < na; ++j) at[j] = make_double4(frac[3 * j], frac[3 * j + 1], frac[3 * j + 2], q[j]);
+    HIP_TRY(upload(bs.atoms, at));
+    HIP_TRY(upload(bs.kreq, kr));
+    HIP_TRY(bs.sk.ensure(sizeof(double2) * kr.size()));
+    HIP_TRY(launch_sk(bs.atoms.as<double4>(), (int)na, bs.kreq.as<int4>(), (int)kr.size(), bs.sk.as<double2>(), nullptr));
+    std::vector<double2> out(kr.size());
+    HIP_TRY(hipMemcpy(out.data(), bs.sk.p, sizeof(double2) * kr.size(), hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < kr.size(); ++t) {
+        Sre[idx[t]] = out[t].x;
+        Sim[idx[t]] = out[t].y;
+    }
+    return PA_OK;
+}
+
+// The 64-orientation guest self term on the device: samples[o] = KE sum_k kpref |S_g(k; o)|^2,
+// the per-block partial sums added in block order on the host.
+int device_self_term(BuildScratch &bs, const KTable &kt, const double invA[9], const pa_guest *guest,
+                     const std::vector<double> &self_q, double samples[64])
+{
+    const int nk = (int)kt.kpref.size(), NS = guest->nsites;
+    for (int o = 0; o < 64; ++o) samples[o] = 0.0;
+    if (nk == 0) return PA_OK;
+    int st = device_ready();
+    if (st) return st;
+    std::vector<double> gsf((size_t)64 * NS * 3);
+    for (int o = 0; o < 64; ++o)
+        for (int a = 0; a < NS; ++a) {
+            double r[3];
+            rotate(&self_q[4 * o], guest->sites + 3 * a, r);
+            matvec(invA, r, &gsf[3 * ((size_t)o * NS + a)]);
+        }
+    std::vector<int4> kn(nk);
+    for (int i = 0; i < nk; ++i) kn[i] = make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0);
+    const int nb = (nk + 63) / 64;
+    HIP_TRY(upload(bs.kn, kn));
+    HIP_TRY(upload(bs.kpref, kt.kpref));
+    HIP_TRY(upload(bs.gsf, gsf));
+    HIP_TRY(upload(bs.q, std::vector<double>(guest->charges, guest->charges + NS)));
+    HIP_TRY(bs.partial.ensure(sizeof(double) * 64 * nb));
+    HIP_TRY(launch_self_term(bs.kn.as<int4>(), bs.kpref.as<double>(), nk, bs.gsf.as<double>(), bs.q.as<double>(), NS,
+                             bs.partial.as<double>(), nullptr));
+    std::vector<double> part((size_t)64 * nb);
+    HIP_TRY(hipMemcpy(part.data(), bs.partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+    for (int blk = 0; blk < nb; ++blk)
+        for (int o = 0; o < 64; ++o) samples[o] += part[(size_t)blk * 64 + o];
+    for (int o = 0; o < 64; ++o) samples[o] *= KE;
+    return PA_OK;
 }
 
 // 64 fixed guest orientations for the orientation-averaged self term: ChainRNG(0x5e1f, 0, o),
@@ -503,6 +581,10 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
+    // with a device, the structure factors and the self term run there (batch_kernels.hip);
+    // PA_BATCH_HOST_ONLY keeps them on the host
+    const bool host_only = (flags & PA_BATCH_HOST_ONLY) != 0;
+    BuildScratch bscratch;
     for (int u = 0; u < b->nfw; ++u) {
         const pa_framework &F = fws[uniq[u]];
         const int n = F.natoms;
@@ -641,7 +723,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             if (charged)
                 for (int i = 0; i < nkf; ++i) need[i] = (fullk || coupled(i)) ? 1 : 0;
             for (size_t t = 0, taken = 0; t < unc.size() && taken < 32; t += unc_stride, ++taken) need[unc[t]] = 1;
-            structure_factor(kt, frac, q, need, kt.Sre, kt.Sim);
+            if (host_only) {
+                structure_factor(kt, frac, q, need, kt.Sre, kt.Sim);
+            } else {
+                int s = device_structure_factor(bscratch, kt, frac, q, need, kt.Sre, kt.Sim);
+                if (s) return s;
+            }
         }
         if (!rep1) {
             if (!unc.empty()) {
@@ -675,6 +762,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             }
             // orientation-averaged guest self term over the FULL k table (src/batch.jl:360-389)
             double samples[64];
+            if (!host_only) {
+                double invA[9];
+                inv3(D.A, invA);
+                int s = device_self_term(bscratch, kt, invA, guest, self_q, samples);
+                if (s) return s;
+            } else
             parallel_for(64, [&](int64_t ob, int64_t oe) {
                 for (int64_t o = ob; o < oe; ++o) {
                     double gs[MAX_SITES][3];
