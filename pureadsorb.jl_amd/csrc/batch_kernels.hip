@@ -1,6 +1,7 @@
-// batch_kernels.hip -- FrameworkBatch tables on gfx950 (FP64): the host structure factor at the
+// batch_kernels.hip -- FrameworkBatch tables on gfx950: the host structure factor at the
 // k-vectors a framework keeps (src/batch.jl:173-195) and the 64-orientation guest self term
-// over its full k table (src/batch.jl:360-389). k = 2 pi A^-T n, so e^{i k.r} = e^{2 pi i n.f}
+// over its full k table (src/batch.jl:360-389), both FP64, and the phase-0 near-stencil atom
+// lists. k = 2 pi A^-T n, so e^{i k.r} = e^{2 pi i n.f}
 // with f = A^-1 r; every phase is one own-sincospi of 2 n.f.
 #include "pa_device.hpp"
 
@@ -61,6 +62,49 @@ __global__ __launch_bounds__(BK_WG) void self_term_kernel(const int4 *kn, const 
     red[w][lane] = acc;
     __syncthreads();
     if (w == 0) partial[(int64_t)blockIdx.x * 64 + lane] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Phase-0 near-stencil lists: for every hard-core cell, the atoms (corefx, sorted by cell) of
+// its neighbour cells in the order i2, i1, i0, written from the cell's nbr_start offset. One
+// wavefront per cell; cell_fw_off (nfw + 1) maps the global cell index to its framework.
+__global__ __launch_bounds__(BK_WG) void nbr_fill_kernel(const FwDesc *fw, int nfw, const int *cell_fw_off, int ncells,
+                                                        const int *cell_start, const int *nbr_start,
+                                                        const uint4 *corefx, uint4 *nbr_atoms)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * (BK_WG / 64) + (threadIdx.x >> 6);
+    if (g >= ncells) return;
+    int lo = 0, hi = nfw - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cell_fw_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const FwDesc &D = fw[lo];
+    const int c = g - cell_fw_off[lo];
+    const int nc0 = D.cg_nc[0], nc1 = D.cg_nc[1], nc2 = D.cg_nc[2];
+    const int k0 = nc0 < 3 ? nc0 : 3, k1 = nc1 < 3 ? nc1 : 3, k2 = nc2 < 3 ? nc2 : 3;
+    const int h0 = c % nc0, h1 = (c / nc0) % nc1, h2 = c / (nc0 * nc1);
+    const int *cs = cell_start + D.cg_off;
+    const uint4 *src = corefx + D.atom_off;
+    int64_t pos = nbr_start[D.cg_off + c];
+    for (int i2 = 0; i2 < k2; ++i2)
+        for (int i1 = 0; i1 < k1; ++i1)
+            for (int i0 = 0; i0 < k0; ++i0) {
+                const int cc = nbr_cell_axis(h0, i0, k0, nc0) +
+                               nc0 * (nbr_cell_axis(h1, i1, k1, nc1) + nc1 * nbr_cell_axis(h2, i2, k2, nc2));
+                const int b0 = cs[cc], len = cs[cc + 1] - b0;
+                for (int i = lane; i < len; i += 64) nbr_atoms[pos + i] = src[b0 + i];
+                pos += len;
+            }
+}
+
+hipError_t launch_nbr_fill(const FwDesc *fw, int nfw, const int *cell_fw_off, int ncells, const int *cell_start,
+                           const int *nbr_start, const uint4 *corefx, uint4 *nbr_atoms, hipStream_t st)
+{
+    if (ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nbr_fill_kernel, dim3((unsigned)((ncells + BK_WG / 64 - 1) / (BK_WG / 64))), dim3(BK_WG), 0, st, fw,
+                       nfw, cell_fw_off, ncells, cell_start, nbr_start, corefx, nbr_atoms);
+    return hipGetLastError();
 }
 
 hipError_t launch_sk(const double4 *atoms, int natoms, const int4 *kreq, int nreq, double2 *out, hipStream_t st)

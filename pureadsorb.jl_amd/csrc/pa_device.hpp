@@ -146,6 +146,14 @@ struct GradArgs {
     const int4 *kblk;           // per k-space workgroup: system, first k-vector, list range (relative)
 };
 
+// Phase-0 near stencil: the i-th (i < k = min(3, nc)) neighbour along one axis of cell index h,
+// among nc cells (each cell once when the axis has fewer than 3 cells).
+__host__ __device__ inline int nbr_cell_axis(int h, int i, int k, int nc)
+{
+    const int f = k == nc ? 0 : (h - 1 + nc) % nc;
+    return (f + i) % nc;
+}
+
 // One site pair of the move path (FP64): Lennard-Jones inside rc_lj plus the screened Coulomb
 // qq_ke * erfc(alpha r)/r inside rc_ew (qq_ke = KE q_a q_b), on a minimum-image separation r^2.
 __host__ __device__ inline double site_pair_energy(double r2, double sg, double ep, double qq_ke, double alpha,

@@ -33,6 +33,8 @@ hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const dou
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
 hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st);
 hipError_t launch_sk(const double4 *atoms, int natoms, const int4 *kreq, int nreq, double2 *out, hipStream_t st);
+hipError_t launch_nbr_fill(const FwDesc *fw, int nfw, const int *cell_fw_off, int ncells, const int *cell_start,
+                           const int *nbr_start, const uint4 *corefx, uint4 *nbr_atoms, hipStream_t st);
 hipError_t launch_self_term(const int4 *kn, const double *kpref, int nk, const double *gsf, const double *q, int ns,
                             double *partial, hipStream_t st);
 }  // namespace pa
@@ -577,7 +579,8 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<uint4> corefx;
     std::vector<int> coreidx, cell_start;
     std::vector<int> nbr_start;   // per framework cell (cell_start layout): first entry in nbr_atoms
-    std::vector<uint4> nbr_atoms;
+    int64_t nbr_total = 0;        // entries of nbr_atoms (filled on the device from corefx)
+    std::vector<int> cell_fw_off(1, 0);  // prefix over frameworks of their cell counts
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
@@ -656,21 +659,33 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             // near test reads one range per site
             const int nc0 = D.cg_nc[0], nc1 = D.cg_nc[1], nc2 = D.cg_nc[2];
             const int k0 = std::min(3, nc0), k1 = std::min(3, nc1), k2 = std::min(3, nc2);
-            for (int c = 0; c < nct; ++c) {
-                if (nbr_atoms.size() > (size_t)INT32_MAX)
-                    return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
-                nbr_start.push_back((int)nbr_atoms.size());
-                const int h0 = c % nc0, h1 = (c / nc0) % nc1, h2 = c / (nc0 * nc1);
-                auto first = [](int h, int k, int nc) { return k == nc ? 0 : (h - 1 + nc) % nc; };
-                const int f0 = first(h0, k0, nc0), f1 = first(h1, k1, nc1), f2 = first(h2, k2, nc2);
-                for (int i2 = 0; i2 < k2; ++i2)
-                    for (int i1 = 0; i1 < k1; ++i1)
-                        for (int i0 = 0; i0 < k0; ++i0) {
-                            const int cc = (f0 + i0) % nc0 + nc0 * ((f1 + i1) % nc1 + nc1 * ((f2 + i2) % nc2));
-                            nbr_atoms.insert(nbr_atoms.end(), sorted.begin() + cnt[cc], sorted.begin() + cnt[cc + 1]);
-                        }
-            }
-            nbr_start.push_back((int)nbr_atoms.size());
+            // offsets only: nbr_fill_kernel copies the lists on the device (nbr_cells order)
+            auto axis_nbrs = [](int nc, int k) {
+                std::vector<int> t((size_t)nc * k);
+                for (int h = 0; h < nc; ++h)
+                    for (int i = 0; i < k; ++i) t[(size_t)h * k + i] = nbr_cell_axis(h, i, k, nc);
+                return t;
+            };
+            const std::vector<int> nb0 = axis_nbrs(nc0, k0), nb1 = axis_nbrs(nc1, k1), nb2 = axis_nbrs(nc2, k2);
+            for (int h2 = 0; h2 < nc2; ++h2)
+                for (int h1 = 0; h1 < nc1; ++h1)
+                    for (int h0 = 0; h0 < nc0; ++h0) {
+                        if (nbr_total > (int64_t)INT32_MAX)
+                            return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
+                        nbr_start.push_back((int)nbr_total);
+                        for (int i2 = 0; i2 < k2; ++i2)
+                            for (int i1 = 0; i1 < k1; ++i1) {
+                                const int row = nc0 * (nb1[(size_t)h1 * k1 + i1] + nc1 * nb2[(size_t)h2 * k2 + i2]);
+                                for (int i0 = 0; i0 < k0; ++i0) {
+                                    const int cc = nb0[(size_t)h0 * k0 + i0] + row;
+                                    nbr_total += cnt[cc + 1] - cnt[cc];
+                                }
+                            }
+                    }
+            if (nbr_total > (int64_t)INT32_MAX)
+                return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
+            nbr_start.push_back((int)nbr_total);
+            cell_fw_off.push_back(cell_fw_off.back() + nct);
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
         FwGradHost &gh = b->fgh[u];
@@ -944,7 +959,15 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     HIP_TRY(upload(b->d_coreidx, coreidx));
     HIP_TRY(upload(b->d_cell_start, cell_start));
     HIP_TRY(upload(b->d_nbr_start, nbr_start));
-    HIP_TRY(upload(b->d_nbr_atoms, nbr_atoms));
+    HIP_TRY(b->d_nbr_atoms.ensure(sizeof(uint4) * (size_t)std::max<int64_t>(1, nbr_total)));
+    {
+        DevAlloc d_cfo;
+        HIP_TRY(upload(d_cfo, cell_fw_off));
+        HIP_TRY(launch_nbr_fill(b->d_fw.as<FwDesc>(), b->nfw, d_cfo.as<int>(), cell_fw_off.back(),
+                                b->d_cell_start.as<int>(), b->d_nbr_start.as<int>(), b->d_corefx.as<uint4>(),
+                                b->d_nbr_atoms.as<uint4>(), nullptr));
+        HIP_TRY(hipDeviceSynchronize());
+    }
     *out = b.release();
     return ok();
 }
