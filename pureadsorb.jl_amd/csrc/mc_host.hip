@@ -289,6 +289,33 @@ McArgs base_mc(pa_state *st)
     return a;
 }
 
+// Runs fn(n) for every system on the host threads; returns the status of the lowest-numbered
+// failing system with its message (what a serial loop stopping at the first failure reports).
+template <class F>
+int run_per_system(int nsys, F &&fn)
+{
+    std::vector<int> code(nsys, 0);
+    std::vector<std::string> msg(nsys);
+    parallel_for(
+        nsys,
+        [&](int64_t n0, int64_t n1) {
+            for (int64_t n = n0; n < n1; ++n) {
+                const int r = fn((int)n);
+                if (r) {
+                    char buf[1024];
+                    pa_last_error(buf, sizeof buf);
+                    code[n] = r;
+                    msg[n] = buf;
+                    return;
+                }
+            }
+        },
+        2);
+    for (int n = 0; n < nsys; ++n)
+        if (code[n]) return fail(code[n], "%s", msg[n].c_str());
+    return PA_OK;
+}
+
 int check_temperature(double T)
 {
     if (!(T > 0)) return fail(PA_ERR_ARG, "T=%g must be a positive temperature in K", T);
@@ -351,7 +378,8 @@ extern "C" int pa_state_create(pa_batch *b, const pa_guest *guest, const int32_t
     S->const_p.resize(nsys);
     S->const_q.resize(nsys);
     const double kT = KB * T;
-    for (int n = 0; n < nsys; ++n) {
+    // systems are independent: built on the host threads, each keeping its status and message
+    auto build_system = [&](int n) -> int {
         const int u = b->fw_of[n];
         const FwDesc &F = b->desc[u];
         S->seed[n] = splitmix64(seed, (uint64_t)(n + 1));
@@ -406,7 +434,10 @@ extern "C" int pa_state_create(pa_batch *b, const pa_guest *guest, const int32_t
         };
         S->const_p[n] = term(0.0);
         S->const_q[n] = term(1.0) - S->const_p[n];
-    }
+        return PA_OK;
+    };
+    int s_err = run_per_system(nsys, build_system);
+    if (s_err) return s_err;
     std::vector<int> occ(ncounts, ncounts + nsys), zeros5((size_t)nsys * NMOVES, 0), zi(nsys, 0);
     std::vector<uint64_t> ctr(nsys, 0);
     HIP_TRY(upload(S->d_guest_off, S->guest_off));
@@ -593,13 +624,11 @@ static int state_snapshot(pa_state *st, std::vector<double4> &pos, std::vector<d
     HIP_TRY(download(occ, st->d_occ, st->nsys));
     sk_rebuilt.assign(st->nsys, {});
     E.assign(st->nsys, 0.0);
-    for (int n = 0; n < st->nsys; ++n) {
+    return run_per_system(st->nsys, [&](int n) {
         const int u = st->b->fw_of[n];
         rebuild_sk(st->b, u, pos, quat, st->guest_off[n], occ[n], sk_rebuilt[n]);
-        int s = total_energy_h(st->b, u, pos, quat, st->guest_off[n], occ[n], sk_rebuilt[n], &E[n], nullptr);
-        if (s) return s;
-    }
-    return PA_OK;
+        return total_energy_h(st->b, u, pos, quat, st->guest_off[n], occ[n], sk_rebuilt[n], &E[n], nullptr);
+    });
 }
 
 extern "C" int pa_total_energy(pa_state *st, double *E)

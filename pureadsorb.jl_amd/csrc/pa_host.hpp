@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pa_device.hpp"
@@ -64,6 +65,26 @@ struct FwGradHost {
     int rep[3] = {1, 1, 1};
     bool image_order = true;     // atoms are `replicate` images of the first natoms/nimg (image-major)
 };
+
+// Runs fn(begin, end) over [0, n) split across up to 16 host threads (serially below
+// min_parallel items).
+template <class F>
+void parallel_for(int64_t n, F &&fn, int64_t min_parallel = 64)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < min_parallel || nt == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    int64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t b = (int64_t)t * per, e = std::min(n, b + per);
+        if (b >= e) break;
+        th.emplace_back([&fn, b, e] { fn(b, e); });
+    }
+    for (auto &x : th) x.join();
+}
 
 template <class T>
 inline hipError_t upload(DevAlloc &d, const std::vector<T> &v)

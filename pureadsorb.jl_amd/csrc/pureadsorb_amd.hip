@@ -144,23 +144,6 @@ double ewald_alpha(double cutoff, double precision)
     return (lo + hi) / 2.0 / cutoff;
 }
 
-template <class F>
-void parallel_for(int64_t n, F &&fn)
-{
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 64 || nt == 1) {
-        fn(0, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    int64_t per = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        int64_t b = (int64_t)t * per, e = std::min(n, b + per);
-        if (b >= e) break;
-        th.emplace_back([&fn, b, e] { fn(b, e); });
-    }
-    for (auto &x : th) x.join();
-}
 
 // k-vector table of one stored cell (src/ewald.jl:142-157), host structure factor
 // (src/ewald.jl:159-170) and prefactors (src/ewald.jl:36-39).
