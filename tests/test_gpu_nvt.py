@@ -220,3 +220,24 @@ def test_audit_after_production_start_counts_earlier_moves():
                         fugacity=1e5, nblocks=2)
         d = pa.audit_energy(st)
         assert abs(d[0]) < 1e-9
+
+
+def test_long_nvt_chain_passes_the_audit():
+    """test/nvt_tests.jl:255-275: 1,500 cycles x 6 attempts on RUBTAK 3x3x3 with one audit on the
+    last cycle; the chain's running energy and structure factor stay within the audit's
+    tolerance over ~9,000 attempted moves."""
+    import os
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    tr = pa.read_forcefield(os.path.join(golden, "trappe.yaml"))
+    co2 = pa.read_guest(os.path.join(golden, "co2.yaml"), tr)
+    sc = pa.replicate(pa.read_cif(os.path.join(golden, "RUBTAK.cif")), (3, 3, 3))
+    b = pa.FrameworkBatch([sc], tr, co2, pa.EwaldParams(12.0, 1e-6), fullk=True)
+    st = pa.SystemState(b, co2, [6], T=T, seed=3)
+    r = pa.run_nvt(b, st, co2, tr, T=T, n_warmup=0, n_production=1500, n_widom_per_cycle=5, n_audit=1500,
+                   step_trans=0.3, step_rot=0.4, widom_seed=1003, nblocks=5)[0]
+    print(f"1500 cycles: energy {r.energy:.6f} +- {r.energy_err:.6f} eV, mu_ex {r.mu_ex:.6f} eV, "
+          f"acceptance {r.acceptance}", flush=True)
+    assert np.isfinite(r.energy) and np.isfinite(r.mu_ex) and np.isfinite(r.q_st)
+    assert all(0.0 <= x <= 1.0 for x in r.acceptance)
+    assert sum(st.info(0)["attempted"]) == 1500 * 6
+    assert abs(pa.audit_energy(st)[0]) < 1e-9
