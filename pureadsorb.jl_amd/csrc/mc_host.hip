@@ -52,6 +52,8 @@ struct pa_state {
 
 namespace {
 
+int ensure_chain_tables(pa_state *st);
+
 // ---- FP64 host energies (the same terms the kernel sums, in one serial order) ----------------
 
 void guest_sites_h(const pa_batch *b, const double p[3], const double q[4], double out[][3])
@@ -254,6 +256,8 @@ McArgs base_mc(pa_state *st)
     a.fw_of = b->d_fw_of.as<int>();
     a.atom64 = b->d_atom64.as<double4>();
     a.atype = b->d_atype.as<int>();
+    a.atomf = st->d_atomf64.as<double4>();
+    a.atypef = st->d_atype_s.as<int>();
     a.k64 = b->d_k64.as<double4>();
     a.kn = b->d_kn.as<int4>();
     a.ktab_w = 0;
@@ -464,6 +468,8 @@ extern "C" int pa_state_create(pa_batch *b, const pa_guest *guest, const int32_t
     HIP_TRY(upload(S->d_step_r, zd));
     HIP_TRY(upload(S->d_fug, zd));
     S->acc_at_audit.assign(nsys, 0);
+    int s_tab = ensure_chain_tables(S.get());
+    if (s_tab) return s_tab;
     *out = S.release();
     return ok();
 }

@@ -6,7 +6,8 @@
 // chain's counter-based stream (select_and_propose, src/moves.jl:275-288), then the workgroup
 // splits the three sums a move needs across its lanes:
 //   * the new pose's host-guest real-space energy over the framework atoms
-//     (host_guest_realspace_energy_range, src/guest.jl:176-200),
+//     (host_guest_realspace_energy_range, src/guest.jl:176-200), pairs inside the cutoff
+//     compacted through LDS before the LJ + erfc evaluation (fused multiply-adds, FP64),
 //   * the guest-guest real-space change over (other guest, site, site) triples
 //     (guest_guest_move_delta / guest_pair_realspace_energy_range, src/guest.jl:112-156),
 //   * the reciprocal change over the k-vectors against the running structure factor
@@ -52,11 +53,15 @@ __device__ inline double mi_r2(const FwDesc &F, const double a[3], const double 
     return m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
 }
 
+constexpr int MC_RING = 128;  // per-wavefront ring of in-cutoff host-guest pairs
+
 template <int NS>
 __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
 {
 #pragma clang fp contract(off)
     __shared__ double red[3][MC_NW];
+    __shared__ double ring_r2[MC_NW][MC_RING], ring_qq[MC_NW][MC_RING];
+    __shared__ int ring_code[MC_NW][MC_RING];
     // per group: the attempt's slot, acceptance uniform and proposed pose
     __shared__ int sp_slot[MC_SPEC];
     __shared__ double sp_u[MC_SPEC], sp_p[MC_SPEC][3], sp_q[MC_SPEC][4];
@@ -65,8 +70,6 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
     const int n = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const FwDesc &F = a.fw[a.fw_of[n]];
     const int gr0 = a.guest_off[n], cap = a.guest_off[n + 1] - gr0, nslots = a.guest_off[a.nsys];
-    const double4 *atoms = a.atom64 + F.atom_off;
-    const int *types = a.atype + F.atom_off;
     const double4 *ks = a.k64 + F.k_off;
     double2 *Sk = a.Sk + a.sk_off[n];
     double2 *dS = a.dS + (size_t)MC_SPEC * a.sk_off[n];   // MC_SPEC slices of nk
@@ -207,23 +210,58 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             for (int i = 0; i < 4; ++i) sp_q[gi][i] = nq[i];
         }
 
-        // (1) host-guest real space of the new pose
+        // (1) host-guest real space of the new pose: minimum images in fractional coordinates;
+        // the (site, atom) pairs inside the cutoff go through the wavefront's ring in LDS and are
+        // evaluated 64 at a time, so LJ + erfc run on full wavefronts
         double hsum = 0.0;
-        if (active && has_new)
+        if (active && has_new) {
+            double sfn[NS][3];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) matvec(F.invA, ns[s], sfn[s]);
+            double *qr2 = ring_r2[wid], *qqq = ring_qq[wid];
+            int *qcode = ring_code[wid];
+            unsigned head = 0, tail = 0;
+            auto drain = [&](unsigned cnt) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if ((unsigned)lane < cnt) {
+                    const unsigned i = (head + lane) & (MC_RING - 1);
+                    const double2 p = a.pair64[qcode[i]];
+                    hsum += cw_pair(qr2[i], p.x, p.y, qqq[i], alpha, a.rc_lj2, a.rc_ew2);
+                }
+                head += cnt;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            };
+            const double4 *af = a.atomf + F.atom_off;
+            const int *tf = a.atypef + F.atom_off;
 #pragma unroll 1
-            for (int j = gt; j < natoms; j += MC_GW) {
-                const double4 h = atoms[j];
-                const double hp[3] = {h.x, h.y, h.z};
-                const int ht = types[j];
+            for (int jb = gt - lane; jb < natoms; jb += MC_GW) {
+                const int j = jb + lane;
+                const bool valid = j < natoms;
+                const double4 h = valid ? af[j] : make_double4(0, 0, 0, 0);
+                const int ht = valid ? tf[j] : 0;
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    const double r2 = mi_r2(F, ns[s], hp);
-                    if (r2 < a.rc_lj2 || r2 < a.rc_ew2) {
-                        const double2 p = a.pair64[s * a.ntypes_c + ht];
-                        hsum += site_pair_energy(r2, p.x, p.y, KE * G.q[s] * h.w, alpha, a.rc_lj2, a.rc_ew2);
+                    const double r2 = cw_r2(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
+                    const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
+                    const uint64_t mask = __ballot(in);
+                    if (in) {
+                        const unsigned below = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        const unsigned i = (tail + below) & (MC_RING - 1);
+                        qr2[i] = r2;
+                        qqq[i] = KE * G.q[s] * h.w;
+                        qcode[i] = s * a.ntypes_c + ht;
                     }
+                    tail += (unsigned)__popcll(mask);
+                    if (tail - head >= 64) drain(64);
                 }
             }
+            drain(tail - head);
+        }
         // (2) guest-guest real space: the moving (or inserted, or deleted) guest against every
         // other live guest, one (other guest, site, site) triple per lane step
         double gsum = 0.0;
@@ -437,7 +475,7 @@ hipError_t launch_mc(const McArgs &a, hipStream_t st)
     const int ns = a.guest.nsites;
     // the per-axis phase tables: old and new pose, every site, kn_max[d] + 1 powers per axis
     const size_t lds = a.guest.charged ? (size_t)MC_SPEC * 2 * ns * a.ktab_w * sizeof(double2) : 0;
-    if (lds > 96 * 1024) return hipErrorInvalidValue;
+    if (lds > 72 * 1024) return hipErrorInvalidValue;  // + ~21 KB of static LDS (rings, reductions)
     switch (ns) {
     case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, lds, st, a); break;
     case 2: hipLaunchKernelGGL(mc_chain_kernel<2>, grid, block, lds, st, a); break;

@@ -38,39 +38,6 @@ __device__ inline double cw_wave_sum(double v)
     return v;
 }
 
-// The insertion energies need FP64 accuracy (~1e-14 relative), not bit-for-bit agreement with
-// another code path, so the helpers below fuse multiply-adds (the shared helpers keep every
-// rounding step of the reference's formulas for the paths that need exact agreement).
-// r^2 of the cell vector A (f - rint(f)) for a fractional offset f.
-__device__ inline double cw_r2(const double A[9], double f0, double f1, double f2)
-{
-    f0 -= __builtin_rint(f0);
-    f1 -= __builtin_rint(f1);
-    f2 -= __builtin_rint(f2);
-    const double m0 = __builtin_fma(A[0], f0, __builtin_fma(A[3], f1, A[6] * f2));
-    const double m1 = __builtin_fma(A[1], f0, __builtin_fma(A[4], f1, A[7] * f2));
-    const double m2 = __builtin_fma(A[2], f0, __builtin_fma(A[5], f1, A[8] * f2));
-    return __builtin_fma(m0, m0, __builtin_fma(m1, m1, m2 * m2));
-}
-
-// LJ inside rc_lj plus qq_ke erfc(alpha r)/r inside rc_ew (site_pair_energy with one division
-// for 1/r shared by both terms).
-__device__ inline double cw_pair(double r2, double sg, double ep, double qq_ke, double alpha, double rc_lj2,
-                                 double rc_ew2)
-{
-    const double r = __builtin_sqrt(r2);
-    const double ir = 1.0 / r;
-    double e = 0.0;
-    if (r2 < rc_lj2) {
-        const double s = sg * ir;
-        double x = s * s;
-        x = x * x * x;
-        e = 4.0 * ep * __builtin_fma(x, x, -x);
-    }
-    if (r2 < rc_ew2) e = __builtin_fma(qq_ke * pair_erfc_f64_t<true>(alpha * r), ir, e);
-    return e;
-}
-
 constexpr int CW_RING = 128;  // per-wavefront ring of in-cutoff pairs (< 64 pending + one append)
 
 // Guests of up to three sites fit 4 wavefronts per SIMD without spilling; larger guests keep

@@ -146,6 +146,40 @@ struct GradArgs {
     const int4 *kblk;           // per k-space workgroup: system, first k-vector, list range (relative)
 };
 
+// Energy sums that need FP64 accuracy (~1e-14 relative) but no bit-for-bit agreement with
+// another code path (test insertions into chains, the move path's host-guest sum) use the
+// helpers below, which fuse multiply-adds; the helpers above keep every rounding step of the
+// reference's formulas for the paths that need exact agreement.
+// r^2 of the cell vector A (f - rint(f)) for a fractional offset f.
+__device__ inline double cw_r2(const double A[9], double f0, double f1, double f2)
+{
+    f0 -= __builtin_rint(f0);
+    f1 -= __builtin_rint(f1);
+    f2 -= __builtin_rint(f2);
+    const double m0 = __builtin_fma(A[0], f0, __builtin_fma(A[3], f1, A[6] * f2));
+    const double m1 = __builtin_fma(A[1], f0, __builtin_fma(A[4], f1, A[7] * f2));
+    const double m2 = __builtin_fma(A[2], f0, __builtin_fma(A[5], f1, A[8] * f2));
+    return __builtin_fma(m0, m0, __builtin_fma(m1, m1, m2 * m2));
+}
+
+// LJ inside rc_lj plus qq_ke erfc(alpha r)/r inside rc_ew (site_pair_energy with one division
+// for 1/r shared by both terms).
+__device__ inline double cw_pair(double r2, double sg, double ep, double qq_ke, double alpha, double rc_lj2,
+                                 double rc_ew2)
+{
+    const double r = __builtin_sqrt(r2);
+    const double ir = 1.0 / r;
+    double e = 0.0;
+    if (r2 < rc_lj2) {
+        const double s = sg * ir;
+        double x = s * s;
+        x = x * x * x;
+        e = 4.0 * ep * __builtin_fma(x, x, -x);
+    }
+    if (r2 < rc_ew2) e = __builtin_fma(qq_ke * pair_erfc_f64_t<true>(alpha * r), ir, e);
+    return e;
+}
+
 // Phase-0 near stencil: the i-th (i < k = min(3, nc)) neighbour along one axis of cell index h,
 // among nc cells (each cell once when the axis has fewer than 3 cells).
 __host__ __device__ inline int nbr_cell_axis(int h, int i, int k, int nc)
@@ -225,6 +259,8 @@ struct McArgs {
     const int *fw_of;
     const double4 *atom64;
     const int *atype;
+    const double4 *atomf;       // fractional x, y, z and charge (the test-insertion tables' order)
+    const int *atypef;          // compact types in the same order
     const double4 *k64;         // kx, ky, kz, kprefactor
     const int4 *kn;             // n of k = 2 pi A^-T n, same order as k64
     int ktab_w;                 // max over frameworks of kn_max[0] + kn_max[1] + kn_max[2] + 3
