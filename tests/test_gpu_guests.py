@@ -1,6 +1,7 @@
 """GPU parity for guests of every supported site count (the kernels are instantiated for 1 to 6
 sites): FP64 and mixed insertion energies against the oracle, phase-0 flags, the mixed Widom
-result against the FP64 path, and move traces of the FP64 move kernel against the oracle chain.
+result against the FP64 path, move traces of the FP64 move kernel against the oracle chain, and
+test insertions into the loaded chain plus a short NVT run against the oracle's.
 
 Bars as in test_gpu_widom.py and test_gpu_mc.py: FP64 energies within 1e-11 relative; mixed
 energies within 1e-5 relative below 1e3 eV (3e-4 on sub-angstrom contacts); every insertion the
@@ -108,6 +109,24 @@ def test_guest_move_traces(ns):
         # an uncharged guest keeps no structure factor on the device
         assert np.allclose(st.sk(0), m.sk(), rtol=0, atol=1e-9)
     assert np.all(np.abs(pa.audit_energy(st)) < 1e-8)
+    # test insertions into the loaded configuration and a short NVT run, against the oracle chain
+    rng = np.random.default_rng(ns)
+    pos = rng.random((100, 3)) @ o.cell.T
+    qq = rng.normal(size=(100, 4))
+    qq /= np.linalg.norm(qq, axis=1)[:, None]
+    dU = pa.widom_chain_energies(st, 0, pos, qq)
+    refc = np.array([m.chain_insertion(x, y) for x, y in zip(pos, qq)])
+    fin = np.isfinite(refc) & (np.abs(refc) < 1e8)
+    assert fin.sum() > 50
+    assert np.all(np.abs(dU[fin] - refc[fin]) <= 1e-9 * (1 + np.abs(refc[fin]))), ns
+    kw = dict(T=T, n_warmup=1, n_production=4, n_widom_per_cycle=8, n_audit=2, step_trans=0.3, step_rot=0.2,
+              widom_seed=5, nblocks=2)
+    r = pa.run_nvt(b, st, g, ff, **kw)[0]
+    rf = O.nvt_run([m], **kw)[0]
+    _, sw, _ = pa.nvt_samples(st, 0, 4)
+    print(f"{ns} sites: NVT mu_ex {r.mu_ex:.9f} (oracle {rf['mu_ex']:.9f}), acceptance {r.acceptance}", flush=True)
+    assert r.acceptance == rf["acceptance"]
+    assert np.allclose(sw, rf["sW"], rtol=1e-8, atol=1e-300)
 
 
 @pytest.mark.parametrize("ns", [1, 2, 5])
