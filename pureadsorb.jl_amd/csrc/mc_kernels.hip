@@ -237,12 +237,18 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             };
             const double4 *af = a.atomf + F.atom_off;
             const int *tf = a.atypef + F.atom_off;
+            double4 h_next = gt < natoms ? af[gt] : make_double4(0, 0, 0, 0);
+            int ht_next = gt < natoms ? tf[gt] : 0;
 #pragma unroll 1
             for (int jb = gt - lane; jb < natoms; jb += MC_GW) {
                 const int j = jb + lane;
                 const bool valid = j < natoms;
-                const double4 h = valid ? af[j] : make_double4(0, 0, 0, 0);
-                const int ht = valid ? tf[j] : 0;
+                const double4 h = h_next;
+                const int ht = ht_next;
+                if (j + MC_GW < natoms) {
+                    h_next = af[j + MC_GW];
+                    ht_next = tf[j + MC_GW];
+                }
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     const double r2 = cw_r2(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
@@ -299,11 +305,27 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                 }
             __syncthreads();
             const int4 *kn = a.kn + F.k_off;
-            if (active)
+            if (active) {
+                // one chain's attempt is latency-bound: the next k-vector's loads are issued
+                // before the current one is evaluated
+                double kw_next = 0.0;
+                int4 nn_next = make_int4(0, 0, 0, 0);
+                double2 S_next = make_double2(0.0, 0.0);
+                if (gt < nk) {
+                    kw_next = ks[gt].w;
+                    nn_next = kn[gt];
+                    S_next = Sk[gt];
+                }
 #pragma unroll 1
                 for (int k = gt; k < nk; k += MC_GW) {
-                    const double4 kv = ks[k];
-                    const int4 nn = kn[k];
+                    const double kw = kw_next;
+                    const int4 nn = nn_next;
+                    const double2 S = S_next;
+                    if (k + MC_GW < nk) {
+                        kw_next = ks[k + MC_GW].w;
+                        nn_next = kn[k + MC_GW];
+                        S_next = Sk[k + MC_GW];
+                    }
                     const int ax = abs(nn.x), ay = abs(nn.y), az = abs(nn.z);
                     double sre = 0.0, sim = 0.0;
 #pragma unroll
@@ -318,10 +340,10 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                             sim += sg * G.q[s] * cc.y;
                         }
                     }
-                    const double2 S = Sk[k];
                     mydS[k] = make_double2(sre, sim);
-                    rsum += kv.w * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
+                    rsum += kw * (2.0 * (S.x * sre + S.y * sim) + (sre * sre + sim * sim));
                 }
+            }
         }
         hsum = wave_sum_d(hsum);
         gsum = wave_sum_d(gsum);
@@ -415,10 +437,23 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             ++acc[ty - 1];
             if (G.charged) {
                 const double2 *fdS = dS + (size_t)first * nk;
+                double2 dd_next = make_double2(0.0, 0.0), S_next = make_double2(0.0, 0.0);
+                double sa_next = 0.0;
+                if (tid < nk) {
+                    dd_next = fdS[tid];
+                    S_next = Sk[tid];
+                    sa_next = skabs[tid];
+                }
                 for (int k = tid; k < nk; k += MC_WG) {
-                    const double2 dd = fdS[k], S = Sk[k];
+                    const double2 dd = dd_next, S = S_next;
+                    const double sa = sa_next;
+                    if (k + MC_WG < nk) {  // other entries than the ones written below
+                        dd_next = fdS[k + MC_WG];
+                        S_next = Sk[k + MC_WG];
+                        sa_next = skabs[k + MC_WG];
+                    }
                     Sk[k] = make_double2(S.x + dd.x, S.y + dd.y);
-                    skabs[k] += ds_bound;
+                    skabs[k] = sa + ds_bound;
                 }
             }
         }
