@@ -258,6 +258,7 @@ McArgs base_mc(pa_state *st)
     a.atype = b->d_atype.as<int>();
     a.atomf = st->d_atomf64.as<double4>();
     a.atypef = st->d_atype_s.as<int>();
+    a.upper = b->upper_all ? 1 : 0;
     a.k64 = b->d_k64.as<double4>();
     a.kn = b->d_kn.as<int4>();
     a.ktab_w = 0;
@@ -1105,6 +1106,7 @@ ChainWidomArgs base_chain_widom(pa_state *st)
     a.round_off = st->d_round_off.as<int>();
     a.half_perp = st->d_half_perp.as<double>();
     a.cull_extra = st->cull_extra;
+    a.upper = b->upper_all ? 1 : 0;
     a.k64 = b->d_k64.as<double4>();
     a.kn = b->d_kn.as<int4>();
     a.pair64 = b->d_pair64.as<double2>();

@@ -55,7 +55,7 @@ __device__ inline double mi_r2(const FwDesc &F, const double a[3], const double 
 
 constexpr int MC_RING = 128;  // per-wavefront ring of in-cutoff host-guest pairs
 
-template <int NS>
+template <int NS, bool UPPER>
 __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
 {
 #pragma clang fp contract(off)
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    const double r2 = cw_r2(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
+                    const double r2 = cw_r2_t<UPPER>(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
                     const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
                     const uint64_t mask = __ballot(in);
                     if (in) {
@@ -511,13 +511,19 @@ hipError_t launch_mc(const McArgs &a, hipStream_t st)
     // the per-axis phase tables: old and new pose, every site, kn_max[d] + 1 powers per axis
     const size_t lds = a.guest.charged ? (size_t)MC_SPEC * 2 * ns * a.ktab_w * sizeof(double2) : 0;
     if (lds > 72 * 1024) return hipErrorInvalidValue;  // + ~21 KB of static LDS (rings, reductions)
-    switch (ns) {
-    case 1: hipLaunchKernelGGL(mc_chain_kernel<1>, grid, block, lds, st, a); break;
-    case 2: hipLaunchKernelGGL(mc_chain_kernel<2>, grid, block, lds, st, a); break;
-    case 3: hipLaunchKernelGGL(mc_chain_kernel<3>, grid, block, lds, st, a); break;
-    case 4: hipLaunchKernelGGL(mc_chain_kernel<4>, grid, block, lds, st, a); break;
-    case 5: hipLaunchKernelGGL(mc_chain_kernel<5>, grid, block, lds, st, a); break;
-    case 6: hipLaunchKernelGGL(mc_chain_kernel<6>, grid, block, lds, st, a); break;
+    switch (ns * 2 + (a.upper ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((mc_chain_kernel<1, false>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((mc_chain_kernel<1, true>), grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((mc_chain_kernel<2, false>), grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL((mc_chain_kernel<2, true>), grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((mc_chain_kernel<3, false>), grid, block, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((mc_chain_kernel<3, true>), grid, block, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((mc_chain_kernel<4, false>), grid, block, lds, st, a); break;
+    case 9: hipLaunchKernelGGL((mc_chain_kernel<4, true>), grid, block, lds, st, a); break;
+    case 10: hipLaunchKernelGGL((mc_chain_kernel<5, false>), grid, block, lds, st, a); break;
+    case 11: hipLaunchKernelGGL((mc_chain_kernel<5, true>), grid, block, lds, st, a); break;
+    case 12: hipLaunchKernelGGL((mc_chain_kernel<6, false>), grid, block, lds, st, a); break;
+    case 13: hipLaunchKernelGGL((mc_chain_kernel<6, true>), grid, block, lds, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -42,7 +42,7 @@ constexpr int CW_RING = 128;  // per-wavefront ring of in-cutoff pairs (< 64 pen
 
 // Guests of up to three sites fit 4 wavefronts per SIMD without spilling; larger guests keep
 // the compiler's own register budget.
-template <int NS>
+template <int NS, bool UPPER>
 __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(ChainWidomArgs a)
 {
     __shared__ double ring_r2[CW_NW][CW_RING], ring_qq[CW_NW][CW_RING];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
 #pragma unroll 1
         for (int c = c0; c < c1; ++c) {
             const double4 cc = a.chunk[c];
-            const double dist = fmin(__builtin_sqrt(cw_r2(F.A, pf[0] - cc.x, pf[1] - cc.y, pf[2] - cc.z)), halfw);
+            const double dist = fmin(__builtin_sqrt(cw_r2_t<UPPER>(F.A, pf[0] - cc.x, pf[1] - cc.y, pf[2] - cc.z)), halfw);
             if (dist > cc.w + a.cull_extra) continue;
             const int j = (c - c0) * 64 + lane;
             const bool valid = j < F.natoms;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
             const int ht = valid ? types[j] : 0;
 #pragma unroll
             for (int x = 0; x < NS; ++x) {
-                const double r2 = cw_r2(F.A, sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z);
+                const double r2 = cw_r2_t<UPPER>(F.A, sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z);
                 const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
                 const uint64_t mask = __ballot(in);
                 if (in) {
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
         const double pb[3] = {p4.x + r[0], p4.y + r[1], p4.z + r[2]};
         double fb[3];
         matvec(F.invA, pb, fb);
-        const double r2 = cw_r2(F.A, sf[sa][0] - fb[0], sf[sa][1] - fb[1], sf[sa][2] - fb[2]);
+        const double r2 = cw_r2_t<UPPER>(F.A, sf[sa][0] - fb[0], sf[sa][1] - fb[1], sf[sa][2] - fb[2]);
         if (r2 < a.rc_lj2 || r2 < a.rc_ew2) {
             const double2 p = a.pair64[sa * a.ntypes_c + G.type[sb]];
             e += cw_pair(r2, p.x, p.y, KE * G.q[sa] * G.q[sb], F.alpha, a.rc_lj2, a.rc_ew2);
@@ -250,13 +250,19 @@ hipError_t launch_chain_widom(const ChainWidomArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
     const dim3 grid((unsigned)((a.nitems + CW_NW - 1) / CW_NW)), block(CW_WG);
-    switch (a.guest.nsites) {
-    case 1: hipLaunchKernelGGL(chain_widom_kernel<1>, grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(chain_widom_kernel<2>, grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL(chain_widom_kernel<3>, grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(chain_widom_kernel<4>, grid, block, 0, st, a); break;
-    case 5: hipLaunchKernelGGL(chain_widom_kernel<5>, grid, block, 0, st, a); break;
-    case 6: hipLaunchKernelGGL(chain_widom_kernel<6>, grid, block, 0, st, a); break;
+    switch (a.guest.nsites * 2 + (a.upper ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((chain_widom_kernel<1, false>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((chain_widom_kernel<1, true>), grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((chain_widom_kernel<2, false>), grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL((chain_widom_kernel<2, true>), grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL((chain_widom_kernel<3, false>), grid, block, 0, st, a); break;
+    case 7: hipLaunchKernelGGL((chain_widom_kernel<3, true>), grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((chain_widom_kernel<4, false>), grid, block, 0, st, a); break;
+    case 9: hipLaunchKernelGGL((chain_widom_kernel<4, true>), grid, block, 0, st, a); break;
+    case 10: hipLaunchKernelGGL((chain_widom_kernel<5, false>), grid, block, 0, st, a); break;
+    case 11: hipLaunchKernelGGL((chain_widom_kernel<5, true>), grid, block, 0, st, a); break;
+    case 12: hipLaunchKernelGGL((chain_widom_kernel<6, false>), grid, block, 0, st, a); break;
+    case 13: hipLaunchKernelGGL((chain_widom_kernel<6, true>), grid, block, 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -151,15 +151,23 @@ struct GradArgs {
 // helpers below, which fuse multiply-adds; the helpers above keep every rounding step of the
 // reference's formulas for the paths that need exact agreement.
 // r^2 of the cell vector A (f - rint(f)) for a fractional offset f.
-__device__ inline double cw_r2(const double A[9], double f0, double f1, double f2)
+// UPPER: A[1] == A[2] == A[5] == 0 (every cell built from its parameters). The products with
+// those exact zeros are dropped, which leaves every result bit for bit unchanged.
+template <bool UPPER>
+__device__ inline double cw_r2_t(const double A[9], double f0, double f1, double f2)
 {
     f0 -= __builtin_rint(f0);
     f1 -= __builtin_rint(f1);
     f2 -= __builtin_rint(f2);
     const double m0 = __builtin_fma(A[0], f0, __builtin_fma(A[3], f1, A[6] * f2));
-    const double m1 = __builtin_fma(A[1], f0, __builtin_fma(A[4], f1, A[7] * f2));
-    const double m2 = __builtin_fma(A[2], f0, __builtin_fma(A[5], f1, A[8] * f2));
+    const double m1 = UPPER ? __builtin_fma(A[4], f1, A[7] * f2) : __builtin_fma(A[1], f0, __builtin_fma(A[4], f1, A[7] * f2));
+    const double m2 = UPPER ? A[8] * f2 : __builtin_fma(A[2], f0, __builtin_fma(A[5], f1, A[8] * f2));
     return __builtin_fma(m0, m0, __builtin_fma(m1, m1, m2 * m2));
+}
+
+__device__ inline double cw_r2(const double A[9], double f0, double f1, double f2)
+{
+    return cw_r2_t<false>(A, f0, f1, f2);
 }
 
 // LJ inside rc_lj plus qq_ke erfc(alpha r)/r inside rc_ew (site_pair_energy with one division
@@ -293,6 +301,7 @@ struct McArgs {
     const double *fug;          // per chain, eV/Å^3
     const double *const_p, *const_q;       // exchange constant term p + q N (src/moves.jl:728-773)
     double kT;
+    int upper;                  // every framework's A has A[1] == A[2] == A[5] == 0
     // mode 1 sampling: after every cycle c of this launch, samp_*[n * samp_stride + samp_base + c]
     int sampling;
     int64_t samp_base, samp_stride;
@@ -328,6 +337,7 @@ struct ChainWidomArgs {
     const int *round_off;       // nfw + 1
     const double *half_perp;    // per framework: half the smallest perpendicular width (Å)
     double cull_extra;          // largest cutoff + guest radius + rounding margin (Å)
+    int upper;                  // every framework's A has A[1] == A[2] == A[5] == 0
     const double4 *k64;         // kx, ky, kz, kprefactor (every k-vector)
     const int4 *kn;
     const double2 *pair64;

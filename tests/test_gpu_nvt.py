@@ -24,11 +24,16 @@ T = 298.15
 SEED = 5
 
 
-def _porous_system(nsys=1, natoms=40, rep=2, gamma=90):
+def _porous_system(nsys=1, natoms=40, rep=2, gamma=90, rotate=0.0):
+    """rotate: angle (rad) about z applied to the cell, which makes A not upper triangular."""
     ff = synth.synthetic_forcefield(11.0)
     g = synth.co2_guest(ff)
     uc = synth.synthetic_framework(12.0, 12.0, 12.0, 90, 90, gamma, natoms, 5, charge_sd=0.5)
     fw = pa.replicate(uc, (rep, rep, rep))
+    if rotate:
+        c, s = np.cos(rotate), np.sin(rotate)
+        R = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+        fw = pa.Framework(R @ fw.cell, fw.frac, fw.labels, fw.symbols, fw.charges, fw.replication)
     ew = pa.EwaldParams(11.0)
     b = pa.FrameworkBatch([fw] * nsys, ff, g, ew, fullk=True)
     return fw, ff, g, ew, b
@@ -57,11 +62,14 @@ def _chains(st, caps, fw, ff, g, ew):
     return o, out
 
 
-@pytest.mark.parametrize("rep,gamma", [(2, 90), (3, 90), (3, 110)])
-def test_chain_energies_match_oracle_in_a_loaded_configuration(rep, gamma):
+@pytest.mark.parametrize("rep,gamma,rotate", [(2, 90, 0.0), (3, 90, 0.0), (3, 110, 0.0), (2, 110, 0.3)])
+def test_chain_energies_match_oracle_in_a_loaded_configuration(rep, gamma, rotate):
     """(3, 3, 3) cells are wide enough that most atom chunks lie beyond the cutoff of an
-    insertion and are skipped whole; the triclinic case checks the distance bound there."""
-    fw, ff, g, ew, b = _porous_system(rep=rep, gamma=gamma)
+    insertion and are skipped whole; the triclinic case checks the distance bound there. The
+    rotated cell (A not upper triangular) runs the general distance variant of the move and
+    test-insertion kernels."""
+    fw, ff, g, ew, b = _porous_system(rep=rep, gamma=gamma, rotate=rotate)
+    assert (abs(fw.cell[1, 0]) > 1.0) == bool(rotate)
     st = pa.SystemState(b, g, [8], T=T, seed=SEED, capacities=[20])
     o, (m,) = _chains(st, [20], fw, ff, g, ew)
     tr = pa.mc_moves(st, pa.MOVE_TRANSLATION, 60, T=T, step_trans=0.5, trace=True)[0]
