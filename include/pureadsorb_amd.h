@@ -321,6 +321,55 @@ int pa_nvt_last_timing(const pa_state *state, pa_nvt_timing *out);
 int pa_batch_set_bin_scale(pa_batch *batch, double kappa);
 int pa_batch_bin_scale(const pa_batch *batch, double *kappa);
 
+/* ---- Input files, host only (no device is touched) ----
+ * Each reader parses one file into an opaque handle that owns the parsed values; the accessors
+ * copy them into caller buffers sized from the *_info calls, in the layouts of pa_framework,
+ * pa_forcefield and pa_guest. Errors are PA_ERR_ARG with the reference's ArgumentError wording
+ * (PA_ERR_DIM where it throws DimensionMismatch or indexes past an array). */
+typedef struct pa_cif_file pa_cif_file;
+typedef struct pa_ff_file pa_ff_file;
+typedef struct pa_guest_file pa_guest_file;
+
+/* read_cif (src/structure.jl:37-69): one P1 data block with one _atom_site loop holding label,
+ * type_symbol, fract_x/y/z and charge columns. */
+int pa_read_cif(const char *path, pa_cif_file **out);
+/* natoms and the cell (src/cell.jl:2-12: column-major, lattice vectors as columns); either may
+ * be NULL. */
+int pa_cif_info(const pa_cif_file *cif, int32_t *natoms, double cell[9]);
+/* frac: natoms x 3 fractional coordinates; charges: natoms, e (either may be NULL). */
+int pa_cif_atoms(const pa_cif_file *cif, double *frac, double *charges);
+#define PA_CIF_LABELS 0
+#define PA_CIF_SYMBOLS 1
+/* Framework.labels or .symbols as consecutive NUL-terminated strings; *needed = bytes required.
+ * buf == NULL only reports the size. */
+int pa_cif_strings(const pa_cif_file *cif, int32_t which, char *buf, size_t cap, size_t *needed);
+/* pa_framework.types: typeindex(ff, symbol * "_") of every atom (src/batch.jl:309). */
+int pa_cif_types(const pa_cif_file *cif, const pa_ff_file *ff, int32_t *types);
+void pa_cif_destroy(pa_cif_file *cif);
+
+/* read_forcefield (src/forcefield.jl:54-61): `parameters: name: [sigma, epsilon]` (a null
+ * sigma reads as 1, a null epsilon as 0), `cutoff` (required), `tail_correction` (default
+ * true). Types keep the file's order. */
+int pa_read_forcefield(const char *path, pa_ff_file **out);
+int pa_ff_info(const pa_ff_file *ff, int32_t *ntypes, double *cutoff, int32_t *tail);
+/* Per-type sigma / epsilon (ntypes) and the Lorentz-Berthelot tables of ForceField
+ * (src/forcefield.jl:24-39; ntypes x ntypes, column-major); any pointer may be NULL. */
+int pa_ff_params(const pa_ff_file *ff, double *sigma_type, double *epsilon_type, double *sigma, double *epsilon);
+/* Type names as consecutive NUL-terminated strings (as pa_cif_strings). */
+int pa_ff_names(const pa_ff_file *ff, char *buf, size_t cap, size_t *needed);
+/* typeindex (src/forcefield.jl:41-45), 0-based. */
+int pa_ff_typeindex(const pa_ff_file *ff, const char *name, int32_t *index);
+void pa_ff_destroy(pa_ff_file *ff);
+
+/* read_guest (src/forcefield.jl:91-101): `positions`, `symbols` (looked up in ff), `charges`,
+ * `critical_temperature`, `critical_pressure` (digit-group underscores allowed, as
+ * parse_number), `acentric_factor`. */
+int pa_read_guest(const char *path, const pa_ff_file *ff, pa_guest_file **out);
+int pa_guest_info(const pa_guest_file *g, int32_t *nsites, double *tc, double *pc, double *omega);
+/* sites: nsites x 3 (Å); types: 0-based force-field indices; charges: e (any may be NULL). */
+int pa_guest_sites(const pa_guest_file *g, double *sites, int32_t *types, double *charges);
+void pa_guest_destroy(pa_guest_file *g);
+
 /* Timing of the last pa_widom or pa_widom_grad call on `batch`. */
 int pa_last_timing(const pa_batch *batch, pa_timing *out);
 

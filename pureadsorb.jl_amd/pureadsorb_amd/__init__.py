@@ -10,6 +10,7 @@ from .batch import FrameworkBatch, nframeworks
 from .forcefield import EwaldParams, ForceField, Guest, read_forcefield, read_guest, tail_coefficient, tail_delta, typeindex
 from .fugacity import Fugacity, cubic_roots, peng_robinson_fugacity
 from .isotherm import IsothermResult, combine_replicas, run_isotherm
+from .io_native import cif_types_native, read_cif_native, read_forcefield_native, read_guest_native
 from .mc import (MOVE_DELETION, MOVE_INSERTION, MOVE_REINSERTION, MOVE_ROTATION, MOVE_TRANSLATION, GCMCResult,
                  NVTResult, SystemState, audit_energy, gcmc_samples, mc_delete, mc_exchange, mc_insert, mc_moves, mc_step,
                  nvt_last_timing, nvt_samples, run_gcmc, run_nvt, total_energy, widom_chain_energies)
@@ -28,4 +29,5 @@ __all__ = [
     "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "run_gcmc",
     "GCMCResult", "gcmc_samples", "NVTResult", "run_nvt", "nvt_samples", "widom_chain_energies", "nvt_last_timing",
     "Fugacity", "cubic_roots", "peng_robinson_fugacity", "IsothermResult", "combine_replicas", "run_isotherm",
+    "read_cif_native", "read_forcefield_native", "read_guest_native", "cif_types_native",
 ]

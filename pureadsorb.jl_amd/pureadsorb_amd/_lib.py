@@ -32,6 +32,9 @@ EXPORTS = (
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
     "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
     "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples", "pa_nvt_last_timing",
+    "pa_read_cif", "pa_cif_info", "pa_cif_atoms", "pa_cif_strings", "pa_cif_types", "pa_cif_destroy",
+    "pa_read_forcefield", "pa_ff_info", "pa_ff_params", "pa_ff_names", "pa_ff_typeindex", "pa_ff_destroy",
+    "pa_read_guest", "pa_guest_info", "pa_guest_sites", "pa_guest_destroy",
 )
 
 
@@ -239,6 +242,28 @@ def lib():
     L.pa_device_name.argtypes = [C.c_char_p, C.c_size_t]
     L.pa_synchronize.restype = C.c_int
     L.pa_synchronize.argtypes = []
+    # input files (host only)
+    i32p, dp, szp = C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_size_t)
+    for name, args in (
+        ("pa_read_cif", [C.c_char_p, C.POINTER(vp)]),
+        ("pa_cif_info", [vp, i32p, vp]),
+        ("pa_cif_atoms", [vp, vp, vp]),
+        ("pa_cif_strings", [vp, C.c_int32, C.c_char_p, C.c_size_t, szp]),
+        ("pa_cif_types", [vp, vp, vp]),
+        ("pa_read_forcefield", [C.c_char_p, C.POINTER(vp)]),
+        ("pa_ff_info", [vp, i32p, dp, i32p]),
+        ("pa_ff_params", [vp, vp, vp, vp, vp]),
+        ("pa_ff_names", [vp, C.c_char_p, C.c_size_t, szp]),
+        ("pa_ff_typeindex", [vp, C.c_char_p, i32p]),
+        ("pa_read_guest", [C.c_char_p, vp, C.POINTER(vp)]),
+        ("pa_guest_info", [vp, i32p, dp, dp, dp]),
+        ("pa_guest_sites", [vp, vp, vp, vp]),
+    ):
+        getattr(L, name).restype = C.c_int
+        getattr(L, name).argtypes = args
+    for name in ("pa_cif_destroy", "pa_ff_destroy", "pa_guest_destroy"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [vp]
     _lib = L
     return L
 
