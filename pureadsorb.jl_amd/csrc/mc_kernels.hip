@@ -253,7 +253,7 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
                 for (int s = 0; s < NS; ++s) {
                     const double r2 = cw_r2_t<UPPER>(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
                     const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
-                    const uint64_t mask = __ballot(in);
+                    const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                     if (in) {
                         const unsigned below = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
             for (int x = 0; x < NS; ++x) {
                 const double r2 = cw_r2_t<UPPER>(F.A, sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z);
                 const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
-                const uint64_t mask = __ballot(in);
+                const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                 if (in) {
                     const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                         }
                     }
                 }
-                const uint64_t m = __ballot(inc);
+                const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
                 if (lane == 0) wcount[wave] = __popcll(m);
                 __syncthreads();
                 int base = 0, total = 0;
@@ -399,6 +399,13 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                 p = 0;
             }
         }
+        // pad the list to a multiple of 64 with far-away sentinels (no lane of the sweep needs a
+        // bounds test); listcap is a multiple of 64, so the padding stays inside the list
+        const int cnt64 = (cnt + 63) & ~63;
+        for (int e = cnt + tid; e < cnt64; e += BIN_WG) {
+            ent[e] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+            etype[e] = 0;
+        }
         __syncthreads();
 
         // ---- evaluate the bin's insertions against the list
@@ -419,23 +426,20 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             }
             double acc = 0.0;
             uint32_t head = 0, tail = 0;
-            for (int e0 = 0; e0 < cnt; e0 += 64) {
-                const int e = e0 + lane;
-                const bool valid = e < cnt;
-                float4 E = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-                int ht = 0;
-                if (valid) {
-                    E = ent[e];
-                    ht = etype[e];
-                }
+            for (int e0 = 0; e0 < cnt64; e0 += 64) {
+                const float4 E = ent[e0 + lane];
+                const int ht = etype[e0 + lane];
 #pragma unroll
                 for (int t = 0; t < NS; ++t) {
                     const float dx = sxr[t] - E.x, dy = syr[t] - E.y, dz = szr[t] - E.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    const bool in = valid && (r2 < rc_max2);
-                    const uint64_t mask = __ballot(in);
+                    // sentinel entries: r2 overflows to +inf and fails the test
+                    const bool in = r2 < rc_max2;
+                    const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                     if (in) {
-                        const uint32_t slot = (tail + (uint32_t)__popcll(mask & below)) & (BRING - 1);
+                        const uint32_t below_in = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        const uint32_t slot = (tail + below_in) & (BRING - 1);
                         ring_r2[slot] = r2;
                         ring_q[slot] = E.w;
                         ring_info[slot] = (t << 16) | (t * ntc + ht);
@@ -510,6 +514,7 @@ static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t 
 template <int NS>
 static hipError_t launch_binned_energy_ns(const BinnedArgs &a, bool upper, hipStream_t st)
 {
+    if (a.listcap <= 0 || a.listcap % 64 != 0) return hipErrorInvalidValue;  // the sweep reads whole 64-entry rows
     const size_t lds = binned_lds_bytes(a.listcap, NS);
     const dim3 grid((unsigned)a.nbins), block(BIN_WG);
     if (a.guest.charged) {
