@@ -267,6 +267,7 @@ McArgs base_mc(pa_state *st)
     a.ntypes_c = b->ntypes_c;
     a.rc_lj2 = b->rc_lj * b->rc_lj;
     a.rc_ew2 = b->gdev.charged ? b->rc_ew * b->rc_ew : 0.0;
+    a.rc_max2 = std::max(a.rc_lj2, a.rc_ew2);
     a.guest = b->gdev;
     a.nsys = st->nsys;
     a.guest_off = st->d_guest_off.as<int>();
@@ -1113,6 +1114,7 @@ ChainWidomArgs base_chain_widom(pa_state *st)
     a.ntypes_c = b->ntypes_c;
     a.rc_lj2 = b->rc_lj * b->rc_lj;
     a.rc_ew2 = b->gdev.charged ? b->rc_ew * b->rc_ew : 0.0;
+    a.rc_max2 = std::max(a.rc_lj2, a.rc_ew2);
     a.guest = b->gdev;
     a.nsys = st->nsys;
     a.guest_off = st->d_guest_off.as<int>();

@@ -237,22 +237,27 @@ __global__ __launch_bounds__(MC_WG) void mc_chain_kernel(McArgs a)
             };
             const double4 *af = a.atomf + F.atom_off;
             const int *tf = a.atypef + F.atom_off;
-            double4 h_next = gt < natoms ? af[gt] : make_double4(0, 0, 0, 0);
+            // h holds atom j while j < natoms and a NaN position past the last atom, whose r2 fails
+            // every cutoff test
+            const double nan = __builtin_nan("");
+            double4 h_next = gt < natoms ? af[gt] : make_double4(nan, nan, nan, 0.0);
             int ht_next = gt < natoms ? tf[gt] : 0;
 #pragma unroll 1
             for (int jb = gt - lane; jb < natoms; jb += MC_GW) {
                 const int j = jb + lane;
-                const bool valid = j < natoms;
                 const double4 h = h_next;
                 const int ht = ht_next;
                 if (j + MC_GW < natoms) {
                     h_next = af[j + MC_GW];
                     ht_next = tf[j + MC_GW];
+                } else {
+                    h_next = make_double4(nan, nan, nan, 0.0);
+                    ht_next = 0;
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     const double r2 = cw_r2_t<UPPER>(F.A, sfn[s][0] - h.x, sfn[s][1] - h.y, sfn[s][2] - h.z);
-                    const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
+                    const bool in = r2 < a.rc_max2;
                     const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                     if (in) {
                         const unsigned below = __builtin_amdgcn_mbcnt_hi(

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
                 const double2 p = a.pair64[code];
                 e += cw_pair(qr2[i], p.x, p.y, qqq[i], F.alpha, a.rc_lj2, a.rc_ew2);
             }
-            head += n;
+            head = __builtin_amdgcn_readfirstlane(head + n);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -118,19 +118,21 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
         const double halfw = a.half_perp[u];
         const double4 *af = a.atomf64 + F.atom_off;
         const int *types = a.atype + F.atom_off;
+        const double nan = __builtin_nan("");
 #pragma unroll 1
         for (int c = c0; c < c1; ++c) {
             const double4 cc = a.chunk[c];
             const double dist = fmin(__builtin_sqrt(cw_r2_t<UPPER>(F.A, pf[0] - cc.x, pf[1] - cc.y, pf[2] - cc.z)), halfw);
-            if (dist > cc.w + a.cull_extra) continue;
+            // the same value in every lane (one insertion per wavefront): a uniform branch
+            if (__builtin_amdgcn_readfirstlane((int)(dist > cc.w + a.cull_extra))) continue;
             const int j = (c - c0) * 64 + lane;
-            const bool valid = j < F.natoms;
-            const double4 h = valid ? af[j] : make_double4(0, 0, 0, 0);
-            const int ht = valid ? types[j] : 0;
+            // lanes past the last atom take a NaN position, whose r2 fails every cutoff test
+            const double4 h = j < F.natoms ? af[j] : make_double4(nan, nan, nan, 0.0);
+            const int ht = j < F.natoms ? types[j] : 0;
 #pragma unroll
             for (int x = 0; x < NS; ++x) {
                 const double r2 = cw_r2_t<UPPER>(F.A, sf[x][0] - h.x, sf[x][1] - h.y, sf[x][2] - h.z);
-                const bool in = valid && (r2 < a.rc_lj2 || r2 < a.rc_ew2);
+                const bool in = r2 < a.rc_max2;
                 const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                 if (in) {
                     const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(CW_WG, (NS <= 3 ? 4 : 1)) void chain_widom_kernel(C
                     qqq[i] = KE * G.q[x] * h.w;
                     qcode[i] = x * a.ntypes_c + ht;
                 }
-                tail += (unsigned)__popcll(mask);
+                tail = __builtin_amdgcn_readfirstlane(tail + (unsigned)__popcll(mask));
                 if (tail - head >= 64) drain(64);
             }
         }

@@ -275,6 +275,7 @@ struct McArgs {
     const double2 *pair64;      // [site][type]: sigma, epsilon
     int ntypes_c;
     double rc_lj2, rc_ew2;
+    double rc_max2;             // max(rc_lj2, rc_ew2): the host-guest pair test
     GuestDev guest;
     int nsys;
     // chain state (SystemState, src/state.jl:89-104)
@@ -343,6 +344,7 @@ struct ChainWidomArgs {
     const double2 *pair64;
     int ntypes_c;
     double rc_lj2, rc_ew2;
+    double rc_max2;             // max(rc_lj2, rc_ew2): the host-guest pair test
     GuestDev guest;
     int nsys;
     const int *guest_off;
