@@ -321,6 +321,18 @@ int pa_nvt_last_timing(const pa_state *state, pa_nvt_timing *out);
 int pa_batch_set_bin_scale(pa_batch *batch, double kappa);
 int pa_batch_bin_scale(const pa_batch *batch, double *kappa);
 
+/* Mixed-path neighbour-list capacity of `batch`, in entries per pass (no counterpart in the
+ * reference). 0 (the initial value) sizes it per call from the atom density (1.3x the mean
+ * count within the list radius, within the kernel's LDS budget); a fixed capacity must be a
+ * multiple of 64 between one list-build slice (768) and the LDS limit. A bin whose list holds
+ * more entries is built and swept in several passes; results agree with a single pass to FP64
+ * summation order. */
+int pa_batch_set_list_capacity(pa_batch *batch, int32_t capacity);
+/* Capacity the last mixed-path pa_widom / pa_widom_energies call on `batch` used, and the
+ * largest number of passes any of its bins needed (both 0 before the first such call; passes 0
+ * after a pa_widom_noalloc call, which keeps no counter). Reads one counter from the device. */
+int pa_batch_list_stats(pa_batch *batch, int32_t *capacity, int32_t *max_passes);
+
 /* ---- Input files, host only (no device is touched) ----
  * Each reader parses one file into an opaque handle that owns the parsed values; the accessors
  * copy them into caller buffers sized from the *_info calls, in the layouts of pa_framework,

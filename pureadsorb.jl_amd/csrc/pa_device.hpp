@@ -99,7 +99,9 @@ struct BinnedArgs {
     const float2 *pair32;
     int ntypes_c;
     float rc_lj2f, rc_ew2f, rc_max2f;
-    int listcap;               // neighbour-list capacity (entries) of one pass
+    int listcap;               // neighbour-list capacity (entries) of one pass, >= BIN_WG
+    int *extra_passes;         // max over bins of (passes - 1) of this call's energy launches
+    int zero_passes;           // nonzero on the call's first chunk: bin_zero_kernel clears it
     GuestDev guest;
     // hard-core rejection (phase 0): atoms of each framework sorted by grid cell
     const uint4 *corefx;       // fixed-point x, y, z and compact type, by cell

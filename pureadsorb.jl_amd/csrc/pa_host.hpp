@@ -111,6 +111,47 @@ inline hipError_t upload_cached(DevAlloc &d, const std::vector<T> &v)
     return hipSuccess;
 }
 
+// Per-call device buffers of the Widom pipeline. Normally each is the batch's own allocation,
+// grown on demand (and an upload of unchanged bytes is skipped). Given a fixed region (the
+// no-allocation entry point), the buffers are carved from it in request order, 256-byte
+// aligned, and uploads always copy; a request past the region's end fails with `need` set to
+// the bytes the call would have needed. Every request of a call precedes its first launch.
+struct CallScratch {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0, need = 0;
+    hipError_t take(DevAlloc &own, size_t bytes, void **p)
+    {
+        if (!base) {
+            hipError_t e = own.ensure(bytes);
+            *p = own.p;
+            return e;
+        }
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || bytes > cap - off) {
+            need = off + bytes;
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        *p = base + off;
+        used = off + bytes;
+        return hipSuccess;
+    }
+    template <class T>
+    hipError_t put(DevAlloc &own, const std::vector<T> &v, T **p)
+    {
+        if (!base) {
+            hipError_t e = upload_cached(own, v);
+            *p = own.as<T>();
+            return e;
+        }
+        void *q = nullptr;
+        hipError_t e = take(own, sizeof(T) * std::max<size_t>(v.size(), 1), &q);
+        *p = (T *)q;
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(q, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+    }
+};
+
 }  // namespace pa
 
 // The opaque handle of the C ABI.
@@ -150,11 +191,16 @@ struct pa_batch {
     // host copies of the FP64 tables (the move path's from-scratch energies and audit)
     bool fullk = false;
     double bin_scale = 0.0;  // mixed path: bin edge multiplier (0: not yet tuned, 1 used meanwhile)
+    pa::CallScratch scr;     // where the current call's per-call buffers come from
+    int device = 0;          // HIP device the batch's tables live on
+    int list_cap = 0;        // mixed path: fixed neighbour-list capacity (0: sized per call)
+    int list_cap_used = 0;   // capacity of the last mixed-path call
+    bool passes_recorded = false;  // d_passes holds the last mixed-path call's pass count
     std::vector<double4> atom64_h, k64_h;
     std::vector<int4> kn_h;    // n of every k-vector of k64_h
     std::vector<double2> S64_h;
     std::vector<int> atype_h;
-    pa::DevAlloc d_corefx, d_coreidx, d_cell_start, d_rho2, d_cand, d_ncand, d_nbr_start, d_nbr_atoms;
+    pa::DevAlloc d_corefx, d_coreidx, d_cell_start, d_rho2, d_cand, d_ncand, d_nbr_start, d_nbr_atoms, d_passes;
     // gradient scratch
     pa::DevAlloc d_list_g, d_msys, d_flag, d_count, d_sel_tmp, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg,
         d_loff, d_lsite, d_lw, d_blk, d_kblk;

@@ -28,6 +28,7 @@ hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes
 hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
 int binned_listcap_max(int ns);
+int binned_listcap_min();
 hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
@@ -923,6 +924,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     // ---- device upload
     int st = device_ready();
     if (st) return st;
+    HIP_TRY(hipGetDevice(&b->device));
     HIP_TRY(hipStreamCreate(&b->stream));
     // timing-only events: no system-scope fence (no cache write-back and invalidation around the
     // kernels they bracket); results reach the host through copies and stream synchronisation
@@ -1077,6 +1079,23 @@ static LseH lse_merge_h(LseH x, LseH y)
     return LseH{m, x.w * ex + y.w * ey, x.u * ex + y.u * ey};
 }
 
+// A per-call buffer request (CallScratch): past the end of a fixed region it is an argument
+// error that states both sizes, otherwise a device error.
+static int scratch_fail(const pa_batch *b, hipError_t e, const char *what, int line)
+{
+    if (b->scr.base && b->scr.need > 0)
+        return fail(PA_ERR_ARG,
+                    "the call needs %zu bytes of device scratch, more than the %zu bytes the no-allocation entry point "
+                    "holds; pass fewer insertions or a smaller chunk",
+                    b->scr.need, b->scr.cap);
+    return fail(PA_ERR_DEVICE, "HIP error %s at %s:%d (%s)", hipGetErrorString(e), __FILE__, line, what);
+}
+#define SCR_TRY(b, expr)                                                                                \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return scratch_fail(b, e_, #expr, __LINE__);                              \
+    } while (0)
+
 // Smallest bin edge (Å) of the mixed path, and the insertions a bin aims to hold when a system
 // has few insertions for its volume.
 constexpr double BIN_EDGE_MIN = 2.0;
@@ -1089,14 +1108,19 @@ constexpr double BIN_TARGET_INSERTIONS = 64.0;
 // With keys_out, the mixed path stops after phase 0 and binning and returns each insertion's
 // bin key (-1: rejected) instead of energies.
 static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb, int64_t ge, int64_t out_base,
-                        uint64_t seed, uint32_t precision, int64_t chunk, int64_t *launches, double kT,
+                        uint64_t seed, uint32_t precision, int64_t chunk, int64_t *launches, double kT, double *dU,
                         std::vector<int> *keys_out = nullptr)
 {
     const int nsys = b->nsys;
-    HIP_TRY(upload_cached(b->d_seg, seg));
+    pa::CallScratch &scr = b->scr;
+    int64_t *d_seg = nullptr;
+    SCR_TRY(b, scr.put(b->d_seg, seg, &d_seg));
     const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
     const int64_t nchunks = (ge - gb + CH - 1) / CH;
-    while ((int64_t)b->kev.size() < 2 * nchunks) {
+    // event pairs around the energy launches; a call into a fixed scratch region creates none
+    // and records none
+    const bool timed = scr.base == nullptr;
+    while (timed && (int64_t)b->kev.size() < 2 * nchunks) {
         hipEvent_t e;
         HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         b->kev.push_back(e);
@@ -1104,16 +1128,16 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     *launches = 0;
     if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_F64) {
         WidomArgs a = base_args(b);
-        a.seg = b->d_seg.as<int64_t>();
+        a.seg = d_seg;
         a.seed = seed;
-        a.dU = b->d_dU.as<double>();
+        a.dU = dU;
         a.out_base = out_base;
         for (int64_t g0 = gb; g0 < ge; g0 += CH) {
             a.g0 = g0;
             a.g1 = std::min(ge, g0 + CH);
-            HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
+            if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
             HIP_TRY(launch_energy(a, (int)precision, b->upper_all, b->stream));
-            HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
+            if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
             ++*launches;
         }
         return PA_OK;
@@ -1190,39 +1214,55 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         }
     }
     const int nbins = base[nsys];
-    const int listcap = (int)std::min<double>(binned_listcap_max(b->nsites), std::ceil(cap_est / 64.0) * 64.0);
-    HIP_TRY(upload_cached(b->d_bins, bins));
-    HIP_TRY(upload_cached(b->d_bin_base, base));
-    HIP_TRY(upload_cached(b->d_rho2, rho2));
-    HIP_TRY(b->d_bin_count.ensure(sizeof(int) * (nbins + 1)));
-    HIP_TRY(b->d_bin_start.ensure(sizeof(int) * (nbins + 1)));
-    HIP_TRY(b->d_cursor.ensure(sizeof(int) * nbins));
+    // list capacity: the estimate from the mean density, or the batch's fixed value, never below
+    // one slice of the list build (binned_listcap_min) so that every pass makes progress
+    const int cap_hi = binned_listcap_max(b->nsites), cap_lo = binned_listcap_min();
+    int listcap = b->list_cap > 0 ? b->list_cap : (int)std::min<double>(cap_hi, std::ceil(cap_est / 64.0) * 64.0);
+    listcap = std::max(cap_lo, std::min(cap_hi, listcap));
+    if (cap_lo > cap_hi) return fail(PA_ERR_INTERNAL, "list capacity bounds %d > %d", cap_lo, cap_hi);
+    b->list_cap_used = listcap;
+    BinnedArgs A{};
+    BinSys *d_bins = nullptr;
+    int *d_base = nullptr;
+    float *d_rho2 = nullptr;
+    SCR_TRY(b, scr.put(b->d_bins, bins, &d_bins));
+    SCR_TRY(b, scr.put(b->d_bin_base, base, &d_base));
+    SCR_TRY(b, scr.put(b->d_rho2, rho2, &d_rho2));
+    void *p = nullptr;
+    SCR_TRY(b, scr.take(b->d_bin_count, sizeof(int) * (nbins + 1), &p));
+    A.bin_count = (int *)p;
+    SCR_TRY(b, scr.take(b->d_bin_start, sizeof(int) * (nbins + 1), &p));
+    A.bin_start = (int *)p;
+    SCR_TRY(b, scr.take(b->d_cursor, sizeof(int) * nbins, &p));
+    A.cursor = (int *)p;
     size_t scan_bytes = 0;
     HIP_TRY(binned_scan_bytes(nbins, &scan_bytes));
-    HIP_TRY(b->d_scan_tmp.ensure(scan_bytes));
+    void *scan_tmp = nullptr;
+    SCR_TRY(b, scr.take(b->d_scan_tmp, scan_bytes, &scan_tmp));
     const int64_t chunk_n = std::min<int64_t>(CH, ge - gb);
-    HIP_TRY(b->d_sitefx.ensure(sizeof(uint32_t) * 3 * b->nsites * chunk_n));
-    HIP_TRY(b->d_key.ensure(sizeof(int) * chunk_n));
-    HIP_TRY(b->d_order.ensure(sizeof(int) * chunk_n));
-    HIP_TRY(b->d_cand.ensure(sizeof(int) * chunk_n));
-    HIP_TRY(b->d_ncand.ensure(sizeof(int)));
+    SCR_TRY(b, scr.take(b->d_sitefx, sizeof(uint32_t) * 3 * b->nsites * chunk_n, &p));
+    A.sitefx = (uint32_t *)p;
+    SCR_TRY(b, scr.take(b->d_key, sizeof(int) * chunk_n, &p));
+    A.key = (int *)p;
+    SCR_TRY(b, scr.take(b->d_order, sizeof(int) * chunk_n, &p));
+    A.order = (int *)p;
+    SCR_TRY(b, scr.take(b->d_cand, sizeof(int) * chunk_n, &p));
+    A.cand = (int *)p;
+    SCR_TRY(b, scr.take(b->d_ncand, sizeof(int), &p));
+    A.ncand = (int *)p;
+    SCR_TRY(b, scr.take(b->d_passes, sizeof(int), &p));
+    A.extra_passes = (int *)p;
+    b->passes_recorded = scr.base == nullptr;  // a fixed region's counter is not kept after the call
 
-    BinnedArgs A{};
     A.fw = b->d_fw.as<FwDesc>();
     A.fw_of = b->d_fw_of.as<int>();
-    A.seg = b->d_seg.as<int64_t>();
-    A.bins = b->d_bins.as<BinSys>();
-    A.bin_base = b->d_bin_base.as<int>();
+    A.seg = d_seg;
+    A.bins = d_bins;
+    A.bin_base = d_base;
     A.nsys = nsys;
     A.nbins = nbins;
     A.seed = seed;
-    A.sitefx = b->d_sitefx.as<uint32_t>();
-    A.key = b->d_key.as<int>();
-    A.bin_count = b->d_bin_count.as<int>();
-    A.bin_start = b->d_bin_start.as<int>();
-    A.cursor = b->d_cursor.as<int>();
-    A.order = b->d_order.as<int>();
-    A.dU = b->d_dU.as<double>();
+    A.dU = dU;
     A.out_base = out_base;
     A.atomfx = b->d_atomfx.as<uint4>();
     A.atype = b->d_atype.as<int>();
@@ -1239,13 +1279,12 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.cell_start = b->d_cell_start.as<int>();
     A.nbr_start = b->d_nbr_start.as<int>();
     A.nbr_atoms = b->d_nbr_atoms.as<uint4>();
-    A.rho2 = b->d_rho2.as<float>();
-    A.cand = b->d_cand.as<int>();
-    A.ncand = b->d_ncand.as<int>();
+    A.rho2 = d_rho2;
     for (int64_t g0 = gb; g0 < ge; g0 += CH) {
         A.g0 = g0;
         A.g1 = std::min(ge, g0 + CH);
-        HIP_TRY(launch_binning(A, b->d_scan_tmp.p, scan_bytes, b->stream));
+        A.zero_passes = g0 == gb;
+        HIP_TRY(launch_binning(A, scan_tmp, scan_bytes, b->stream));
         if (keys_out) {
             keys_out->resize(ge - gb);
             HIP_TRY(hipMemcpyAsync(keys_out->data() + (g0 - gb), A.key, sizeof(int) * (A.g1 - A.g0),
@@ -1253,9 +1292,9 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
             HIP_TRY(hipStreamSynchronize(b->stream));
             continue;
         }
-        HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
+        if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
         HIP_TRY(launch_binned_energy(A, b->upper_all, b->stream));
-        HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
+        if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
         ++*launches;
     }
     return PA_OK;
@@ -1308,7 +1347,13 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     std::vector<int64_t> seg(nsys + 1, 0);
     for (int64_t s = 0; s < nsys; ++s) seg[s + 1] = seg[s] + ns[s];
     const int64_t G = seg[nsys];
-    HIP_TRY(b->d_dU.ensure(sizeof(double) * G));
+    pa::CallScratch &scr = b->scr;
+    double *dU = nullptr;
+    {
+        void *p = nullptr;
+        SCR_TRY(b, scr.take(b->d_dU, sizeof(double) * G, &p));
+        dU = (double *)p;
+    }
 
     // tiles: each block of each system split into runs of at most TILE insertions
     const int64_t TILE = 2048;
@@ -1327,12 +1372,21 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
         }
     }
     const int ntiles = (int)tile_g0.size();
-    HIP_TRY(upload_cached(b->d_tile_g0, tile_g0));
-    HIP_TRY(upload_cached(b->d_tile_n, tile_n));
-    HIP_TRY(b->d_tiles.ensure(sizeof(double4) * ntiles));
+    int64_t *d_tile_g0 = nullptr;
+    int *d_tile_n = nullptr;
+    double4 *d_tiles = nullptr;
+    SCR_TRY(b, scr.put(b->d_tile_g0, tile_g0, &d_tile_g0));
+    SCR_TRY(b, scr.put(b->d_tile_n, tile_n, &d_tile_n));
+    {
+        void *p = nullptr;
+        SCR_TRY(b, scr.take(b->d_tiles, sizeof(double4) * ntiles, &p));
+        d_tiles = (double4 *)p;
+    }
 
     const double kT_phase0 = (precision & PA_SINGLE_PHASE) ? 0.0 : kT;
-    if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_MIXED && b->bin_scale == 0.0 && G >= ((int64_t)1 << 18)) {
+    // a call into a fixed scratch region never tunes: it uses the batch's bin scale as it stands
+    if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_MIXED && b->bin_scale == 0.0 && G >= ((int64_t)1 << 18) &&
+        !scr.base) {
         // autotune the bin edge multiplier once per batch: time the energy pipeline on the first
         // (up to) 2^20 insertions for each candidate and keep the fastest
         const double cand[4] = {0.75, 1.0, 1.25, 1.5};
@@ -1343,7 +1397,7 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
                 b->bin_scale = kappa;
                 int64_t nl = 0;
                 HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-                st = run_energies(b, seg, 0, ge, 0, seed, precision, chunk, &nl, kT_phase0);
+                st = run_energies(b, seg, 0, ge, 0, seed, precision, chunk, &nl, kT_phase0, dU);
                 if (st) { b->bin_scale = 0.0; return st; }
                 HIP_TRY(hipEventRecord(b->ev[1], b->stream));
                 HIP_TRY(hipEventSynchronize(b->ev[1]));
@@ -1357,21 +1411,20 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     int64_t launches = 0;
     // the mixed path runs phase 0 (hard-core rejection) unless PA_SINGLE_PHASE; the FP64 plain
     // path evaluates every insertion
-    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches, kT_phase0);
+    st = run_energies(b, seg, 0, G, 0, seed, precision, chunk, &launches, kT_phase0, dU);
     if (st) return st;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
-    HIP_TRY(launch_tiles(b->d_dU.as<double>(), b->d_tile_g0.as<int64_t>(), b->d_tile_n.as<int>(), ntiles, kT,
-                         b->d_tiles.as<double4>(), b->stream));
+    HIP_TRY(launch_tiles(dU, d_tile_g0, d_tile_n, ntiles, kT, d_tiles, b->stream));
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
     std::vector<double4> tiles(ntiles);
-    HIP_TRY(hipMemcpyAsync(tiles.data(), b->d_tiles.p, sizeof(double4) * ntiles, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(tiles.data(), d_tiles, sizeof(double4) * ntiles, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     float e01 = 0, e12 = 0, e02 = 0;
     HIP_TRY(hipEventElapsedTime(&e01, b->ev[0], b->ev[1]));
     HIP_TRY(hipEventElapsedTime(&e12, b->ev[1], b->ev[2]));
     HIP_TRY(hipEventElapsedTime(&e02, b->ev[0], b->ev[2]));
     double kernel_ms = 0.0;
-    for (size_t i = 0; i + 1 < b->kev.size() && (int64_t)(i / 2) < launches; i += 2) {
+    for (size_t i = 0; !scr.base && i + 1 < b->kev.size() && (int64_t)(i / 2) < launches; i += 2) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, b->kev[i], b->kev[i + 1]));
         kernel_ms += ms;
@@ -1756,7 +1809,7 @@ extern "C" int pa_widom_energies(pa_batch *b, int32_t sys, uint64_t seed, int64_
     for (int s = sys + 1; s <= b->nsys; ++s) seg[s] = j0 + n;
     HIP_TRY(b->d_dU.ensure(sizeof(double) * n));
     int64_t launches = 0;
-    st = run_energies(b, seg, j0, j0 + n, j0, seed, precision, 0, &launches, 0.0);
+    st = run_energies(b, seg, j0, j0 + n, j0, seed, precision, 0, &launches, 0.0, b->d_dU.as<double>());
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(dU, b->d_dU.p, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1779,7 +1832,7 @@ extern "C" int pa_hardcore_flags(pa_batch *b, int32_t sys, uint64_t seed, double
     std::vector<int> keys;
     int64_t launches = 0;
     st = run_energies(b, seg, seg[sys] + j0, seg[sys] + j0 + n, seg[sys] + j0, seed, PA_PREC_MIXED, 0, &launches, KB * T,
-                      &keys);
+                      b->d_dU.as<double>(), &keys);
     if (st) return st;
     for (int64_t i = 0; i < n; ++i) flags[i] = keys[i] < 0 ? 1 : 0;
     return ok();
@@ -1814,6 +1867,31 @@ extern "C" int pa_batch_bin_scale(const pa_batch *b, double *kappa)
 {
     if (!b || !kappa) return fail(PA_ERR_ARG, "NULL argument");
     *kappa = b->bin_scale;
+    return ok();
+}
+
+extern "C" int pa_batch_set_list_capacity(pa_batch *b, int32_t capacity)
+{
+    if (!b) return fail(PA_ERR_ARG, "NULL batch");
+    const int lo = binned_listcap_min(), hi = binned_listcap_max(b->nsites);
+    if (capacity != 0 && (capacity < lo || capacity > hi || capacity % 64 != 0))
+        return fail(PA_ERR_ARG, "list capacity %d must be a multiple of 64 in [%d, %d], or 0 to size it per call", capacity,
+                    lo, hi);
+    b->list_cap = capacity;
+    return ok();
+}
+
+extern "C" int pa_batch_list_stats(pa_batch *b, int32_t *capacity, int32_t *max_passes)
+{
+    if (!b) return fail(PA_ERR_ARG, "NULL batch");
+    if (b->host_only) return fail(PA_ERR_ARG, "batch was built with PA_BATCH_HOST_ONLY and has no device tables");
+    int extra = 0;
+    if (b->list_cap_used > 0 && b->passes_recorded && b->d_passes.p) {
+        HIP_TRY(hipMemcpyAsync(&extra, b->d_passes.p, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+    }
+    if (capacity) *capacity = b->list_cap_used;
+    if (max_passes) *max_passes = b->list_cap_used > 0 && b->passes_recorded ? extra + 1 : 0;
     return ok();
 }
 

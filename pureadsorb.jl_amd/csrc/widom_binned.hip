@@ -24,6 +24,7 @@ namespace pa {
 
 constexpr int BIN_WG = 768;
 constexpr int BIN_NW = BIN_WG / 64;
+static_assert(BIN_WG % 64 == 0, "a bin workgroup is whole wavefronts, and the smallest list capacity is BIN_WG");
 constexpr int BRING = 128;  // per-wave ring; drained after every site sweep (< 64 + 64 pending)
 constexpr int BIN_MAX_TYPES = 64;
 
@@ -225,6 +226,7 @@ __global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
     if (i <= a.nbins) a.bin_count[i] = 0;
     if (i < a.nbins) a.cursor[i] = 0;
     if (i == 0) *a.ncand = 0;
+    if (i == 0 && a.zero_passes) *a.extra_passes = 0;
 }
 
 __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
@@ -382,6 +384,9 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                     total += c;
                 }
                 __syncthreads();
+                // a slice adds at most BIN_WG entries and cap >= BIN_WG (checked at launch), so a
+                // pass that starts with an empty list always takes its first slice: every pass
+                // advances (combo, p)
                 if (cnt + total > cap) {
                     full = true;
                     break;
@@ -495,6 +500,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
         __syncthreads();
         ++pass;
     }
+    if (pass > 0 && tid == 0) atomicMax(a.extra_passes, pass);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -514,7 +520,9 @@ static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t 
 template <int NS>
 static hipError_t launch_binned_energy_ns(const BinnedArgs &a, bool upper, hipStream_t st)
 {
-    if (a.listcap <= 0 || a.listcap % 64 != 0) return hipErrorInvalidValue;  // the sweep reads whole 64-entry rows
+    // the sweep reads whole 64-entry rows; a capacity below one slice of the list build could
+    // never take that slice and would never finish
+    if (a.listcap < BIN_WG || a.listcap % 64 != 0 || !a.extra_passes) return hipErrorInvalidValue;
     const size_t lds = binned_lds_bytes(a.listcap, NS);
     const dim3 grid((unsigned)a.nbins), block(BIN_WG);
     if (a.guest.charged) {
@@ -537,6 +545,8 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes)
 // Largest neighbour-list capacity (entries, a multiple of 64) that keeps a bin workgroup's LDS
 // within BIN_LDS_BUDGET, so that two workgroups share a CU's 160 KB.
 constexpr size_t BIN_LDS_BUDGET = 80 * 1024;
+int binned_listcap_min() { return BIN_WG; }
+
 int binned_listcap_max(int ns)
 {
     const size_t fixed = binned_lds_bytes(0, ns);

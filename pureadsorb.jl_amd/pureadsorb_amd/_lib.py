@@ -31,6 +31,7 @@ EXPORTS = (
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
     "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
+    "pa_batch_set_list_capacity", "pa_batch_list_stats",
     "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples", "pa_nvt_last_timing",
     "pa_read_cif", "pa_cif_info", "pa_cif_atoms", "pa_cif_strings", "pa_cif_types", "pa_cif_destroy",
     "pa_read_forcefield", "pa_ff_info", "pa_ff_params", "pa_ff_names", "pa_ff_typeindex", "pa_ff_destroy",
@@ -194,6 +195,10 @@ def lib():
     L.pa_batch_set_bin_scale.argtypes = [vp, C.c_double]
     L.pa_batch_bin_scale.restype = C.c_int
     L.pa_batch_bin_scale.argtypes = [vp, C.POINTER(C.c_double)]
+    L.pa_batch_set_list_capacity.restype = C.c_int
+    L.pa_batch_set_list_capacity.argtypes = [vp, C.c_int32]
+    L.pa_batch_list_stats.restype = C.c_int
+    L.pa_batch_list_stats.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.pa_state_create.restype = C.c_int
     L.pa_state_create.argtypes = [vp, C.POINTER(PaGuest), vp, vp, C.c_double, C.c_uint64, C.POINTER(vp)]
     L.pa_state_destroy.restype = None

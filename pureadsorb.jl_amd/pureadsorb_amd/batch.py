@@ -77,6 +77,7 @@ class FrameworkBatch:
         h = C.c_void_p()
         _lib.check(L.pa_batch_create(arr, self.nsys, C.byref(pff), C.byref(g.s), C.byref(pew), flags, C.byref(h)))
         self._h = h
+        self._list_capacity = 0
         self.host_only = bool(host_only)
         nfw, nsys = C.c_int32(), C.c_int32()
         _lib.check(L.pa_batch_nframeworks(h, C.byref(nfw), C.byref(nsys)))
@@ -126,6 +127,22 @@ class FrameworkBatch:
     @bin_scale.setter
     def bin_scale(self, kappa: float):
         _lib.check(_lib.lib().pa_batch_set_bin_scale(self.handle, float(kappa)))
+
+    @property
+    def list_capacity(self) -> int:
+        """Fixed mixed-path neighbour-list capacity (entries per pass); 0 sizes it per call."""
+        return self._list_capacity
+
+    @list_capacity.setter
+    def list_capacity(self, capacity: int):
+        _lib.check(_lib.lib().pa_batch_set_list_capacity(self.handle, int(capacity)))
+        self._list_capacity = int(capacity)
+
+    def list_stats(self) -> tuple[int, int]:
+        """(capacity, largest number of list passes of any bin) of the last mixed-path call."""
+        cap, passes = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(_lib.lib().pa_batch_list_stats(self.handle, ctypes.byref(cap), ctypes.byref(passes)))
+        return cap.value, passes.value
 
 
 def nframeworks(batch: FrameworkBatch) -> int:

@@ -288,27 +288,67 @@ def test_narrow_cell_takes_images_across_the_boundary():
         assert np.max(np.abs(dm[w] - ref[w])) < 3e-6
 
 
-def test_dense_cluster_overflows_the_list():
-    """2,000 atoms packed into a 12 Å sphere of a 40 Å box: bins near the cluster see far more
-    atoms than the mean density predicts, so their lists are built and swept in several passes."""
-    ff = synth.synthetic_forcefield(12.0)
-    g = synth.co2_guest(ff)
-    rng = np.random.default_rng(3)
+def _cluster_framework(box, n=2000, radius=12.0, seed=3):
+    """n atoms packed (0.9 Å apart) into a sphere of `radius` at the centre of a cubic box."""
+    rng = np.random.default_rng(seed)
     pts = []
-    while len(pts) < 2000:
-        p = rng.uniform(-12, 12, 3)
-        if np.linalg.norm(p) < 12 and all(np.linalg.norm(p - x) > 0.9 for x in pts[-200:]):
+    while len(pts) < n:
+        p = rng.uniform(-radius, radius, 3)
+        if np.linalg.norm(p) < radius and all(np.linalg.norm(p - x) > 0.9 for x in pts[-200:]):
             pts.append(p)
-    frac = (np.array(pts) + 20.0) / 40.0
+    frac = (np.array(pts) + box / 2) / box
     q = rng.normal(0, 0.3, len(frac))
     q -= q.mean()
     sym = [("C", "H", "O", "Zn")[i % 4] for i in range(len(frac))]
-    fw = pa.Framework(np.diag([40.0, 40.0, 40.0]), frac, sym, sym, q)
-    ref, dm = _mixed_vs_oracle(fw, ff, g, pa.EwaldParams(12.0, 1e-6))
-    _check_relative(dm, ref, "dense cluster")
+    return pa.Framework(np.diag([box, box, box]), frac, sym, sym, q)
+
+
+def test_dense_cluster_overflows_the_list():
+    """2,000 atoms packed into a 12 Å sphere of a 40 Å box, with the neighbour list held to its
+    smallest capacity (768 entries): bins near the cluster hold up to ~2,000 atom images, so their
+    lists are built and swept in several passes (the reciprocal term only on the first, the later
+    passes adding to the stored energy). Every energy against the oracle, and against the same
+    batch with the capacity sized per call."""
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    fw = _cluster_framework(40.0)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    ref, dm_auto = _mixed_vs_oracle(fw, ff, g, ew)
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    b.list_capacity = 768
+    dm = pa.widom_energies(b, 0, 5, 0, len(ref), precision="mixed")
+    cap, passes = b.list_stats()
+    print(f"dense cluster: capacity {cap}, largest bin needed {passes} passes", flush=True)
+    assert cap == 768 and passes >= 3
+    _check_relative(dm, ref, "dense cluster, 768-entry list")
+    _check_relative(dm_auto, ref, "dense cluster, capacity per call")
     w = ref < 10 * pa.KB * 298.15
     assert w.sum() > 10
     assert np.max(np.abs(dm[w] - ref[w])) < 3e-6
+    assert np.max(np.abs(dm[w] - dm_auto[w])) < 1e-9
+
+
+def test_sparse_box_cluster_list_takes_several_passes():
+    """Regression: the same cluster in a 60 Å box with 2^20 insertions. The per-call capacity
+    estimate from the mean density (~450 entries) is below one 768-atom slice of the list build,
+    so it is raised to 768 and the bins at the cluster take several passes; the call finishes and
+    the first 4,096 energies match the oracle."""
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    fw = _cluster_framework(60.0)
+    ew = pa.EwaldParams(12.0, 1e-6)
+    b = pa.FrameworkBatch([fw], ff, g, ew)
+    n = 1 << 20
+    print("sparse box cluster: 2^20 mixed insertions", flush=True)
+    dm = pa.widom_energies(b, 0, 11, 0, n, precision="mixed")
+    cap, passes = b.list_stats()
+    print(f"sparse box cluster: capacity {cap}, largest bin needed {passes} passes", flush=True)
+    assert cap == 768 and passes >= 2
+    m = 4096
+    o = oracle_for(fw, ff, g, ew)
+    ref = o.energies(11, 0, 0, m, nthreads=oracle_threads())
+    _check_relative(dm[:m], ref, "sparse box cluster")
+    assert np.all(np.isfinite(dm))
 
 
 def test_config2_batch_subset_against_oracle():

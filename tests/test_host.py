@@ -104,6 +104,23 @@ def test_batch_errors_keep_reference_wording(rubtak, trappe, co2, config1):
         _host_batch(bad, ff, g, ew)
 
 
+def test_list_capacity_knob_keeps_every_pass_progressing(config1):
+    """The mixed path's neighbour list is built in slices of 768 atoms (one per work-item of a bin
+    workgroup); a capacity below one slice could never take a slice and would never finish, so
+    the knob refuses it, as it refuses a capacity that is not whole 64-entry rows or exceeds the
+    LDS budget. Before any mixed call the statistics report nothing."""
+    fw, ff, g, ew = config1
+    b = _host_batch(fw, ff, g, ew)
+    for bad in (64, 448, 700, 800, 1 << 20, -64):
+        with pytest.raises(pa.ArgumentError, match="list capacity"):
+            b.list_capacity = bad
+    b.list_capacity = 768
+    assert b.list_capacity == 768
+    b.list_capacity = 0
+    with pytest.raises(pa.ArgumentError, match="HOST_ONLY"):
+        b.list_stats()
+
+
 def test_widom_argument_checks_keep_reference_wording(config1):
     """Checked before any device work (src/widom.jl:302-330; test/widom_tests.jl:183-230)."""
     fw, ff, g, ew = config1
