@@ -1515,6 +1515,25 @@ static void or_qpow(const double q[4], double t, double o[4])
     o[0] = ax[0] * sh; o[1] = ax[1] * sh; o[2] = ax[2] * sh; o[3] = cos(half);
 }
 
+/* Sampler pieces exported for the distribution tests: n successive rand_normal draws
+ * (src/rng.jl:109-141) and n rand_quaternion draws from one ChainRNG(chain_seed, cycle, move)
+ * stream, quaternion_pow and qmul (src/moves.jl:81-110). */
+void or_normals(uint64_t chain_seed, uint64_t cycle, uint64_t move, int64_t n, double *out)
+{
+    or_chain c = or_chain_new(chain_seed, cycle, move);
+    for (int64_t i = 0; i < n; ++i) out[i] = or_rand_normal(&c);
+}
+
+void or_quats(uint64_t chain_seed, uint64_t cycle, uint64_t move, int64_t n, double *out)
+{
+    or_chain c = or_chain_new(chain_seed, cycle, move);
+    for (int64_t i = 0; i < n; ++i) or_rand_quat(&c, out + 4 * i);
+}
+
+void or_quat_pow(const double q[4], double t, double out[4]) { or_qpow(q, t, out); }
+
+void or_quat_mul(const double q2[4], const double q1[4], double out[4]) { or_qmul(q2, q1, out); }
+
 /* One attempt of `type` (1 translation, 2 rotation, 3 reinsertion, 4 insertion, 5 deletion).
  * fug: eV/Å^3. Returns accepted (0/1); *dU receives the energy change. Returns -1 when an
  * insertion the Metropolis test accepts finds the chain at capacity (nothing is changed). */

@@ -5,6 +5,7 @@ Each wrapper names the reference function it restates (file:line into PureAdsorb
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -66,6 +67,13 @@ def lib():
     L.or_minimum_image.argtypes = [_dp, _dp, _dp, _dp]
     L.or_rotate.restype = None
     L.or_rotate.argtypes = [_dp, _dp, _dp]
+    for name in ("or_normals", "or_quats"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp]
+    L.or_quat_pow.restype = None
+    L.or_quat_pow.argtypes = [_dp, C.c_double, _dp]
+    L.or_quat_mul.restype = None
+    L.or_quat_mul.argtypes = [_dp, _dp, _dp]
     L.or_erfc_dev.restype = C.c_double
     L.or_erfc_dev.argtypes = [C.c_double]
     L.or_erfc_coef.restype = C.c_double
@@ -240,6 +248,41 @@ def shoemake(u1: float, u2: float, u3: float) -> np.ndarray:
     q = np.zeros(4)
     lib().or_shoemake(u1, u2, u3, q)
     return q
+
+
+def normals(chain_seed: int, cycle: int, move: int, n: int) -> np.ndarray:
+    """n successive rand_normal draws of one ChainRNG stream (src/rng.jl:109-141)."""
+    out = np.zeros(max(int(n), 1))
+    lib().or_normals(chain_seed & (2**64 - 1), cycle & (2**64 - 1), move & (2**64 - 1), int(n), out)
+    return out[:int(n)]
+
+
+def quaternions(chain_seed: int, cycle: int, move: int, n: int) -> np.ndarray:
+    """n successive rand_quaternion draws (x, y, z, w) of one ChainRNG stream."""
+    out = np.zeros((max(int(n), 1), 4))
+    lib().or_quats(chain_seed & (2**64 - 1), cycle & (2**64 - 1), move & (2**64 - 1), int(n), out)
+    return out[:int(n)]
+
+
+def quaternion_pow(q, t: float) -> np.ndarray:
+    """quaternion_pow (src/moves.jl:99-107)."""
+    out = np.zeros(4)
+    lib().or_quat_pow(np.ascontiguousarray(q, dtype=np.float64), float(t), out)
+    return out
+
+
+def qmul(q2, q1) -> np.ndarray:
+    """qmul (src/moves.jl:81-88): apply q1, then q2."""
+    out = np.zeros(4)
+    lib().or_quat_mul(np.ascontiguousarray(q2, dtype=np.float64), np.ascontiguousarray(q1, dtype=np.float64), out)
+    return out
+
+
+def rotate(q, v) -> np.ndarray:
+    """rotate (src/energy.jl:13-16)."""
+    out = np.zeros(3)
+    lib().or_rotate(np.ascontiguousarray(q, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64), out)
+    return out
 
 
 def pose(seed: int, sys: int, j: int, A: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -590,6 +633,43 @@ def block_mean_sem(x, nblocks):
     m = sum(means) / nblocks
     v = sum((y - m) ** 2 for y in means) / (nblocks - 1)
     return m, float(np.sqrt(v / nblocks))
+
+
+def fluctuation_qst(energy, loading, kT, nblocks):
+    """fluctuation_qst (src/gcmc.jl:71-113): q_st = kT - cov(U, N) / var(N) from the pooled
+    per-cycle samples, its delete-one-block jackknife error over `nblocks` contiguous blocks (the
+    remainder in the last) and the pooled correlation corr(U, N). Returns (q_st, q_err, corr)."""
+    U = np.asarray(energy, dtype=np.float64)
+    N = np.asarray(loading, dtype=np.float64)
+    ns = len(U)
+    if len(N) != ns:
+        raise ValueError("energy and loading must have the same length")
+    if nblocks < 2:
+        raise ValueError(f"nblocks={nblocks}: at least two blocks are needed for a jackknife error")
+    if ns < 2 * nblocks:
+        raise ValueError(f"{ns} cycles: need at least 2*nblocks={2 * nblocks} production cycles")
+    bl = ns // nblocks
+    sums = np.zeros((nblocks, 5))  # U, N, U^2, N^2, U N per block, summed in sample order
+    cnt = np.zeros(nblocks)
+    for b in range(nblocks):
+        lo, hi = b * bl, (ns if b == nblocks - 1 else (b + 1) * bl)
+        for i in range(lo, hi):
+            u, n = U[i], N[i]
+            sums[b] += (u, n, u * u, n * n, u * n)
+        cnt[b] = hi - lo
+    tot = sums.sum(axis=0)
+
+    def qst_of(sU, sN, sUN, sN2, n):
+        mU, mN = sU / n, sN / n
+        return kT - (sUN / n - mU * mN) / (sN2 / n - mN * mN)
+
+    q = qst_of(tot[0], tot[1], tot[4], tot[3], ns)
+    loo = np.array([qst_of(tot[0] - sums[b, 0], tot[1] - sums[b, 1], tot[4] - sums[b, 4], tot[3] - sums[b, 3],
+                           ns - cnt[b]) for b in range(nblocks)])
+    qerr = math.sqrt((nblocks - 1) / nblocks * float(np.sum((loo - loo.mean()) ** 2)))
+    mU, mN = tot[0] / ns, tot[1] / ns
+    corr = (tot[4] / ns - mU * mN) / math.sqrt((tot[2] / ns - mU * mU) * (tot[3] / ns - mN * mN))
+    return float(q), qerr, float(corr)
 
 
 def nvt_run(chains, *, T, n_warmup, n_production, n_widom_per_cycle, n_audit, step_trans, step_rot,
