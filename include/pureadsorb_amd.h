@@ -140,6 +140,23 @@ int pa_batch_framework_info(const pa_batch *batch, int32_t fw, pa_framework_info
 int pa_widom(pa_batch *batch, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
              int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out);
 
+/* Size of the device array behind pa_widom_noalloc, bytes. */
+#define PA_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* pa_widom that allocates no device memory. No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as pa_widom with the batch's bin scale as it
+ * stands (it never runs the bin-scale tuner; fix the scale with pa_batch_set_bin_scale to
+ * compare results bit for bit). Every per-call device buffer (energies, bins, pose coordinates,
+ * tile sums, scan storage) is carved from one static device array of PA_NOALLOC_SCRATCH_BYTES per
+ * device (64 MiB, held until the process ends): 1e6 insertions of a 3-site guest fit with the
+ * default chunk. A call that needs more fails with PA_ERR_ARG, stating both sizes, before any
+ * launch. All calls on one device share the array: calls that overlap in time on one device
+ * race and must be serialised by the caller. The array keeps the last call's contents; every
+ * call clears what it needs zeroed. Tables created by pa_batch_create stay the batch's own. */
+int pa_widom_noalloc(pa_batch *batch, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed, int32_t nblocks,
+                     int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out);
+
 /* Gradients of ln K_H and q_st for every system of the batch, on the same insertion streams as
  * pa_widom. No counterpart in the reference (it has no derivative path); the quantities are the
  * reweighting estimators

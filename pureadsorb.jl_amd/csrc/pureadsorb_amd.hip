@@ -1518,6 +1518,61 @@ extern "C" int pa_widom(pa_batch *b, const pa_guest *guest, double T, int64_t ni
     return widom_core(b, guest, T, ninsert, seed, nblocks, run, chunk, precision, out, nullptr);
 }
 
+// ------------------------------------------------------------------------------------------
+// pa_widom_noalloc: pa_widom with every per-call device buffer carved from one fixed array
+// ------------------------------------------------------------------------------------------
+// The array lives in this file's code object, once per device. Host code never uses its name
+// as an address: the device address comes from hipGetSymbolAddress with that device current,
+// fetched on the first call on each device and kept for the process.
+static __device__ __align__(256) unsigned char scratch[PA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`.
+int scratch_region(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        return fail(PA_ERR_DEVICE, "device %d: the no-allocation entry point serves devices 0..%d", device,
+                    SCRATCH_MAX_DEVICES - 1);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(PA_ERR_DEVICE, "HIP error %s: no device address for the scratch array on device %d",
+                        hipGetErrorString(e), device);
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return PA_OK;
+}
+}  // namespace
+
+extern "C" int pa_widom_noalloc(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed,
+                                int32_t nblocks, int32_t run, int64_t chunk, uint32_t precision, pa_widom_result *out)
+{
+    if (!b) return fail(PA_ERR_ARG, "NULL argument");
+    unsigned char *base = nullptr;
+    int st = scratch_region(b->device, &base);
+    if (st) return st;
+    // the batch's per-call buffers come from the array for this call only
+    struct Bind {
+        CallScratch &s;
+        ~Bind() { s = CallScratch{}; }
+    } bind{b->scr};
+    b->scr = CallScratch{};
+    b->scr.base = base;
+    b->scr.cap = PA_NOALLOC_SCRATCH_BYTES;
+    return widom_core(b, guest, T, ninsert, seed, nblocks, run, chunk, precision, out, nullptr);
+}
+
 extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64_t ninsert, uint64_t seed,
                              int32_t nblocks, int32_t run, uint32_t precision, pa_widom_result *out, double *dlnkh_deps,
                              double *dlnkh_dsig, double *dqst_deps, double *dqst_dsig, double *dlnkh_dq, double *dqst_dq)

@@ -27,7 +27,7 @@ PA_BATCH_HOST_ONLY = 2
 # Every symbol include/pureadsorb_amd.h declares.
 EXPORTS = (
     "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
-    "pa_batch_framework_info", "pa_widom", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
+    "pa_batch_framework_info", "pa_widom", "pa_widom_noalloc", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
     "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
@@ -188,6 +188,8 @@ def lib():
     L.pa_widom.restype = C.c_int
     L.pa_widom.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_int64,
                            C.c_uint32, vp]
+    L.pa_widom_noalloc.restype = C.c_int
+    L.pa_widom_noalloc.argtypes = L.pa_widom.argtypes
     L.pa_widom_grad.restype = C.c_int
     L.pa_widom_grad.argtypes = [vp, C.POINTER(PaGuest), C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32,
                                 C.c_uint32, vp, vp, vp, vp, vp, vp, vp]

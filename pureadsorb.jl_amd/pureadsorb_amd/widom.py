@@ -61,7 +61,7 @@ def _precision(p) -> int:
 
 def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str = "mi355x", seed: int = 0,
           chunk: int | None = None, nblocks: int = 10, run: int | None = None,
-          precision: str = "mixed", single_phase: bool = False) -> list[WidomResult]:
+          precision: str = "mixed", single_phase: bool = False, no_alloc: bool = False) -> list[WidomResult]:
     """Widom test-particle insertion on the device; one WidomResult per system.
 
     Insertion j (0-based) of system s takes its pose from ChainRNG(splitmix64(seed, s+1), j+1, 0),
@@ -71,6 +71,10 @@ def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str 
     The mixed path first rejects insertions whose Boltzmann weight a rigorous bound proves to be
     exactly 0.0 in Float64 (src/widom.jl:99-151, src/reject.jl); `single_phase=True` evaluates
     every insertion instead (the reference's `widom_singlephase`). Results are the same.
+
+    `no_alloc=True` calls pa_widom_noalloc: no device allocation, every per-call buffer from one
+    fixed 64 MiB device array shared by all such calls on the device (overlapping calls race),
+    and no bin-scale tuning (the batch's scale as it stands).
     """
     if backend not in ("mi355x", "hip"):
         raise _lib.ArgumentError(f"backend={backend!r}: this backend runs on the MI355X ('mi355x')")
@@ -84,9 +88,10 @@ def widom(batch: FrameworkBatch, guest: Guest, *, T, ninsert: int, backend: str 
     T = float(getattr(T, "magnitude", T))
     g = _GuestArgs(guest)
     out = (_lib.PaWidomResult * batch.nsys)()
-    _lib.check(_lib.lib().pa_widom(batch.handle, C.byref(g.s), T, int(ninsert), int(seed) & (2**64 - 1), int(nblocks),
-                                   0 if run is None else int(run), 0 if chunk is None else int(chunk),
-                                   _precision(precision) | (_lib.PA_SINGLE_PHASE if single_phase else 0), out))
+    fn = _lib.lib().pa_widom_noalloc if no_alloc else _lib.lib().pa_widom
+    _lib.check(fn(batch.handle, C.byref(g.s), T, int(ninsert), int(seed) & (2**64 - 1), int(nblocks),
+                  0 if run is None else int(run), 0 if chunk is None else int(chunk),
+                  _precision(precision) | (_lib.PA_SINGLE_PHASE if single_phase else 0), out))
     return [WidomResult(r.mu_ex, r.mu_ex_err, r.K_H, r.K_H_err, r.q_st, r.q_st_err, int(r.nsamples), int(r.nblocks),
                         r.ln_K_H) for r in out]
 
