@@ -153,3 +153,80 @@ def test_grad_uncharged_guest_with_tail(config0):
         _close_types(getattr(gd, key), ref[key], key)
     # an uncharged guest sees no framework charge
     assert np.all(gd.dlnkh_dq == 0.0) and np.all(ref["dlnkh_dq"] == 0.0)
+
+
+def _ff_scaled(ff, t, kind, factor):
+    """`ff` with type t's own epsilon or sigma scaled by `factor`, Lorentz-Berthelot re-mixed."""
+    eps = np.diag(ff.epsilon).copy()
+    sig = np.diag(ff.sigma).copy()
+    (eps if kind == "eps" else sig)[t] *= factor
+    return pa.ForceField.from_types(ff.names, list(sig), list(eps), cutoff=ff.cutoff, tail=ff.tail)
+
+
+def test_grad_config3_binding_frameworks_against_oracle_and_fd():
+    """Config 3's frameworks (the first 64 of the 256-framework draw, CO2, LJ + Ewald): the four
+    smallest whose device ln K_H shows binding insertions (> -50) at 20,480 insertions each, against
+    the FP64 oracle gradient; on the smallest of them, every framework type's epsilon and sigma
+    and one unit-cell charge (all its images) against central finite differences of the device's
+    FP64 Widom (relative step 1e-4, common random numbers)."""
+    fws = synth.config2_frameworks(256)[:64]
+    ff = synth.synthetic_forcefield(12.0)
+    g = synth.co2_guest(ff)
+    ew = pa.EwaldParams(12.0)
+    per = 20_480  # a multiple of the 256-insertion run: every system gets exactly this many
+    scan = pa.widom(pa.FrameworkBatch(fws, ff, g, ew), g, T=T, ninsert=64 * per, seed=21, nblocks=4, precision="mixed")
+    binding = sorted((fws[i].natoms, i) for i in range(64) if scan[i].ln_K_H > -50)
+    print(f"config 3: {len(binding)} of 64 frameworks bind; smallest {binding[:4]}", flush=True)
+    assert len(binding) >= 4
+    pick = [i for _, i in binding[:4]]
+    sub = [fws[i] for i in pick]
+    b = pa.FrameworkBatch(sub, ff, g, ew)
+    gd = pa.widom_grad(b, g, T=T, ninsert=4 * per, seed=21, nblocks=4, precision="mixed")
+    ns = pa.system_counts(4 * per, 4, pa.default_run(4 * per, 4))
+    for s, fw in enumerate(sub):
+        assert ns[s] == per
+        ref = oracle_for(fw, ff, g, ew).widom_grad(T, int(ns[s]), 21, s, nthreads=oracle_threads())
+        print(f"fw {pick[s]}: {fw.natoms} atoms, {ns[s]} insertions, ln K_H device {gd[s].result.ln_K_H:.6f} oracle "
+              f"{ref['ln_K_H']:.6f}", flush=True)
+        assert ref["ln_K_H"] > -50
+        assert abs(gd[s].result.ln_K_H - ref["ln_K_H"]) <= 1e-4
+        for key in ("dlnkh_deps", "dlnkh_dsig", "dqst_deps", "dqst_dsig"):
+            _close_types(getattr(gd[s], key), ref[key], f"fw {pick[s]} {key}")
+        for key in ("dlnkh_dq", "dqst_dq"):
+            _close_atoms(getattr(gd[s], key), ref[key], f"fw {pick[s]} {key}")
+
+    # finite differences on the smallest binding framework, FP64 device path throughout
+    fw = sub[0]
+    h, seed = 1e-4, 23
+
+    def lnk_qst(ffx, fwx):
+        bx = pa.FrameworkBatch([fwx], ffx, synth.co2_guest(ffx), ew)
+        r = pa.widom(bx, synth.co2_guest(ffx), T=T, ninsert=per, seed=seed, nblocks=4, precision="f64")[0]
+        return r.ln_K_H, r.q_st
+
+    g1 = pa.widom_grad(pa.FrameworkBatch([fw], ff, g, ew), g, T=T, ninsert=per, seed=seed, nblocks=4,
+                       precision="f64")[0]
+    for t in range(4):
+        for kind in ("eps", "sig"):
+            lp, qp = lnk_qst(_ff_scaled(ff, t, kind, 1 + h), fw)
+            lm, qm = lnk_qst(_ff_scaled(ff, t, kind, 1 - h), fw)
+            base = ff.epsilon[t, t] if kind == "eps" else ff.sigma[t, t]
+            fd_l, fd_q = (lp - lm) / (2 * h * base), (qp - qm) / (2 * h * base)
+            an_l, an_q = getattr(g1, "dlnkh_d" + kind)[t], getattr(g1, "dqst_d" + kind)[t]
+            print(f"{ff.names[t]} {kind}: dlnKH fd {fd_l:.8g} an {an_l:.8g}; dqst fd {fd_q:.8g} an {an_q:.8g}", flush=True)
+            assert abs(an_l - fd_l) <= RTOL * abs(fd_l) + 1e-9
+            assert abs(an_q - fd_q) <= RTOL * abs(fd_q) + 1e-9
+    nimg = int(np.prod(fw.replication))
+    nu = fw.natoms // nimg
+    a = int(np.argmax(np.abs(g1.dlnkh_dq[:nu])))
+    res = []
+    for d in (h, -h):
+        q = np.array(fw.charges)
+        q[a::nu] += d
+        res.append(lnk_qst(ff, pa.Framework(fw.cell, fw.frac, fw.labels, fw.symbols, q, fw.replication)))
+    fd_l = (res[0][0] - res[1][0]) / (2 * h)
+    fd_q = (res[0][1] - res[1][1]) / (2 * h)
+    print(f"atom {a} ({nimg} images): dlnKH fd {fd_l:.8g} an {nimg * g1.dlnkh_dq[a]:.8g}; dqst fd {fd_q:.8g} an "
+          f"{nimg * g1.dqst_dq[a]:.8g}", flush=True)
+    assert abs(nimg * g1.dlnkh_dq[a] - fd_l) <= RTOL * abs(fd_l)
+    assert abs(nimg * g1.dqst_dq[a] - fd_q) <= RTOL * abs(fd_q) + 1e-6
