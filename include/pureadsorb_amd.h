@@ -126,6 +126,47 @@ int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_forcefield *
                     const pa_ewald *ewald, uint32_t flags, pa_batch **out);
 void pa_batch_destroy(pa_batch *batch);
 
+/* A FrameworkBatch the host has already built, as its plain arrays (src/batch.jl:72-104), indices
+ * 0-based. Per-framework arrays hold nframeworks entries (one per distinct framework); framework
+ * f's atoms are atom_offsets[f] .. atom_offsets[f+1]-1 of positions / types / charges (any order,
+ * e.g. the reference's cell-sorted one) and its k-vectors k_offsets[f] .. k_offsets[f+1]-1 of ks /
+ * kprefactor / Shost. Volumes and inverse cells are derived from the cells. */
+typedef struct pa_batch_tables {
+    int32_t nframeworks;
+    int32_t nsys;
+    const int32_t *framework_of;       /* nsys: framework of each system */
+    const int32_t *atom_offsets;       /* nframeworks + 1, atom_offsets[0] = 0 */
+    const double *positions;           /* natoms x 3, Cartesian Å */
+    const int32_t *types;              /* natoms: compact LJ type */
+    const double *charges;             /* natoms, e */
+    const double *cells;               /* nframeworks x 9: Å, column-major, lattice vectors as columns */
+    const double *alphas;              /* nframeworks: Ewald splitting parameter, Å^-1 */
+    const int32_t *k_offsets;          /* nframeworks + 1, k_offsets[0] = 0 */
+    const double *ks;                  /* nk x 3: k-vectors, Å^-1 (reciprocal-lattice vectors of the cell) */
+    const double *kprefactor;          /* nk */
+    const double *Shost;               /* nk x 2: real and imaginary part of the host structure factor */
+    const double *constant_offset;     /* nframeworks, eV */
+    const double *self_term_halfrange; /* nframeworks, eV */
+    int32_t ntypes;                    /* compact LJ types, 1..64 */
+    const double *sigma;               /* ntypes x ntypes, Å, column-major */
+    const double *epsilon;             /* ntypes x ntypes, eV, column-major */
+    const int32_t *compact_to_orig;    /* ntypes: force-field type of each compact type */
+    const double *bs;                  /* nframeworks: hard-core bound B_s, eV (+inf: no phase-0 rejection) */
+    const double *kmin;                /* nframeworks x nsites x ntypes: K_min(site, type), eV Å */
+    double cutoff;                     /* LJ cutoff, Å */
+    double ewald_cutoff;               /* Å */
+    int32_t fullk;                     /* 1: the k tables hold every k-vector, 0: the coupled subset */
+} pa_batch_tables;
+
+/* Takes over a FrameworkBatch built by the reference (src/batch.jl:237-432), so that
+ * `widom(batch::FrameworkBatch, guest; backend)` (src/widom.jl:255-301) runs on the device
+ * without rebuilding the batch: the tables are copied as they are (constant_offset, B_s, K_min
+ * included) and uploaded once. `guest` carries force-field type indices (guest_types_orig), each
+ * of which must appear in compact_to_orig. The batch serves pa_widom, pa_widom_noalloc,
+ * pa_widom_energies, pa_hardcore_flags and pa_widom_poses; pa_widom_grad and pa_state_create need
+ * the force field itself (per-type parameters, tail flag) and refuse it. */
+int pa_batch_from_tables(const pa_batch_tables *tables, const pa_guest *guest, pa_batch **out);
+
 /* nframeworks(batch) (src/batch.jl:123) and framework_of (src/batch.jl:103), 0-based. */
 int pa_batch_nframeworks(const pa_batch *batch, int32_t *nframeworks, int32_t *nsys);
 int pa_batch_framework_of(const pa_batch *batch, int32_t *framework_of /* nsys */);

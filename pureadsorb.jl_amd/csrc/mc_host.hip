@@ -347,6 +347,9 @@ extern "C" int pa_state_create(pa_batch *b, const pa_guest *guest, const int32_t
     *out = nullptr;
     if (!b || !guest || !ncounts) return fail(PA_ERR_ARG, "NULL argument");
     if (b->host_only) return fail(PA_ERR_ARG, "batch was built with PA_BATCH_HOST_ONLY and has no device tables");
+    if (b->from_tables)
+        return fail(PA_ERR_ARG, "SystemState needs the force field (tail flag, per-type parameters): build the batch with "
+                                "pa_batch_create, not from FrameworkBatch tables");
     bool same = guest->nsites == b->nsites;
     for (int a = 0; same && a < b->nsites; ++a) {
         same = guest->types[a] == b->gtypes[a] && guest->charges[a] == b->gq[a];

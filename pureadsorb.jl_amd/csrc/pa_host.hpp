@@ -209,6 +209,7 @@ struct pa_batch {
     std::vector<hipEvent_t> kev;  // event pairs around each binned energy launch
     pa_timing last{};
     bool host_only = false;
+    bool from_tables = false;  // built by pa_batch_from_tables: no force field behind the tables
     ~pa_batch()
     {
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);

@@ -25,7 +25,7 @@ hipError_t launch_poses(const FwDesc *fw, const int *fw_of, int sys, uint64_t se
                         double *quat, hipStream_t st);
 hipError_t binned_scan_bytes(int nbins, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
-hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st);
+hipError_t launch_binned_energy(const BinnedArgs &a, hipStream_t st);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
 int binned_listcap_max(int ns);
 int binned_listcap_min();
@@ -449,6 +449,155 @@ extern "C" int pa_synchronize(void)
     return ok();
 }
 
+namespace {
+// Host tables of a batch under construction, in the layout they are uploaded in.
+struct BatchTables {
+    std::vector<double4> atom64;
+    std::vector<int> atype;
+    std::vector<uint4> atomfx;
+    std::vector<double4> k64;
+    std::vector<double2> S64;
+    std::vector<int4> kn;
+    std::vector<float2> kc;
+    std::vector<uint4> corefx;
+    std::vector<int> coreidx, cell_start;
+    std::vector<int> nbr_start;         // per framework cell (cell_start layout): first entry in nbr_atoms
+    int64_t nbr_total = 0;              // entries of nbr_atoms (filled on the device from corefx)
+    std::vector<int> cell_fw_off{0};    // prefix over frameworks of their cell counts
+    std::vector<double2> pair64;
+    std::vector<float2> pair32;
+};
+
+// Appends framework D's n atoms (fractional coordinates, Cartesian positions or NULL to take
+// A f, charges, compact LJ types) to the atom tables, with its hard-core cell grid and the
+// offsets of its near-stencil lists. L: the cell's perpendicular widths. Sets D.atom_off,
+// D.natoms, D.cg_off and D.cg_nc.
+int add_atoms(BatchTables &T, FwDesc &D, const double *frac, const double *cart, const double *q, const int *ctype,
+              int n, const double L[3])
+{
+    D.atom_off = (int)T.atom64.size();
+    D.natoms = n;
+    for (int j = 0; j < n; ++j) {
+        double f[3] = {frac[3 * j], frac[3 * j + 1], frac[3 * j + 2]}, p[3];
+        if (cart) {
+            p[0] = cart[3 * j];
+            p[1] = cart[3 * j + 1];
+            p[2] = cart[3 * j + 2];
+        } else {
+            matvec(D.A, f, p);
+        }
+        T.atom64.push_back(make_double4(p[0], p[1], p[2], q[j]));
+        T.atype.push_back(ctype[j]);
+        float qf = (float)q[j];
+        uint32_t qb;
+        std::memcpy(&qb, &qf, 4);
+        T.atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
+    }
+    // hard-core cell grid: cells of at least HC_CELL Å, at most 64 per axis; atoms sorted by cell
+    {
+        int nct = 1;
+        for (int d = 0; d < 3; ++d) {
+            D.cg_nc[d] = std::max(1, std::min(64, (int)std::floor(L[d] / HC_CELL)));
+            nct *= D.cg_nc[d];
+        }
+        D.cg_off = (int)T.cell_start.size();
+        std::vector<int> cnt(nct + 1, 0), cell_of(n);
+        for (int j = 0; j < n; ++j) {
+            const uint4 &x = T.atomfx[D.atom_off + j];
+            const uint32_t fx[3] = {x.x, x.y, x.z};
+            int c[3];
+            for (int d = 0; d < 3; ++d) c[d] = (int)(((uint64_t)fx[d] * (uint64_t)D.cg_nc[d]) >> 32);
+            cell_of[j] = c[0] + D.cg_nc[0] * (c[1] + D.cg_nc[1] * c[2]);
+            ++cnt[cell_of[j] + 1];
+        }
+        for (int c = 0; c < nct; ++c) cnt[c + 1] += cnt[c];
+        T.cell_start.insert(T.cell_start.end(), cnt.begin(), cnt.end());
+        std::vector<uint4> sorted(n);
+        std::vector<int> sidx(n);
+        std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+        for (int j = 0; j < n; ++j) {
+            const uint4 &x = T.atomfx[D.atom_off + j];
+            const int pos = cur[cell_of[j]]++;
+            sorted[pos] = make_uint4(x.x, x.y, x.z, (uint32_t)T.atype[D.atom_off + j]);
+            sidx[pos] = j;
+        }
+        T.corefx.insert(T.corefx.end(), sorted.begin(), sorted.end());
+        T.coreidx.insert(T.coreidx.end(), sidx.begin(), sidx.end());
+        // near-stencil atom lists: for every cell, the atoms of the cells within one cell of
+        // it (each cell once when an axis has fewer than 3 cells), contiguous, so phase 0's
+        // near test reads one range per site
+        const int nc0 = D.cg_nc[0], nc1 = D.cg_nc[1], nc2 = D.cg_nc[2];
+        const int k0 = std::min(3, nc0), k1 = std::min(3, nc1), k2 = std::min(3, nc2);
+        // offsets only: nbr_fill_kernel copies the lists on the device (nbr_cells order)
+        auto axis_nbrs = [](int nc, int k) {
+            std::vector<int> t((size_t)nc * k);
+            for (int h = 0; h < nc; ++h)
+                for (int i = 0; i < k; ++i) t[(size_t)h * k + i] = nbr_cell_axis(h, i, k, nc);
+            return t;
+        };
+        const std::vector<int> nb0 = axis_nbrs(nc0, k0), nb1 = axis_nbrs(nc1, k1), nb2 = axis_nbrs(nc2, k2);
+        for (int h2 = 0; h2 < nc2; ++h2)
+            for (int h1 = 0; h1 < nc1; ++h1)
+                for (int h0 = 0; h0 < nc0; ++h0) {
+                    if (T.nbr_total > (int64_t)INT32_MAX)
+                        return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
+                    T.nbr_start.push_back((int)T.nbr_total);
+                    for (int i2 = 0; i2 < k2; ++i2)
+                        for (int i1 = 0; i1 < k1; ++i1) {
+                            const int row = nc0 * (nb1[(size_t)h1 * k1 + i1] + nc1 * nb2[(size_t)h2 * k2 + i2]);
+                            for (int i0 = 0; i0 < k0; ++i0) {
+                                const int cc = nb0[(size_t)h0 * k0 + i0] + row;
+                                T.nbr_total += cnt[cc + 1] - cnt[cc];
+                            }
+                        }
+                }
+        if (T.nbr_total > (int64_t)INT32_MAX)
+            return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
+        T.nbr_start.push_back((int)T.nbr_total);
+        T.cell_fw_off.push_back(T.cell_fw_off.back() + nct);
+    }
+    return PA_OK;
+}
+
+// Creates the batch's stream and events, uploads every table and fills the near-stencil lists
+// on the device.
+int upload_batch(pa_batch *b, const BatchTables &T)
+{
+    int st = device_ready();
+    if (st) return st;
+    HIP_TRY(hipGetDevice(&b->device));
+    HIP_TRY(hipStreamCreate(&b->stream));
+    // timing-only events: no system-scope fence (no cache write-back and invalidation around the
+    // kernels they bracket); results reach the host through copies and stream synchronisation
+    for (auto &e : b->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    HIP_TRY(upload(b->d_fw, b->desc));
+    HIP_TRY(upload(b->d_fw_of, b->fw_of));
+    HIP_TRY(upload(b->d_atom64, T.atom64));
+    HIP_TRY(upload(b->d_atype, T.atype));
+    HIP_TRY(upload(b->d_atomfx, T.atomfx));
+    HIP_TRY(upload(b->d_k64, T.k64));
+    HIP_TRY(upload(b->d_S64, T.S64));
+    HIP_TRY(upload(b->d_kn, T.kn));
+    HIP_TRY(upload(b->d_kc, T.kc));
+    HIP_TRY(upload(b->d_pair64, T.pair64));
+    HIP_TRY(upload(b->d_pair32, T.pair32));
+    HIP_TRY(upload(b->d_corefx, T.corefx));
+    HIP_TRY(upload(b->d_coreidx, T.coreidx));
+    HIP_TRY(upload(b->d_cell_start, T.cell_start));
+    HIP_TRY(upload(b->d_nbr_start, T.nbr_start));
+    HIP_TRY(b->d_nbr_atoms.ensure(sizeof(uint4) * (size_t)std::max<int64_t>(1, T.nbr_total)));
+    {
+        DevAlloc d_cfo;
+        HIP_TRY(upload(d_cfo, T.cell_fw_off));
+        HIP_TRY(launch_nbr_fill(b->d_fw.as<FwDesc>(), b->nfw, d_cfo.as<int>(), T.cell_fw_off.back(),
+                                b->d_cell_start.as<int>(), b->d_nbr_start.as<int>(), b->d_corefx.as<uint4>(),
+                                b->d_nbr_atoms.as<uint4>(), nullptr));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    return PA_OK;
+}
+}  // namespace
+
 extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_forcefield *ff, const pa_guest *guest,
                                const pa_ewald *ewald, uint32_t flags, pa_batch **out)
 {
@@ -553,18 +702,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
     std::vector<double> self_q(64 * 4);
     for (int o = 0; o < 64; ++o) self_orientation(o + 1, &self_q[4 * o]);
 
-    std::vector<double4> atom64;
-    std::vector<int> atype;
-    std::vector<uint4> atomfx;
-    std::vector<double4> k64;
-    std::vector<double2> S64;
-    std::vector<int4> kn;
-    std::vector<float2> kc;
-    std::vector<uint4> corefx;
-    std::vector<int> coreidx, cell_start;
-    std::vector<int> nbr_start;   // per framework cell (cell_start layout): first entry in nbr_atoms
-    int64_t nbr_total = 0;        // entries of nbr_atoms (filled on the device from corefx)
-    std::vector<int> cell_fw_off(1, 0);  // prefix over frameworks of their cell counts
+    BatchTables T;
     b->desc.resize(b->nfw);
     b->info.resize(b->nfw);
     b->fgh.resize(b->nfw);
@@ -595,81 +733,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         D.volume = V;
         D.alpha = alpha;
         D.alphaf = (float)alpha;
-        D.atom_off = (int)atom64.size();
-        D.natoms = n;
         std::vector<double> frac(F.frac, F.frac + 3 * (size_t)n), q(F.charges, F.charges + n);
-        for (int j = 0; j < n; ++j) {
-            double f[3] = {frac[3 * j], frac[3 * j + 1], frac[3 * j + 2]}, p[3];
-            matvec(D.A, f, p);
-            atom64.push_back(make_double4(p[0], p[1], p[2], q[j]));
-            atype.push_back(orig_to_c[F.types[j]]);
-            float qf = (float)q[j];
-            uint32_t qb;
-            std::memcpy(&qb, &qf, 4);
-            atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
-        }
-        // hard-core cell grid: cells of at least HC_CELL Å, at most 64 per axis; atoms sorted by cell
+        std::vector<int> ctype(n);
+        for (int j = 0; j < n; ++j) ctype[j] = orig_to_c[F.types[j]];
         {
-            int nct = 1;
-            for (int d = 0; d < 3; ++d) {
-                D.cg_nc[d] = std::max(1, std::min(64, (int)std::floor(L[d] / HC_CELL)));
-                nct *= D.cg_nc[d];
-            }
-            D.cg_off = (int)cell_start.size();
-            std::vector<int> cnt(nct + 1, 0), cell_of(n);
-            for (int j = 0; j < n; ++j) {
-                const uint4 &x = atomfx[D.atom_off + j];
-                const uint32_t fx[3] = {x.x, x.y, x.z};
-                int c[3];
-                for (int d = 0; d < 3; ++d) c[d] = (int)(((uint64_t)fx[d] * (uint64_t)D.cg_nc[d]) >> 32);
-                cell_of[j] = c[0] + D.cg_nc[0] * (c[1] + D.cg_nc[1] * c[2]);
-                ++cnt[cell_of[j] + 1];
-            }
-            for (int c = 0; c < nct; ++c) cnt[c + 1] += cnt[c];
-            cell_start.insert(cell_start.end(), cnt.begin(), cnt.end());
-            std::vector<uint4> sorted(n);
-            std::vector<int> sidx(n);
-            std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-            for (int j = 0; j < n; ++j) {
-                const uint4 &x = atomfx[D.atom_off + j];
-                const int pos = cur[cell_of[j]]++;
-                sorted[pos] = make_uint4(x.x, x.y, x.z, (uint32_t)atype[D.atom_off + j]);
-                sidx[pos] = j;
-            }
-            corefx.insert(corefx.end(), sorted.begin(), sorted.end());
-            coreidx.insert(coreidx.end(), sidx.begin(), sidx.end());
-            // near-stencil atom lists: for every cell, the atoms of the cells within one cell of
-            // it (each cell once when an axis has fewer than 3 cells), contiguous, so phase 0's
-            // near test reads one range per site
-            const int nc0 = D.cg_nc[0], nc1 = D.cg_nc[1], nc2 = D.cg_nc[2];
-            const int k0 = std::min(3, nc0), k1 = std::min(3, nc1), k2 = std::min(3, nc2);
-            // offsets only: nbr_fill_kernel copies the lists on the device (nbr_cells order)
-            auto axis_nbrs = [](int nc, int k) {
-                std::vector<int> t((size_t)nc * k);
-                for (int h = 0; h < nc; ++h)
-                    for (int i = 0; i < k; ++i) t[(size_t)h * k + i] = nbr_cell_axis(h, i, k, nc);
-                return t;
-            };
-            const std::vector<int> nb0 = axis_nbrs(nc0, k0), nb1 = axis_nbrs(nc1, k1), nb2 = axis_nbrs(nc2, k2);
-            for (int h2 = 0; h2 < nc2; ++h2)
-                for (int h1 = 0; h1 < nc1; ++h1)
-                    for (int h0 = 0; h0 < nc0; ++h0) {
-                        if (nbr_total > (int64_t)INT32_MAX)
-                            return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
-                        nbr_start.push_back((int)nbr_total);
-                        for (int i2 = 0; i2 < k2; ++i2)
-                            for (int i1 = 0; i1 < k1; ++i1) {
-                                const int row = nc0 * (nb1[(size_t)h1 * k1 + i1] + nc1 * nb2[(size_t)h2 * k2 + i2]);
-                                for (int i0 = 0; i0 < k0; ++i0) {
-                                    const int cc = nb0[(size_t)h0 * k0 + i0] + row;
-                                    nbr_total += cnt[cc + 1] - cnt[cc];
-                                }
-                            }
-                    }
-            if (nbr_total > (int64_t)INT32_MAX)
-                return fail(PA_ERR_ARG, "the frameworks' phase-0 neighbour lists exceed 2^31 entries");
-            nbr_start.push_back((int)nbr_total);
-            cell_fw_off.push_back(cell_fw_off.back() + nct);
+            int s = add_atoms(T, D, frac.data(), nullptr, q.data(), ctype.data(), n, L);
+            if (s) return s;
         }
         const bool rep1 = F.replication[0] == 1 && F.replication[1] == 1 && F.replication[2] == 1;
         FwGradHost &gh = b->fgh[u];
@@ -699,7 +768,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         }
         KTable kt;
         double self_mean = 0.0, halfrange = 0.0;
-        D.k_off = (int)k64.size();
+        D.k_off = (int)T.k64.size();
         if (charged || !rep1) build_ktable(D.A, kmax, alpha, V, kt);
         const int nkf = (int)kt.kpref.size();
         auto coupled = [&](int i) {
@@ -752,12 +821,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         if (charged) {
             for (int i = 0; i < nkf; ++i) {
                 if (!fullk && !coupled(i)) continue;
-                k64.push_back(make_double4(kt.k[3 * i], kt.k[3 * i + 1], kt.k[3 * i + 2], kt.kpref[i]));
-                S64.push_back(make_double2(kt.Sre[i], kt.Sim[i]));
-                kn.push_back(make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0));
+                T.k64.push_back(make_double4(kt.k[3 * i], kt.k[3 * i + 1], kt.k[3 * i + 2], kt.kpref[i]));
+                T.S64.push_back(make_double2(kt.Sre[i], kt.Sim[i]));
+                T.kn.push_back(make_int4(kt.n[3 * i], kt.n[3 * i + 1], kt.n[3 * i + 2], 0));
                 gh.kn.insert(gh.kn.end(), &kt.n[3 * i], &kt.n[3 * i] + 3);
                 gh.kpref.push_back(kt.kpref[i]);
-                kc.push_back(make_float2((float)(2.0 * kt.kpref[i] * kt.Sre[i]), (float)(2.0 * kt.kpref[i] * kt.Sim[i])));
+                T.kc.push_back(make_float2((float)(2.0 * kt.kpref[i] * kt.Sre[i]), (float)(2.0 * kt.kpref[i] * kt.Sim[i])));
             }
             // orientation-averaged guest self term over the FULL k table (src/batch.jl:360-389)
             double samples[64];
@@ -795,12 +864,12 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
                             "0.001·KB·300K = %g eV; this guest/cell combination needs the per-insertion reciprocal sum",
                             halfrange, uniq[u] + 1, halfrange, 1.0e-3 * KT_REF);
         }
-        D.nk = (int)k64.size() - D.k_off;
+        D.nk = (int)T.k64.size() - D.k_off;
         D.kn_max[0] = D.kn_max[1] = D.kn_max[2] = 0;
-        for (size_t i = D.k_off; i < kn.size(); ++i) {
-            D.kn_max[0] = std::max(D.kn_max[0], std::abs(kn[i].x));
-            D.kn_max[1] = std::max(D.kn_max[1], std::abs(kn[i].y));
-            D.kn_max[2] = std::max(D.kn_max[2], std::abs(kn[i].z));
+        for (size_t i = D.k_off; i < T.kn.size(); ++i) {
+            D.kn_max[0] = std::max(D.kn_max[0], std::abs(T.kn[i].x));
+            D.kn_max[1] = std::max(D.kn_max[1], std::abs(T.kn[i].y));
+            D.kn_max[2] = std::max(D.kn_max[2], std::abs(T.kn[i].z));
         }
         // constant_offset (src/batch.jl:396-407; tail_delta src/forcefield.jl:103-124)
         double tail = 0.0;
@@ -845,23 +914,23 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         G.qke[a] = (float)(KE * guest->charges[a]);
         G.type[a] = orig_to_c[guest->types[a]];
     }
-    std::vector<double2> pair64((size_t)NS * NTC);
-    std::vector<float2> pair32((size_t)NS * NTC);
+    T.pair64.assign((size_t)NS * NTC, double2{});
+    T.pair32.assign((size_t)NS * NTC, float2{});
     for (int a = 0; a < NS; ++a)
         for (int t = 0; t < NTC; ++t) {
             int go = guest->types[a], ho = c_to_orig[t];
             double sg = ff->sigma[go + NT * ho], ep = ff->epsilon[go + NT * ho];
-            pair64[(size_t)a * NTC + t] = make_double2(sg, ep);
-            pair32[(size_t)a * NTC + t] = make_float2((float)(sg * sg), (float)(4.0 * ep));
+            T.pair64[(size_t)a * NTC + t] = make_double2(sg, ep);
+            T.pair32[(size_t)a * NTC + t] = make_float2((float)(sg * sg), (float)(4.0 * ep));
         }
 
-    b->pair64_h = pair64;
+    b->pair64_h = T.pair64;
     b->fullk = fullk;
-    b->atom64_h = atom64;
-    b->atype_h = atype;
-    b->k64_h = k64;
-    b->kn_h = kn;
-    b->S64_h = S64;
+    b->atom64_h = T.atom64;
+    b->atype_h = T.atype;
+    b->k64_h = T.k64;
+    b->kn_h = T.kn;
+    b->S64_h = T.S64;
 
     // ---- hard-core bound B_s and kmin per framework (src/batch.jl:395,409-423; src/reject.jl).
     // The attractive-pair term eps + |K| erfc(alpha r0)/r0 grows with |K|; it is evaluated on a
@@ -881,13 +950,13 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
                 std::vector<double> npos(NTC, 0.0), kmax(NTC, 0.0);
                 std::vector<std::vector<double>> negK(NTC);
                 for (int j = 0; j < D.natoms; ++j) {
-                    const int c = atype[D.atom_off + j];
-                    const double K = KE * qa * atom64[D.atom_off + j].w;
+                    const int c = T.atype[D.atom_off + j];
+                    const double K = KE * qa * T.atom64[D.atom_off + j].w;
                     if (K >= 0.0) npos[c] += 1.0;
                     else { negK[c].push_back(-K); kmax[c] = std::max(kmax[c], -K); }
                 }
                 for (int c = 0; c < NTC && !unbounded; ++c) {
-                    const double sg = pair64[(size_t)a * NTC + c].x, ep = pair64[(size_t)a * NTC + c].y;
+                    const double sg = T.pair64[(size_t)a * NTC + c].x, ep = T.pair64[(size_t)a * NTC + c].y;
                     acc += npos[c] * ep;
                     b->hc_kmin[((size_t)u * NS + a) * NTC + c] = -kmax[c];
                     if (negK[c].empty()) continue;
@@ -911,7 +980,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
             }
             double Rs = 0.0;
             for (int k = 0; k < D.nk; ++k)
-                Rs += k64[D.k_off + k].w * std::hypot(S64[D.k_off + k].x, S64[D.k_off + k].y);
+                Rs += T.k64[D.k_off + k].w * std::hypot(T.S64[D.k_off + k].x, T.S64[D.k_off + k].y);
             b->hc_bs[u] = unbounded ? INFINITY : acc + 2.0 * KE * qg_abs * Rs;
         }
     });
@@ -921,37 +990,170 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         *out = b.release();
         return ok();
     }
-    // ---- device upload
-    int st = device_ready();
-    if (st) return st;
-    HIP_TRY(hipGetDevice(&b->device));
-    HIP_TRY(hipStreamCreate(&b->stream));
-    // timing-only events: no system-scope fence (no cache write-back and invalidation around the
-    // kernels they bracket); results reach the host through copies and stream synchronisation
-    for (auto &e : b->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-    HIP_TRY(upload(b->d_fw, b->desc));
-    HIP_TRY(upload(b->d_fw_of, b->fw_of));
-    HIP_TRY(upload(b->d_atom64, atom64));
-    HIP_TRY(upload(b->d_atype, atype));
-    HIP_TRY(upload(b->d_atomfx, atomfx));
-    HIP_TRY(upload(b->d_k64, k64));
-    HIP_TRY(upload(b->d_S64, S64));
-    HIP_TRY(upload(b->d_kn, kn));
-    HIP_TRY(upload(b->d_kc, kc));
-    HIP_TRY(upload(b->d_pair64, pair64));
-    HIP_TRY(upload(b->d_pair32, pair32));
-    HIP_TRY(upload(b->d_corefx, corefx));
-    HIP_TRY(upload(b->d_coreidx, coreidx));
-    HIP_TRY(upload(b->d_cell_start, cell_start));
-    HIP_TRY(upload(b->d_nbr_start, nbr_start));
-    HIP_TRY(b->d_nbr_atoms.ensure(sizeof(uint4) * (size_t)std::max<int64_t>(1, nbr_total)));
     {
-        DevAlloc d_cfo;
-        HIP_TRY(upload(d_cfo, cell_fw_off));
-        HIP_TRY(launch_nbr_fill(b->d_fw.as<FwDesc>(), b->nfw, d_cfo.as<int>(), cell_fw_off.back(),
-                                b->d_cell_start.as<int>(), b->d_nbr_start.as<int>(), b->d_corefx.as<uint4>(),
-                                b->d_nbr_atoms.as<uint4>(), nullptr));
-        HIP_TRY(hipDeviceSynchronize());
+        int s = upload_batch(b.get(), T);
+        if (s) return s;
+    }
+    *out = b.release();
+    return ok();
+}
+
+extern "C" int pa_batch_from_tables(const pa_batch_tables *t, const pa_guest *guest, pa_batch **out)
+{
+    if (!out) return fail(PA_ERR_ARG, "out must not be NULL");
+    *out = nullptr;
+    if (!t || !guest) return fail(PA_ERR_ARG, "NULL argument");
+    const int NFW = t->nframeworks, NSYS = t->nsys, NTC = t->ntypes, NS = guest->nsites;
+    if (NFW < 1 || NSYS < 1) return fail(PA_ERR_ARG, "nframeworks=%d and nsys=%d must be >= 1", NFW, NSYS);
+    if (NS < 1 || NS > MAX_SITES) return fail(PA_ERR_ARG, "guest has %d sites; 1..%d are supported", NS, MAX_SITES);
+    if (NTC < 1 || NTC > 64) return fail(PA_ERR_ARG, "ntypes=%d: 1..64 compact LJ types are supported", NTC);
+    if (!t->framework_of || !t->atom_offsets || !t->cells || !t->alphas || !t->k_offsets || !t->constant_offset ||
+        !t->self_term_halfrange || !t->sigma || !t->epsilon || !t->compact_to_orig || !t->bs || !t->kmin)
+        return fail(PA_ERR_ARG, "NULL table");
+    if (t->atom_offsets[0] != 0 || t->k_offsets[0] != 0)
+        return fail(PA_ERR_DIM, "atom_offsets and k_offsets must start at 0");
+    for (int f = 0; f < NFW; ++f)
+        if (t->atom_offsets[f + 1] < t->atom_offsets[f] || t->k_offsets[f + 1] < t->k_offsets[f])
+            return fail(PA_ERR_DIM, "atom_offsets / k_offsets decrease at framework %d", f);
+    const int64_t natoms = t->atom_offsets[NFW], nk = t->k_offsets[NFW];
+    if (natoms > 0 && (!t->positions || !t->types || !t->charges)) return fail(PA_ERR_ARG, "NULL atom arrays");
+    if (nk > 0 && (!t->ks || !t->kprefactor || !t->Shost)) return fail(PA_ERR_ARG, "NULL k-vector arrays");
+    for (int s = 0; s < NSYS; ++s)
+        if (t->framework_of[s] < 0 || t->framework_of[s] >= NFW)
+            return fail(PA_ERR_ARG, "framework_of[%d] = %d is not a framework index 0..%d", s, t->framework_of[s],
+                        NFW - 1);
+    for (int64_t j = 0; j < natoms; ++j)
+        if (t->types[j] < 0 || t->types[j] >= NTC)
+            return fail(PA_ERR_ARG, "atom %lld has compact LJ type %d; the batch has %d", (long long)j, t->types[j], NTC);
+    // the guest in force-field types, and its compact types through compact_to_orig
+    std::vector<int> gtc(NS, -1);
+    for (int a = 0; a < NS; ++a) {
+        for (int c = 0; c < NTC && gtc[a] < 0; ++c)
+            if (t->compact_to_orig[c] == guest->types[a]) gtc[a] = c;
+        if (gtc[a] < 0)
+            return fail(PA_ERR_ARG, "guest site %d has force-field type %d, which is not among the batch's compact types", a,
+                        guest->types[a]);
+    }
+    const double rc_lj = t->cutoff, rc_ew = t->ewald_cutoff, rc = std::max(rc_lj, rc_ew);
+    if (!(rc_lj > 0) || !(rc_ew > 0)) return fail(PA_ERR_ARG, "cutoff=%g and ewald_cutoff=%g must be positive", rc_lj, rc_ew);
+
+    std::unique_ptr<pa_batch> b(new pa_batch());
+    b->nsys = NSYS;
+    b->nfw = NFW;
+    b->fw_of.assign(t->framework_of, t->framework_of + NSYS);
+    b->nsites = NS;
+    b->gsites.assign(guest->sites, guest->sites + 3 * NS);
+    b->gq.assign(guest->charges, guest->charges + NS);
+    b->gtypes.assign(guest->types, guest->types + NS);
+    b->rc_lj = rc_lj;
+    b->rc_ew = rc_ew;
+    b->ntypes_c = NTC;
+    b->c_to_orig.assign(t->compact_to_orig, t->compact_to_orig + NTC);
+    b->from_tables = true;
+    b->fullk = t->fullk != 0;
+    bool charged = false;
+    for (int a = 0; a < NS; ++a) {
+        charged = charged || guest->charges[a] != 0.0;
+        b->r_guest = std::max(b->r_guest, norm3(guest->sites + 3 * a));
+    }
+    b->desc.resize(NFW);
+    b->info.resize(NFW);
+    b->fgh.resize(NFW);
+    BatchTables T;
+    for (int u = 0; u < NFW; ++u) {
+        FwDesc &D = b->desc[u];
+        std::memcpy(D.A, t->cells + 9 * (size_t)u, sizeof D.A);
+        const double V = std::fabs(det3(D.A));
+        if (!(V > 0)) return fail(PA_ERR_ARG, "framework %d has a singular cell", u);
+        double L[3];
+        perpendicular_lengths(D.A, L);
+        int m[3];
+        for (int d = 0; d < 3; ++d) m[d] = (int)std::ceil(2.0 * rc / L[d]);
+        if (m[0] > 1 || m[1] > 1 || m[2] > 1)
+            return fail(PA_ERR_ARG,
+                        "framework %d is too small for cutoff %g (perpendicular widths %.4f, %.4f, %.4f Å must be >= %g); "
+                        "replicate it by (%d, %d, %d) first",
+                        u + 1, rc, L[0], L[1], L[2], 2 * rc, m[0], m[1], m[2]);
+        const double alpha = t->alphas[u];
+        if (charged && !(alpha * rc_ew <= PAIR_ERFC_XMAX))
+            return fail(PA_ERR_ARG, "α·ewald_cutoff = %.17g exceeds PAIR_ERFC_XMAX = %.1f (framework %d)", alpha * rc_ew,
+                        PAIR_ERFC_XMAX, u + 1);
+        inv3(D.A, D.invA);
+        for (int i = 0; i < 9; ++i) D.Af[i] = (float)(D.A[i] * 2.3283064365386963e-10);
+        D.upper = (D.A[1] == 0.0 && D.A[2] == 0.0 && D.A[5] == 0.0) ? 1 : 0;
+        b->upper_all = b->upper_all && D.upper;
+        D.volume = V;
+        D.alpha = alpha;
+        D.alphaf = (float)alpha;
+        D.const_off = t->constant_offset[u];
+        const int a0 = t->atom_offsets[u], n = t->atom_offsets[u + 1] - a0;
+        std::vector<double> frac(3 * (size_t)n);
+        for (int j = 0; j < n; ++j) matvec(D.invA, t->positions + 3 * (size_t)(a0 + j), &frac[3 * (size_t)j]);
+        {
+            int s = add_atoms(T, D, frac.data(), t->positions + 3 * (size_t)a0, t->charges + a0, t->types + a0, n, L);
+            if (s) return s;
+        }
+        FwGradHost &gh = b->fgh[u];
+        gh.frac = frac;
+        gh.counts.assign(NTC, 0.0);
+        for (int j = 0; j < n; ++j) gh.counts[t->types[a0 + j]] += 1.0;
+        // k-vectors: k = 2 pi A^-T n, so the integer coefficients are n = A^T k / (2 pi)
+        D.k_off = (int)T.k64.size();
+        D.kn_max[0] = D.kn_max[1] = D.kn_max[2] = 0;
+        for (int i = t->k_offsets[u]; charged && i < t->k_offsets[u + 1]; ++i) {
+            const double *k = t->ks + 3 * (size_t)i;
+            int nn[3];
+            for (int d = 0; d < 3; ++d) {
+                const double x = (D.A[3 * d] * k[0] + D.A[3 * d + 1] * k[1] + D.A[3 * d + 2] * k[2]) / (2.0 * PI);
+                const double r = std::nearbyint(x);
+                if (std::fabs(x - r) > 1e-6 * std::max(1.0, std::fabs(x)))
+                    return fail(PA_ERR_ARG, "k-vector %d of framework %d is not a reciprocal-lattice vector of its cell", i,
+                                u + 1);
+                nn[d] = (int)r;
+                D.kn_max[d] = std::max(D.kn_max[d], std::abs(nn[d]));
+            }
+            const double kp = t->kprefactor[i], sre = t->Shost[2 * (size_t)i], sim = t->Shost[2 * (size_t)i + 1];
+            T.k64.push_back(make_double4(k[0], k[1], k[2], kp));
+            T.S64.push_back(make_double2(sre, sim));
+            T.kn.push_back(make_int4(nn[0], nn[1], nn[2], 0));
+            T.kc.push_back(make_float2((float)(2.0 * kp * sre), (float)(2.0 * kp * sim)));
+            gh.kn.insert(gh.kn.end(), nn, nn + 3);
+            gh.kpref.push_back(kp);
+        }
+        D.nk = (int)T.k64.size() - D.k_off;
+        const int nk_u = t->k_offsets[u + 1] - t->k_offsets[u];
+        b->info[u] = FwHost{n, D.nk, nk_u, V, alpha, D.const_off, t->self_term_halfrange[u], {L[0], L[1], L[2]}};
+    }
+    // guest and pair tables from the compact LJ tables
+    GuestDev &G = b->gdev;
+    G.nsites = NS;
+    G.charged = charged ? 1 : 0;
+    for (int a = 0; a < NS; ++a) {
+        for (int d = 0; d < 3; ++d) G.site[a][d] = guest->sites[3 * a + d];
+        G.q[a] = guest->charges[a];
+        G.qke[a] = (float)(KE * guest->charges[a]);
+        G.type[a] = gtc[a];
+    }
+    T.pair64.assign((size_t)NS * NTC, double2{});
+    T.pair32.assign((size_t)NS * NTC, float2{});
+    for (int a = 0; a < NS; ++a)
+        for (int c = 0; c < NTC; ++c) {
+            const double sg = t->sigma[gtc[a] + (size_t)NTC * c], ep = t->epsilon[gtc[a] + (size_t)NTC * c];
+            T.pair64[(size_t)a * NTC + c] = make_double2(sg, ep);
+            T.pair32[(size_t)a * NTC + c] = make_float2((float)(sg * sg), (float)(4.0 * ep));
+        }
+    b->pair64_h = T.pair64;
+    b->atom64_h = T.atom64;
+    b->atype_h = T.atype;
+    b->k64_h = T.k64;
+    b->kn_h = T.kn;
+    b->S64_h = T.S64;
+    // the hard-core bound and K_min as the host computed them (src/batch.jl:395,409-423)
+    b->hc_bs.assign(t->bs, t->bs + NFW);
+    b->hc_kmin.assign(t->kmin, t->kmin + (size_t)NFW * NS * NTC);
+    {
+        int s = upload_batch(b.get(), T);
+        if (s) return s;
     }
     *out = b.release();
     return ok();
@@ -1293,7 +1495,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
             continue;
         }
         if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches], b->stream));
-        HIP_TRY(launch_binned_energy(A, b->upper_all, b->stream));
+        HIP_TRY(launch_binned_energy(A, b->stream));
         if (timed) HIP_TRY(hipEventRecord(b->kev[2 * *launches + 1], b->stream));
         ++*launches;
     }
@@ -1578,6 +1780,9 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
                              double *dlnkh_dsig, double *dqst_deps, double *dqst_dsig, double *dlnkh_dq, double *dqst_dq)
 {
     if (!b || !guest) return fail(PA_ERR_ARG, "NULL argument");
+    if (b->from_tables)
+        return fail(PA_ERR_ARG, "pa_widom_grad needs the force field (per-type parameters, tail flag): build the batch "
+                                "with pa_batch_create, not from FrameworkBatch tables");
     const int nsys = b->nsys, NT = b->nt_ff, NS = b->nsites, NTC = b->ntypes_c;
     const bool want_lj = dlnkh_deps || dlnkh_dsig || dqst_deps || dqst_dsig;
     const bool want_q = dlnkh_dq || dqst_dq;

@@ -238,21 +238,35 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
     a.order[a.bin_start[k] + atomicAdd(&a.cursor[k], 1)] = (int)i;
 }
 
-template <bool CHARGED>
-__device__ inline float pair_term(float r2, float qj, int info, const float2 *pair, int ntc, const float *qke,
-                                  float rc_lj2, float rc_ew2, float alpha, float alpha2)
+// Cutoff classes of the binned energy kernel. The ring only holds pairs inside the larger of the
+// two cutoffs, so at most one of the two terms needs its own test: CUT_NONE when the cutoffs are
+// equal or the guest is uncharged (LJ only), CUT_EW when the LJ cutoff is the larger (the
+// screened-Coulomb term is tested), CUT_LJ when the Ewald cutoff is the larger.
+constexpr int CUT_NONE = 0, CUT_EW = 1, CUT_LJ = 2;
+
+// LDS (address space 3) views: a 32-bit LDS address held in a register is dereferenced
+// directly, with no base added.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const f32x2 lds_f32x2;
+typedef __attribute__((address_space(3))) unsigned char lds_byte;
+
+// One compacted (site, atom) pair: r2 (Å^2), qq = KE q_site q_atom (FP32) and the LDS address of
+// the (sigma^2, 4 eps) pair parameters.
+template <bool CHARGED, int CUT>
+__device__ inline float pair_term(float r2, float qq, uint32_t paddr, float rc_lj2, float rc_ew2, float alpha,
+                                  float alpha2)
 {
-    // info: site << 16 | index of the (site, atom type) pair parameters
-    const int site = info >> 16, pidx = info & 0xffff;
-    r2 = fmaxf(r2, 1.0e-30f);
+    // r2 >= +0, and the bit patterns of non-negative floats order like their values: an integer
+    // max is the 1e-30 floor that keeps 1/r finite at a coincident site
+    r2 = __uint_as_float(max(__float_as_uint(r2), 0x0DA24260u));
     float e = 0.0f;
-    if (r2 < rc_lj2) {
-        const float2 p = pair[pidx];
+    if (CUT != CUT_LJ || r2 < rc_lj2) {
+        const f32x2 p = *(lds_f32x2 *)(uintptr_t)paddr;
         const float x = p.x * __builtin_amdgcn_rcpf(r2);
         const float x3 = x * x * x;
         e = p.y * x3 * (x3 - 1.0f);
     }
-    if (CHARGED && r2 < rc_ew2) {
+    if (CHARGED && (CUT != CUT_EW || r2 < rc_ew2)) {
         // pair_erfc_dev in Float32 (src/ewald.jl:242-254): the reference's 8-term series, as
         // the same polynomial in Horner form
         constexpr float XLO = (float)PAIR_ERFC_XLO;
@@ -263,7 +277,7 @@ __device__ inline float pair_term(float r2, float qj, int info, const float2 *pa
         const float tt = __builtin_amdgcn_rcpf(1.0f + (0.5f * alpha) * (r2 * rinv));
         const float xi = XA * tt + XB;
         const float arg2 = pair_erfc_poly32_log2e(xi) - (alpha2 * 1.4426950408889634f) * r2;
-        e += qke[site] * qj * (tt * __builtin_amdgcn_exp2f(arg2)) * rinv;
+        e += qq * (tt * __builtin_amdgcn_exp2f(arg2)) * rinv;
     }
     return e;
 }
@@ -287,7 +301,7 @@ __host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
     return off;
 }
 
-template <int NS, bool CHARGED, bool UPPER>
+template <int NS, bool CHARGED, int CUT>
 __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -308,6 +322,8 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     int *ring_info = (int *)(ring_q + BRING);
     off += (size_t)BIN_NW * BRING * 12;
     float2 *pair_lds = (float2 *)(smem + off);
+    // LDS address of the pair table (the ring stores per-pair parameter addresses)
+    const uint32_t pair_addr = (uint32_t)(uintptr_t)((lds_byte *)smem + off);
     off += (size_t)NS * BIN_MAX_TYPES * 8;
     float *qke_lds = (float *)(smem + off);
     off += 16 * 4;
@@ -364,9 +380,11 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                     if (combo & 2) { if (fabs(f1) >= B.alt[1]) f1 -= (f1 > 0 ? 1.0 : -1.0); else ok = false; }
                     if (combo & 4) { if (fabs(f2) >= B.alt[2]) f2 -= (f2 > 0 ? 1.0 : -1.0); else ok = false; }
                     if (ok) {
+                        // the general product: the exact zeros of an upper-triangular cell add
+                        // exact zeros
                         const double X = A0 * f0 + A3 * f1 + A6 * f2;
-                        const double Y = UPPER ? A4 * f1 + A7 * f2 : A1 * f0 + A4 * f1 + A7 * f2;
-                        const double Z = UPPER ? A8 * f2 : A2 * f0 + A5 * f1 + A8 * f2;
+                        const double Y = A1 * f0 + A4 * f1 + A7 * f2;
+                        const double Z = A2 * f0 + A5 * f1 + A8 * f2;
                         if (X * X + Y * Y + Z * Z <= rl2) {
                             inc = true;
                             e = make_float4((float)X, (float)Y, (float)Z, __uint_as_float(at.w));
@@ -418,7 +436,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             const int grel = a.order[i0 + ii];
             const uint32_t *sf = a.sitefx + (size_t)grel * 3 * NS;
             uint32_t sfx[NS][3];
-            float sxr[NS], syr[NS], szr[NS];
+            float sxr[NS], syr[NS], szr[NS], qk[NS];
 #pragma unroll
             for (int t = 0; t < NS; ++t) {
                 for (int d = 0; d < 3; ++d) sfx[t][d] = __builtin_amdgcn_readfirstlane(sf[3 * t + d]);
@@ -426,14 +444,16 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                 const double f1 = (double)(int32_t)(sfx[t][1] - cy) * FIX;
                 const double f2 = (double)(int32_t)(sfx[t][2] - cz) * FIX;
                 sxr[t] = (float)(A0 * f0 + A3 * f1 + A6 * f2);
-                syr[t] = (float)(UPPER ? A4 * f1 + A7 * f2 : A1 * f0 + A4 * f1 + A7 * f2);
-                szr[t] = (float)(UPPER ? A8 * f2 : A2 * f0 + A5 * f1 + A8 * f2);
+                syr[t] = (float)(A1 * f0 + A4 * f1 + A7 * f2);
+                szr[t] = (float)(A2 * f0 + A5 * f1 + A8 * f2);
+                qk[t] = a.guest.qke[t];
             }
             double acc = 0.0;
             uint32_t head = 0, tail = 0;
             for (int e0 = 0; e0 < cnt64; e0 += 64) {
                 const float4 E = ent[e0 + lane];
-                const int ht = etype[e0 + lane];
+                // LDS address of this atom type's parameters for site 0; site t adds t*ntc*8
+                const uint32_t hofs = pair_addr + 8u * (uint32_t)etype[e0 + lane];
 #pragma unroll
                 for (int t = 0; t < NS; ++t) {
                     const float dx = sxr[t] - E.x, dy = syr[t] - E.y, dz = szr[t] - E.z;
@@ -446,15 +466,15 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                             (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                         const uint32_t slot = (tail + below_in) & (BRING - 1);
                         ring_r2[slot] = r2;
-                        ring_q[slot] = E.w;
-                        ring_info[slot] = (t << 16) | (t * ntc + ht);
+                        if (CHARGED) ring_q[slot] = qk[t] * E.w;
+                        ring_info[slot] = (int)(hofs + (uint32_t)(t * ntc * 8));
                     }
                     tail += (uint32_t)__popcll(mask);
                     if (tail - head >= 64u) {
                         wave_lds_fence();
                         const uint32_t slot = (head + lane) & (BRING - 1);
-                        acc += (double)pair_term<CHARGED>(ring_r2[slot], ring_q[slot], ring_info[slot], pair_lds, ntc,
-                                                          qke_lds, rc_lj2, rc_ew2, alpha, alpha2);
+                        acc += (double)pair_term<CHARGED, CUT>(ring_r2[slot], CHARGED ? ring_q[slot] : 0.0f,
+                                                               (uint32_t)ring_info[slot], rc_lj2, rc_ew2, alpha, alpha2);
                         head += 64u;
                         wave_lds_fence();
                     }
@@ -463,8 +483,8 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             wave_lds_fence();
             if ((uint32_t)lane < tail - head) {
                 const uint32_t slot = (head + lane) & (BRING - 1);
-                acc += (double)pair_term<CHARGED>(ring_r2[slot], ring_q[slot], ring_info[slot], pair_lds, ntc, qke_lds,
-                                                  rc_lj2, rc_ew2, alpha, alpha2);
+                acc += (double)pair_term<CHARGED, CUT>(ring_r2[slot], CHARGED ? ring_q[slot] : 0.0f,
+                                                       (uint32_t)ring_info[slot], rc_lj2, rc_ew2, alpha, alpha2);
             }
             wave_lds_fence();
             if (CHARGED && pass == 0) {
@@ -518,19 +538,21 @@ static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t 
 }
 
 template <int NS>
-static hipError_t launch_binned_energy_ns(const BinnedArgs &a, bool upper, hipStream_t st)
+static hipError_t launch_binned_energy_ns(const BinnedArgs &a, hipStream_t st)
 {
     // the sweep reads whole 64-entry rows; a capacity below one slice of the list build could
     // never take that slice and would never finish
     if (a.listcap < BIN_WG || a.listcap % 64 != 0 || !a.extra_passes) return hipErrorInvalidValue;
     const size_t lds = binned_lds_bytes(a.listcap, NS);
     const dim3 grid((unsigned)a.nbins), block(BIN_WG);
-    if (a.guest.charged) {
-        if (upper) hipLaunchKernelGGL((widom_binned_kernel<NS, true, true>), grid, block, lds, st, a);
-        else hipLaunchKernelGGL((widom_binned_kernel<NS, true, false>), grid, block, lds, st, a);
+    if (!a.guest.charged) {
+        hipLaunchKernelGGL((widom_binned_kernel<NS, false, CUT_NONE>), grid, block, lds, st, a);
+    } else if (a.rc_lj2f == a.rc_ew2f) {
+        hipLaunchKernelGGL((widom_binned_kernel<NS, true, CUT_NONE>), grid, block, lds, st, a);
+    } else if (a.rc_lj2f > a.rc_ew2f) {
+        hipLaunchKernelGGL((widom_binned_kernel<NS, true, CUT_EW>), grid, block, lds, st, a);
     } else {
-        if (upper) hipLaunchKernelGGL((widom_binned_kernel<NS, false, true>), grid, block, lds, st, a);
-        else hipLaunchKernelGGL((widom_binned_kernel<NS, false, false>), grid, block, lds, st, a);
+        hipLaunchKernelGGL((widom_binned_kernel<NS, true, CUT_LJ>), grid, block, lds, st, a);
     }
     return hipGetLastError();
 }
@@ -582,16 +604,16 @@ hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes
 }
 
 // Step 4: the binned energy kernel.
-hipError_t launch_binned_energy(const BinnedArgs &a, bool upper, hipStream_t st)
+hipError_t launch_binned_energy(const BinnedArgs &a, hipStream_t st)
 {
     if (a.g1 <= a.g0) return hipSuccess;
     switch (a.guest.nsites) {
-    case 1: return launch_binned_energy_ns<1>(a, upper, st);
-    case 2: return launch_binned_energy_ns<2>(a, upper, st);
-    case 3: return launch_binned_energy_ns<3>(a, upper, st);
-    case 4: return launch_binned_energy_ns<4>(a, upper, st);
-    case 5: return launch_binned_energy_ns<5>(a, upper, st);
-    case 6: return launch_binned_energy_ns<6>(a, upper, st);
+    case 1: return launch_binned_energy_ns<1>(a, st);
+    case 2: return launch_binned_energy_ns<2>(a, st);
+    case 3: return launch_binned_energy_ns<3>(a, st);
+    case 4: return launch_binned_energy_ns<4>(a, st);
+    case 5: return launch_binned_energy_ns<5>(a, st);
+    case 6: return launch_binned_energy_ns<6>(a, st);
     default: return hipErrorInvalidValue;
     }
 }

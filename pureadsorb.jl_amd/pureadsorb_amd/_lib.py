@@ -26,7 +26,7 @@ PA_BATCH_HOST_ONLY = 2
 
 # Every symbol include/pureadsorb_amd.h declares.
 EXPORTS = (
-    "pa_batch_create", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
+    "pa_batch_create", "pa_batch_from_tables", "pa_batch_destroy", "pa_batch_nframeworks", "pa_batch_framework_of",
     "pa_batch_framework_info", "pa_widom", "pa_widom_noalloc", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
@@ -92,6 +92,19 @@ class PaWidomResult(C.Structure):
         ("q_st", C.c_double), ("q_st_err", C.c_double),
         ("nsamples", C.c_int64), ("nblocks", C.c_int64),
         ("ln_K_H", C.c_double),
+    ]
+
+
+class PaBatchTables(C.Structure):
+    """pa_batch_tables: a FrameworkBatch's arrays (src/batch.jl:72-104), 0-based."""
+    _fields_ = [
+        ("nframeworks", C.c_int32), ("nsys", C.c_int32), ("framework_of", C.c_void_p), ("atom_offsets", C.c_void_p),
+        ("positions", C.c_void_p), ("types", C.c_void_p), ("charges", C.c_void_p), ("cells", C.c_void_p),
+        ("alphas", C.c_void_p), ("k_offsets", C.c_void_p), ("ks", C.c_void_p), ("kprefactor", C.c_void_p),
+        ("Shost", C.c_void_p), ("constant_offset", C.c_void_p), ("self_term_halfrange", C.c_void_p),
+        ("ntypes", C.c_int32), ("sigma", C.c_void_p), ("epsilon", C.c_void_p), ("compact_to_orig", C.c_void_p),
+        ("bs", C.c_void_p), ("kmin", C.c_void_p), ("cutoff", C.c_double), ("ewald_cutoff", C.c_double),
+        ("fullk", C.c_int32),
     ]
 
 
@@ -177,6 +190,8 @@ def lib():
     L.pa_batch_create.restype = C.c_int
     L.pa_batch_create.argtypes = [C.POINTER(PaFramework), C.c_int32, C.POINTER(PaForcefield), C.POINTER(PaGuest),
                                   C.POINTER(PaEwald), C.c_uint32, C.POINTER(vp)]
+    L.pa_batch_from_tables.restype = C.c_int
+    L.pa_batch_from_tables.argtypes = [C.POINTER(PaBatchTables), C.POINTER(PaGuest), C.POINTER(vp)]
     L.pa_batch_destroy.restype = None
     L.pa_batch_destroy.argtypes = [vp]
     L.pa_batch_nframeworks.restype = C.c_int

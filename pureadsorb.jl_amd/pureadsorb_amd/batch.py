@@ -79,6 +79,72 @@ class FrameworkBatch:
         self._h = h
         self._list_capacity = 0
         self.host_only = bool(host_only)
+        self._read_info()
+
+    @classmethod
+    def from_tables(cls, tables: dict, guest: Guest) -> "FrameworkBatch":
+        """Takes over a FrameworkBatch the reference has built (src/batch.jl:72-104), given as a
+        dict of its arrays with 0-based indices: framework_of (nsys), atom_offsets, positions
+        (natoms, 3) Å, types (compact), charges, cells (nfw, 3, 3) with lattice vectors as
+        columns, alphas, k_offsets, ks (nk, 3), kprefactor, Shost (complex), constant_offset,
+        self_term_halfrange, sigma / epsilon (compact ntypes x ntypes), compact_to_orig, bs,
+        kmin (nfw, nsites, ntypes), cutoff, ewald_cutoff, fullk. `guest` carries force-field type
+        indices. The batch runs `widom`; gradients and SystemState need pa_batch_create."""
+        t = tables
+        nfw = len(t["cells"])
+        arrs = {
+            "framework_of": np.ascontiguousarray(t["framework_of"], dtype=np.int32),
+            "atom_offsets": np.ascontiguousarray(t["atom_offsets"], dtype=np.int32),
+            "positions": np.ascontiguousarray(np.asarray(t["positions"], dtype=np.float64).reshape(-1, 3)),
+            "types": np.ascontiguousarray(t["types"], dtype=np.int32),
+            "charges": np.ascontiguousarray(t["charges"], dtype=np.float64),
+            # column-major 3x3 per framework: the transpose of each row-major matrix, flattened
+            "cells": np.ascontiguousarray(np.asarray(t["cells"], dtype=np.float64).transpose(0, 2, 1).reshape(-1)),
+            "alphas": np.ascontiguousarray(t["alphas"], dtype=np.float64),
+            "k_offsets": np.ascontiguousarray(t["k_offsets"], dtype=np.int32),
+            "ks": np.ascontiguousarray(np.asarray(t["ks"], dtype=np.float64).reshape(-1, 3)),
+            "kprefactor": np.ascontiguousarray(t["kprefactor"], dtype=np.float64),
+            "Shost": np.ascontiguousarray(np.stack([np.real(t["Shost"]), np.imag(t["Shost"])], axis=-1)
+                                          .astype(np.float64).reshape(-1, 2)),
+            "constant_offset": np.ascontiguousarray(t["constant_offset"], dtype=np.float64),
+            "self_term_halfrange": np.ascontiguousarray(t["self_term_halfrange"], dtype=np.float64),
+            "sigma": np.ascontiguousarray(np.asarray(t["sigma"], dtype=np.float64).T.reshape(-1)),
+            "epsilon": np.ascontiguousarray(np.asarray(t["epsilon"], dtype=np.float64).T.reshape(-1)),
+            "compact_to_orig": np.ascontiguousarray(t["compact_to_orig"], dtype=np.int32),
+            "bs": np.ascontiguousarray(t["bs"], dtype=np.float64),
+            "kmin": np.ascontiguousarray(np.asarray(t["kmin"], dtype=np.float64).reshape(-1)),
+        }
+        pt = _lib.PaBatchTables()
+        pt.nframeworks = nfw
+        pt.nsys = len(arrs["framework_of"])
+        for k, a in arrs.items():
+            setattr(pt, k, _ptr(a))
+        pt.ntypes = len(arrs["compact_to_orig"])
+        pt.cutoff = float(t["cutoff"])
+        pt.ewald_cutoff = float(t["ewald_cutoff"])
+        pt.fullk = int(bool(t.get("fullk", False)))
+        g = _GuestArgs(guest)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().pa_batch_from_tables(C.byref(pt), C.byref(g.s), C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self._list_capacity = 0
+        self.host_only = False
+        self.nsys = pt.nsys
+        self.fullk = bool(pt.fullk)
+        self.cutoff = pt.cutoff
+        self.ewald_cutoff = pt.ewald_cutoff
+        self.ntypes = pt.ntypes
+        self.type_names = None
+        self.guest_sites_orig = np.array(guest.sites, dtype=np.float64)
+        self.guest_types_orig = np.array(guest.types, dtype=np.int32)
+        self.guest_charges_orig = np.array(guest.charges, dtype=np.float64)
+        self._read_info()
+        return self
+
+    def _read_info(self):
+        L = _lib.lib()
+        h = self._h
         nfw, nsys = C.c_int32(), C.c_int32()
         _lib.check(L.pa_batch_nframeworks(h, C.byref(nfw), C.byref(nsys)))
         self.nframeworks = nfw.value
