@@ -29,6 +29,7 @@ hipError_t launch_binned_energy(const BinnedArgs &a, hipStream_t st);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
 int binned_listcap_max(int ns);
 int binned_listcap_min();
+int binned_nclasses();
 hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
@@ -1374,6 +1375,11 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
             }
         const double R = rcmax + b->r_guest + rb + 0.02;
         B.rlist2 = (float)(R * R);
+        // radial classes of the list: everything closer than rc - r_bin - r_guest (in range of
+        // every site of every insertion of the bin) in class 0, the rest in equal shells
+        const double d0 = std::max(0.0, rcmax - rb - b->r_guest - 0.05);
+        B.cls_d0 = (float)d0;
+        B.cls_inv_w = (float)((binned_nclasses() - 1) / std::max(R - d0, 1e-3));
         B.reach[0] = B.reach[1] = B.reach[2] = -1;
         for (int d = 0; d < 3; ++d) B.alt[d] = (float)std::max(0.0, 1.0 - R / h.perp[d]);
         B.bin_base = base[s];
@@ -1591,7 +1597,7 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
         !scr.base) {
         // autotune the bin edge multiplier once per batch: time the energy pipeline on the first
         // (up to) 2^20 insertions for each candidate and keep the fastest
-        const double cand[4] = {0.75, 1.0, 1.25, 1.5};
+        const double cand[] = {0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5};
         const int64_t ge = std::min<int64_t>(G, (int64_t)1 << 20);
         double best = 1.0, best_ms = INFINITY;
         for (int rep = 0; rep < 2; ++rep)

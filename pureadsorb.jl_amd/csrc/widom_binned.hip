@@ -26,7 +26,13 @@ constexpr int BIN_WG = 768;
 constexpr int BIN_NW = BIN_WG / 64;
 static_assert(BIN_WG % 64 == 0, "a bin workgroup is whole wavefronts, and the smallest list capacity is BIN_WG");
 constexpr int BRING = 128;  // per-wave ring; drained after every site sweep (< 64 + 64 pending)
+static_assert(BRING == 128, "a drain takes the 64 entries at head, a multiple of 64, of a 128-entry ring");
 constexpr int BIN_MAX_TYPES = 64;
+constexpr int BIN_NCLS = 16;  // radial classes of a sorted neighbour list (4 bits)
+static_assert(BIN_NCLS == 16, "the in-wave class ranks use four class bits");
+// a list row lies wholly inside (outside) the cutoff of a site when its distance bounds clear the
+// cutoff by this much (Å): far above the FP32 rounding of the coordinates, r^2 and the bounds
+constexpr float ROW_MARGIN = 1.0e-3f;
 
 __device__ inline int find_sys64(const int64_t *seg, int nsys, int64_t g)
 {
@@ -298,6 +304,9 @@ __host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
     off += (size_t)ns * BIN_MAX_TYPES * 8;              // pair table
     off += 16 * 4;                                      // KE * q per site
     off += (BIN_NW + 1) * 4;                            // wave counts
+    off += 2 * BIN_NCLS * 4;                            // class counts and cursors
+    off += (size_t)BIN_NW * BIN_NCLS * 4;               // per-wave class counts / bases of a slice
+    off += (size_t)(cap / 64) * 16;                     // row bounds: (dmin, dmax), (prefix max, suffix min)
     return off;
 }
 
@@ -328,6 +337,14 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     float *qke_lds = (float *)(smem + off);
     off += 16 * 4;
     int *wcount = (int *)(smem + off);
+    off += (BIN_NW + 1) * 4;
+    int *cls_cnt = (int *)(smem + off);
+    int *cls_cur = cls_cnt + BIN_NCLS;
+    off += 2 * BIN_NCLS * 4;
+    int *wcls = (int *)(smem + off);
+    off += (size_t)BIN_NW * BIN_NCLS * 4;
+    float2 *rowb = (float2 *)(smem + off);           // per row: (dmin, dmax)
+    float2 *rowc = rowb + cap / 64;                   // per row: (prefix max of dmax, suffix min of dmin)
 
     const int s = find_sys32(a.bin_base, a.nsys, b);
     const BinSys B = a.bins[s];
@@ -339,6 +356,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     const uint32_t cz = to_fixed((bz + 0.5) / B.nb[2]);
     for (int i = tid; i < NS * ntc; i += BIN_WG) pair_lds[(i / ntc) * ntc + (i % ntc)] = a.pair32[i];
     if (tid < NS) qke_lds[tid] = a.guest.qke[tid];
+    if (tid < BIN_NCLS) cls_cnt[tid] = 0;
 
     const double A0 = F.A[0], A1 = F.A[1], A2 = F.A[2], A3 = F.A[3], A4 = F.A[4], A5 = F.A[5], A6 = F.A[6],
                  A7 = F.A[7], A8 = F.A[8];
@@ -347,87 +365,185 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     const int natoms = F.natoms;
     const double rl2 = (double)B.rlist2;
     const float rc_lj2 = a.rc_lj2f, rc_ew2 = a.rc_ew2f, rc_max2 = a.rc_max2f;
+    const float rc_max = __builtin_sqrtf(rc_max2);
     const float alpha = F.alphaf, alpha2 = F.alphaf * F.alphaf;
+    const float cls_d0 = B.cls_d0, cls_inv_w = B.cls_inv_w;
     constexpr double FIX = 2.3283064365386963e-10;  // 2^-32
+    // image combinations: bit d set = the image across the boundary on axis d (only for atoms
+    // within alt[d] of it)
+    auto combo_enabled = [&](int combo) {
+        for (int d = 0; d < 3; ++d)
+            if (((combo >> d) & 1) && !(B.alt[d] < 0.5f)) return false;
+        return true;
+    };
+    // list entry of atom n under image combination `combo`: coordinates relative to the bin
+    // centre in FP32 (computed in FP64), charge, type and radial class; false when the image is
+    // beyond the list radius
+    auto make_entry = [&](int n, int combo, float4 &e, int &et, int &cls) -> bool {
+        const uint4 at = atoms[n];
+        double f0 = (double)(int32_t)(at.x - cx) * FIX;
+        double f1 = (double)(int32_t)(at.y - cy) * FIX;
+        double f2 = (double)(int32_t)(at.z - cz) * FIX;
+        if (combo & 1) { if (fabs(f0) >= B.alt[0]) f0 -= (f0 > 0 ? 1.0 : -1.0); else return false; }
+        if (combo & 2) { if (fabs(f1) >= B.alt[1]) f1 -= (f1 > 0 ? 1.0 : -1.0); else return false; }
+        if (combo & 4) { if (fabs(f2) >= B.alt[2]) f2 -= (f2 > 0 ? 1.0 : -1.0); else return false; }
+        // the general product: the exact zeros of an upper-triangular cell add exact zeros
+        const double X = A0 * f0 + A3 * f1 + A6 * f2;
+        const double Y = A1 * f0 + A4 * f1 + A7 * f2;
+        const double Z = A2 * f0 + A5 * f1 + A8 * f2;
+        const double d2 = X * X + Y * Y + Z * Z;
+        if (!(d2 <= rl2)) return false;
+        e = make_float4((float)X, (float)Y, (float)Z, __uint_as_float(at.w));
+        et = types[n];
+        const float dr = __builtin_sqrtf((float)d2);
+        cls = dr < cls_d0 ? 0 : min(BIN_NCLS - 1, 1 + (int)((dr - cls_d0) * cls_inv_w));
+        return true;
+    };
+    __syncthreads();
+
+    // ---- pass 1: entries per radial class over every image combination (LDS counters; the
+    // counts do not depend on the order of the atomics)
+    for (int combo = 0; combo < 8; ++combo) {
+        if (!combo_enabled(combo)) continue;
+        for (int n = tid; n < natoms; n += BIN_WG) {
+            float4 e;
+            int et, cls;
+            if (make_entry(n, combo, e, et, cls)) atomicAdd(&cls_cnt[cls], 1);
+        }
+    }
+    __syncthreads();
+    int total = 0;
+    for (int k = 0; k < BIN_NCLS; ++k) total += cls_cnt[k];
+    // a list that fits is written in class order in one pass; one that does not is built in
+    // atom order over several passes
+    const bool sorted = total <= cap;
+    if (sorted && tid == 0) {
+        int run = 0;
+        for (int k = 0; k < BIN_NCLS; ++k) {
+            cls_cur[k] = run;
+            run += cls_cnt[k];
+        }
+    }
     __syncthreads();
 
     int combo = 0, p = 0, pass = 0;
     for (;;) {
         // ---- build (a slice of) the bin's neighbour list
         int cnt = 0;
-        bool full = false;
-        while (combo < 8 && !full) {
-            bool enabled = true;
-            for (int d = 0; d < 3; ++d)
-                if (((combo >> d) & 1) && !(B.alt[d] < 0.5f)) enabled = false;
-            if (!enabled) {
-                ++combo;
-                p = 0;
-                continue;
-            }
-            while (p < natoms) {
-                const int n = p + tid;
-                bool inc = false;
-                float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-                int et = 0;
-                if (n < natoms) {
-                    const uint4 at = atoms[n];
-                    double f0 = (double)(int32_t)(at.x - cx) * FIX;
-                    double f1 = (double)(int32_t)(at.y - cy) * FIX;
-                    double f2 = (double)(int32_t)(at.z - cz) * FIX;
-                    bool ok = true;
-                    if (combo & 1) { if (fabs(f0) >= B.alt[0]) f0 -= (f0 > 0 ? 1.0 : -1.0); else ok = false; }
-                    if (combo & 2) { if (fabs(f1) >= B.alt[1]) f1 -= (f1 > 0 ? 1.0 : -1.0); else ok = false; }
-                    if (combo & 4) { if (fabs(f2) >= B.alt[2]) f2 -= (f2 > 0 ? 1.0 : -1.0); else ok = false; }
-                    if (ok) {
-                        // the general product: the exact zeros of an upper-triangular cell add
-                        // exact zeros
-                        const double X = A0 * f0 + A3 * f1 + A6 * f2;
-                        const double Y = A1 * f0 + A4 * f1 + A7 * f2;
-                        const double Z = A2 * f0 + A5 * f1 + A8 * f2;
-                        if (X * X + Y * Y + Z * Z <= rl2) {
-                            inc = true;
-                            e = make_float4((float)X, (float)Y, (float)Z, __uint_as_float(at.w));
-                            et = types[n];
-                        }
+        if (sorted) {
+            // class order; within a class, by slice, wave and lane: deterministic
+            for (int cb = 0; cb < 8; ++cb) {
+                if (!combo_enabled(cb)) continue;
+                for (int p0 = 0; p0 < natoms; p0 += BIN_WG) {
+                    const int n = p0 + tid;
+                    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int et = 0, cls = 0;
+                    const bool inc = n < natoms && make_entry(n, cb, e, et, cls);
+                    // lanes of the same class: the four class bits agree
+                    uint64_t same = __builtin_amdgcn_ballot_w64(inc);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint64_t m = __builtin_amdgcn_ballot_w64(inc && ((cls >> i) & 1));
+                        same &= ((cls >> i) & 1) ? m : ~m;
                     }
+                    const uint32_t rank =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+                    if (lane < BIN_NCLS) wcls[wave * BIN_NCLS + lane] = 0;
+                    wave_lds_fence();
+                    if (inc && rank == 0) wcls[wave * BIN_NCLS + cls] = __popcll(same);
+                    __syncthreads();
+                    if (tid < BIN_NCLS) {
+                        int run = cls_cur[tid];
+                        for (int w = 0; w < BIN_NW; ++w) {
+                            const int c = wcls[w * BIN_NCLS + tid];
+                            wcls[w * BIN_NCLS + tid] = run;
+                            run += c;
+                        }
+                        cls_cur[tid] = run;
+                    }
+                    __syncthreads();
+                    if (inc) {
+                        const int slot = wcls[wave * BIN_NCLS + cls] + (int)rank;
+                        ent[slot] = e;
+                        etype[slot] = (unsigned short)et;
+                    }
+                    __syncthreads();
                 }
-                const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
-                if (lane == 0) wcount[wave] = __popcll(m);
-                __syncthreads();
-                int base = 0, total = 0;
-                for (int w = 0; w < BIN_NW; ++w) {
-                    const int c = wcount[w];
-                    base += (w < wave) ? c : 0;
-                    total += c;
-                }
-                __syncthreads();
-                // a slice adds at most BIN_WG entries and cap >= BIN_WG (checked at launch), so a
-                // pass that starts with an empty list always takes its first slice: every pass
-                // advances (combo, p)
-                if (cnt + total > cap) {
-                    full = true;
-                    break;
-                }
-                if (inc) {
-                    const int slot = cnt + base + __popcll(m & below);
-                    ent[slot] = e;
-                    etype[slot] = (unsigned short)et;
-                }
-                cnt += total;
-                p += BIN_WG;
             }
-            if (!full) {
-                ++combo;
-                p = 0;
+            cnt = total;
+            combo = 8;
+        } else {
+            bool full = false;
+            while (combo < 8 && !full) {
+                if (!combo_enabled(combo)) {
+                    ++combo;
+                    p = 0;
+                    continue;
+                }
+                while (p < natoms) {
+                    const int n = p + tid;
+                    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int et = 0, cls = 0;
+                    const bool inc = n < natoms && make_entry(n, combo, e, et, cls);
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
+                    if (lane == 0) wcount[wave] = __popcll(m);
+                    __syncthreads();
+                    int base = 0, tot = 0;
+                    for (int w = 0; w < BIN_NW; ++w) {
+                        const int c = wcount[w];
+                        base += (w < wave) ? c : 0;
+                        tot += c;
+                    }
+                    __syncthreads();
+                    // a slice adds at most BIN_WG entries and cap >= BIN_WG (checked at launch),
+                    // so a pass that starts with an empty list always takes its first slice:
+                    // every pass advances (combo, p)
+                    if (cnt + tot > cap) {
+                        full = true;
+                        break;
+                    }
+                    if (inc) {
+                        const int slot = cnt + base + __popcll(m & below);
+                        ent[slot] = e;
+                        etype[slot] = (unsigned short)et;
+                    }
+                    cnt += tot;
+                    p += BIN_WG;
+                }
+                if (!full) {
+                    ++combo;
+                    p = 0;
+                }
             }
         }
         // pad the list to a multiple of 64 with far-away sentinels (no lane of the sweep needs a
         // bounds test); listcap is a multiple of 64, so the padding stays inside the list
         const int cnt64 = (cnt + 63) & ~63;
+        const int nrows = cnt64 / 64;
         for (int e = cnt + tid; e < cnt64; e += BIN_WG) {
             ent[e] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
             etype[e] = 0;
+        }
+        __syncthreads();
+        // ---- distance bounds of every 64-entry row (a sentinel's distance is +inf)
+        for (int r = wave; r < nrows; r += BIN_NW) {
+            const float4 E = ent[r * 64 + lane];
+            float lo = __builtin_sqrtf(E.x * E.x + E.y * E.y + E.z * E.z), hi = lo;
+#pragma unroll
+            for (int off2 = 32; off2 >= 1; off2 >>= 1) {
+                lo = fminf(lo, __shfl_xor(lo, off2, 64));
+                hi = fmaxf(hi, __shfl_xor(hi, off2, 64));
+            }
+            if (lane == 0) rowb[r] = make_float2(lo, hi);
+        }
+        __syncthreads();
+        // prefix maximum of the upper bounds and suffix minimum of the lower bounds: the rows
+        // wholly inside a site's cutoff form a prefix, those wholly outside a suffix
+        for (int r = tid; r < nrows; r += BIN_WG) {
+            float pm = 0.0f, sm = INFINITY;
+            for (int q = 0; q <= r; ++q) pm = fmaxf(pm, rowb[q].y);
+            for (int q = r; q < nrows; ++q) sm = fminf(sm, rowb[q].x);
+            rowc[r] = make_float2(pm, sm);
         }
         __syncthreads();
 
@@ -450,29 +566,51 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             }
             double acc = 0.0;
             uint32_t head = 0, tail = 0;
-            for (int e0 = 0; e0 < cnt64; e0 += 64) {
-                const float4 E = ent[e0 + lane];
-                // LDS address of this atom type's parameters for site 0; site t adds t*ntc*8
-                const uint32_t hofs = pair_addr + 8u * (uint32_t)etype[e0 + lane];
 #pragma unroll
-                for (int t = 0; t < NS; ++t) {
+            for (int t = 0; t < NS; ++t) {
+                // rows [0, ra) lie inside the cutoff of this site, rows [rb, nrows) outside it
+                const float delta = __builtin_sqrtf(sxr[t] * sxr[t] + syr[t] * syr[t] + szr[t] * szr[t]);
+                const float thr_in = rc_max - delta - ROW_MARGIN, thr_out = rc_max + delta + ROW_MARGIN;
+                int ra = 0, rskip = 0;
+                for (int r0 = 0; r0 < nrows; r0 += 64) {
+                    const int r = r0 + lane;
+                    const float2 c = r < nrows ? rowc[r] : make_float2(INFINITY, INFINITY);
+                    ra += __popcll(__builtin_amdgcn_ballot_w64(c.x < thr_in));
+                    rskip += __popcll(__builtin_amdgcn_ballot_w64(r < nrows && c.y > thr_out));
+                }
+                ra = __builtin_amdgcn_readfirstlane(ra);
+                const int rend = __builtin_amdgcn_readfirstlane(nrows - rskip);
+                const uint32_t tofs = pair_addr + 8u * (uint32_t)(t * ntc);
+                for (int r = 0; r < ra; ++r) {
+                    // every entry of the row is in range: evaluate without compaction
+                    const float4 E = ent[r * 64 + lane];
+                    const uint32_t paddr = tofs + 8u * (uint32_t)etype[r * 64 + lane];
+                    const float dx = sxr[t] - E.x, dy = syr[t] - E.y, dz = szr[t] - E.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    acc += (double)pair_term<CHARGED, CUT>(r2, CHARGED ? qk[t] * E.w : 0.0f, paddr, rc_lj2, rc_ew2, alpha,
+                                                           alpha2);
+                }
+                for (int r = ra; r < rend; ++r) {
+                    const float4 E = ent[r * 64 + lane];
+                    const uint32_t paddr = tofs + 8u * (uint32_t)etype[r * 64 + lane];
                     const float dx = sxr[t] - E.x, dy = syr[t] - E.y, dz = szr[t] - E.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     // sentinel entries: r2 overflows to +inf and fails the test
                     const bool in = r2 < rc_max2;
                     const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
                     if (in) {
-                        const uint32_t below_in = __builtin_amdgcn_mbcnt_hi(
-                            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                        const uint32_t slot = (tail + below_in) & (BRING - 1);
+                        // tail + the in-range lanes below this one (mbcnt accumulates onto tail)
+                        const uint32_t slot = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, tail)) & (BRING - 1);
                         ring_r2[slot] = r2;
                         if (CHARGED) ring_q[slot] = qk[t] * E.w;
-                        ring_info[slot] = (int)(hofs + (uint32_t)(t * ntc * 8));
+                        ring_info[slot] = (int)paddr;
                     }
                     tail += (uint32_t)__popcll(mask);
                     if (tail - head >= 64u) {
                         wave_lds_fence();
-                        const uint32_t slot = (head + lane) & (BRING - 1);
+                        // head is a multiple of 64: (head + lane) mod BRING
+                        const uint32_t slot = (uint32_t)lane + (head & 64u);
                         acc += (double)pair_term<CHARGED, CUT>(ring_r2[slot], CHARGED ? ring_q[slot] : 0.0f,
                                                                (uint32_t)ring_info[slot], rc_lj2, rc_ew2, alpha, alpha2);
                         head += 64u;
@@ -482,7 +620,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             }
             wave_lds_fence();
             if ((uint32_t)lane < tail - head) {
-                const uint32_t slot = (head + lane) & (BRING - 1);
+                const uint32_t slot = (uint32_t)lane + (head & 64u);
                 acc += (double)pair_term<CHARGED, CUT>(ring_r2[slot], CHARGED ? ring_q[slot] : 0.0f,
                                                        (uint32_t)ring_info[slot], rc_lj2, rc_ew2, alpha, alpha2);
             }
@@ -568,6 +706,7 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes)
 // within BIN_LDS_BUDGET, so that two workgroups share a CU's 160 KB.
 constexpr size_t BIN_LDS_BUDGET = 80 * 1024;
 int binned_listcap_min() { return BIN_WG; }
+int binned_nclasses() { return BIN_NCLS; }
 
 int binned_listcap_max(int ns)
 {
