@@ -30,7 +30,18 @@ struct GuestDev {
     double q[MAX_SITES];
     float qke[MAX_SITES];   // KE * q_s in FP32
     int type[MAX_SITES];
+    int mirror;             // 3 sites, sites 1 and 2 mirror images through site 0, equal charges
 };
+
+// True when a 3-site guest's sites 1 and 2 are mirror images through site 0 (site1 + site2 ==
+// 2 site0 exactly) with equal charges: a linear symmetric molecule such as CO2 or a 3-site N2.
+inline bool guest_mirror(const GuestDev &G)
+{
+    if (G.nsites != 3 || G.q[1] != G.q[2]) return false;
+    for (int d = 0; d < 3; ++d)
+        if (G.site[1][d] + G.site[2][d] != 2.0 * G.site[0][d]) return false;
+    return true;
+}
 
 // Everything one energy launch needs. The flattened insertion index g runs system-major:
 // system s owns g in [seg[s], seg[s+1]); insertion j of system s is g = seg[s] + j.

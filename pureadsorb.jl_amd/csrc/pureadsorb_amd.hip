@@ -915,6 +915,7 @@ extern "C" int pa_batch_create(const pa_framework *fws, int32_t nsys, const pa_f
         G.qke[a] = (float)(KE * guest->charges[a]);
         G.type[a] = orig_to_c[guest->types[a]];
     }
+    G.mirror = guest_mirror(G) ? 1 : 0;
     T.pair64.assign((size_t)NS * NTC, double2{});
     T.pair32.assign((size_t)NS * NTC, float2{});
     for (int a = 0; a < NS; ++a)
@@ -1135,6 +1136,7 @@ extern "C" int pa_batch_from_tables(const pa_batch_tables *t, const pa_guest *gu
         G.qke[a] = (float)(KE * guest->charges[a]);
         G.type[a] = gtc[a];
     }
+    G.mirror = guest_mirror(G) ? 1 : 0;
     T.pair64.assign((size_t)NS * NTC, double2{});
     T.pair32.assign((size_t)NS * NTC, float2{});
     for (int a = 0; a < NS; ++a)
@@ -1596,9 +1598,11 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_MIXED && b->bin_scale == 0.0 && G >= ((int64_t)1 << 18) &&
         !scr.base) {
         // autotune the bin edge multiplier once per batch: time the energy pipeline on the first
-        // (up to) 2^20 insertions for each candidate and keep the fastest
+        // (up to) 2^22 insertions, one default chunk, for each candidate and keep the fastest.
+        // Insertions run system-major, so the sample holds whole systems at the call's own
+        // insertion density
         const double cand[] = {0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5};
-        const int64_t ge = std::min<int64_t>(G, (int64_t)1 << 20);
+        const int64_t ge = std::min<int64_t>(G, (int64_t)1 << 22);
         double best = 1.0, best_ms = INFINITY;
         for (int rep = 0; rep < 2; ++rep)
             for (double kappa : cand) {

@@ -8,7 +8,9 @@
 //  3. bin_scatter_kernel    groups the chunk's insertions by bin;
 //  4. widom_binned_kernel   one workgroup per bin: builds the bin's neighbour list in LDS (every
 //                           atom image within rc + r_guest + r_bin of the bin centre, FP32
-//                           coordinates relative to that centre, computed in FP64), then each
+//                           coordinates relative to that centre, computed in FP64, in order of
+//                           16 radial classes; a longer list than the LDS holds is taken in
+//                           windows of that order over several passes), then each
 //                           wavefront takes one insertion at a time, sweeps the list with lanes
 //                           over atoms, compacts in-cutoff (site, atom) pairs into a per-wave LDS
 //                           ring, and evaluates the FP32 LJ + screened-Coulomb terms on full
@@ -303,9 +305,8 @@ __host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
     off += (size_t)BIN_NW * BRING * 12;                 // rings
     off += (size_t)ns * BIN_MAX_TYPES * 8;              // pair table
     off += 16 * 4;                                      // KE * q per site
-    off += (BIN_NW + 1) * 4;                            // wave counts
-    off += 2 * BIN_NCLS * 4;                            // class counts and cursors
-    off += (size_t)BIN_NW * BIN_NCLS * 4;               // per-wave class counts / bases of a slice
+    off += BIN_NCLS * 4;                                // class sizes
+    off += (size_t)BIN_NW * BIN_NCLS * 4;               // first list position per (wave, class)
     off += (size_t)(cap / 64) * 16;                     // row bounds: (dmin, dmax), (prefix max, suffix min)
     return off;
 }
@@ -319,14 +320,14 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     const int nins = a.bin_start[b + 1] - i0;
     if (nins == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
     const int cap = a.listcap;
     const int ntc = a.ntypes_c;
 
     float4 *ent = (float4 *)smem;
     unsigned short *etype = (unsigned short *)(smem + (size_t)cap * 16);
     size_t off = (size_t)cap * 16 + (((size_t)cap * 2 + 15) & ~(size_t)15);
-    float *ring_r2 = (float *)(smem + off) + wave * BRING * 3;
+    float *ring_all = (float *)(smem + off);
+    float *ring_r2 = ring_all + wave * BRING * 3;
     float *ring_q = ring_r2 + BRING;
     int *ring_info = (int *)(ring_q + BRING);
     off += (size_t)BIN_NW * BRING * 12;
@@ -336,11 +337,8 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     off += (size_t)NS * BIN_MAX_TYPES * 8;
     float *qke_lds = (float *)(smem + off);
     off += 16 * 4;
-    int *wcount = (int *)(smem + off);
-    off += (BIN_NW + 1) * 4;
     int *cls_cnt = (int *)(smem + off);
-    int *cls_cur = cls_cnt + BIN_NCLS;
-    off += 2 * BIN_NCLS * 4;
+    off += BIN_NCLS * 4;
     int *wcls = (int *)(smem + off);
     off += (size_t)BIN_NW * BIN_NCLS * 4;
     float2 *rowb = (float2 *)(smem + off);           // per row: (dmin, dmax)
@@ -356,7 +354,6 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     const uint32_t cz = to_fixed((bz + 0.5) / B.nb[2]);
     for (int i = tid; i < NS * ntc; i += BIN_WG) pair_lds[(i / ntc) * ntc + (i % ntc)] = a.pair32[i];
     if (tid < NS) qke_lds[tid] = a.guest.qke[tid];
-    if (tid < BIN_NCLS) cls_cnt[tid] = 0;
 
     const double A0 = F.A[0], A1 = F.A[1], A2 = F.A[2], A3 = F.A[3], A4 = F.A[4], A5 = F.A[5], A6 = F.A[6],
                  A7 = F.A[7], A8 = F.A[8];
@@ -376,11 +373,10 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             if (((combo >> d) & 1) && !(B.alt[d] < 0.5f)) return false;
         return true;
     };
-    // list entry of atom n under image combination `combo`: coordinates relative to the bin
-    // centre in FP32 (computed in FP64), charge, type and radial class; false when the image is
-    // beyond the list radius
-    auto make_entry = [&](int n, int combo, float4 &e, int &et, int &cls) -> bool {
-        const uint4 at = atoms[n];
+    // list entry of the atom `at` (fixed point, charge bits in w) under image combination
+    // `combo`: coordinates relative to the bin centre in FP32 (computed in FP64), charge and
+    // radial class; false when the image is beyond the list radius
+    auto make_entry = [&](const uint4 at, int combo, float4 &e, int &cls) -> bool {
         double f0 = (double)(int32_t)(at.x - cx) * FIX;
         double f1 = (double)(int32_t)(at.y - cy) * FIX;
         double f2 = (double)(int32_t)(at.z - cz) * FIX;
@@ -394,128 +390,97 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
         const double d2 = X * X + Y * Y + Z * Z;
         if (!(d2 <= rl2)) return false;
         e = make_float4((float)X, (float)Y, (float)Z, __uint_as_float(at.w));
-        et = types[n];
         const float dr = __builtin_sqrtf((float)d2);
         cls = dr < cls_d0 ? 0 : min(BIN_NCLS - 1, 1 + (int)((dr - cls_d0) * cls_inv_w));
         return true;
     };
+    int combos = 0;
+    for (int c = 0; c < 8; ++c)
+        if (combo_enabled(c)) combos |= 1 << c;
+    // running position per class of this wave's entries: 16 ints of the wave's ring, which is
+    // idle while the list is built
+    int *tally = ring_info;
+    auto tally_of = [&](int w) { return (int *)(ring_all + w * BRING * 3 + 2 * BRING); };
+
+    // One scan over the atoms. Wave w takes the 64-atom blocks w, w + BIN_NW, ..., loads each
+    // atom once and tries every enabled image combination of it. The entries of one class are
+    // ordered by (wave, block, combination, lane), so a wave ranks its entries against its own
+    // running positions and the scan takes no workgroup barrier. With `write`, the entries whose
+    // position in the class-ordered list falls in [lo, lo + cap) are stored; without, the
+    // tallies (started at zero) end as the wave's entry counts per class.
+    auto scan = [&](bool write, int lo) {
+        for (int p0 = wave * 64; p0 < natoms; p0 += BIN_WG) {
+            const int n = p0 + lane;
+            const bool live = n < natoms;
+            const uint4 at = live ? atoms[n] : make_uint4(0u, 0u, 0u, 0u);
+            const int et = (live && write) ? types[n] : 0;
+            for (int cb = 0; cb < 8; ++cb) {
+                if (!((combos >> cb) & 1)) continue;
+                float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                int cls = 0;
+                const bool inc = live && make_entry(at, cb, e, cls);
+                uint64_t same = __builtin_amdgcn_ballot_w64(inc);
+                if (same == 0) continue;
+                // lanes of the same class: the four class bits agree
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(inc && ((cls >> i) & 1));
+                    same &= ((cls >> i) & 1) ? m : ~m;
+                }
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+                const int pos = tally[cls] + (int)rank;
+                wave_lds_fence();
+                if (inc && rank == 0) tally[cls] = pos + __popcll(same);
+                wave_lds_fence();
+                if (write && inc) {
+                    const int slot = pos - lo;
+                    if ((uint32_t)slot < (uint32_t)cap) {
+                        ent[slot] = e;
+                        etype[slot] = (unsigned short)et;
+                    }
+                }
+            }
+        }
+    };
     __syncthreads();
 
-    // ---- pass 1: entries per radial class over every image combination (LDS counters; the
-    // counts do not depend on the order of the atomics)
-    for (int combo = 0; combo < 8; ++combo) {
-        if (!combo_enabled(combo)) continue;
-        for (int n = tid; n < natoms; n += BIN_WG) {
-            float4 e;
-            int et, cls;
-            if (make_entry(n, combo, e, et, cls)) atomicAdd(&cls_cnt[cls], 1);
+    // ---- count the entries per (wave, class); then the class sizes and the first position of
+    // every (wave, class) in the class-ordered list
+    if (lane < BIN_NCLS) tally[lane] = 0;
+    wave_lds_fence();
+    scan(false, 0);
+    __syncthreads();
+    if (tid < BIN_NCLS) {
+        int tot = 0;
+        for (int w = 0; w < BIN_NW; ++w) tot += tally_of(w)[tid];
+        cls_cnt[tid] = tot;
+    }
+    __syncthreads();
+    if (tid < BIN_NCLS) {
+        int run = 0;
+        for (int k = 0; k < tid; ++k) run += cls_cnt[k];
+        for (int w = 0; w < BIN_NW; ++w) {
+            wcls[w * BIN_NCLS + tid] = run;
+            run += tally_of(w)[tid];
         }
     }
     __syncthreads();
     int total = 0;
     for (int k = 0; k < BIN_NCLS; ++k) total += cls_cnt[k];
-    // a list that fits is written in class order in one pass; one that does not is built in
-    // atom order over several passes
-    const bool sorted = total <= cap;
-    if (sorted && tid == 0) {
-        int run = 0;
-        for (int k = 0; k < BIN_NCLS; ++k) {
-            cls_cur[k] = run;
-            run += cls_cnt[k];
-        }
-    }
-    __syncthreads();
 
-    int combo = 0, p = 0, pass = 0;
+    // The list is written in class order, nearest class first. A list longer than the capacity
+    // is built and swept in windows of `cap` consecutive entries of that order, one window per
+    // pass, so every window keeps tight row bounds and its far rows are skipped.
+    int pass = 0;
     for (;;) {
-        // ---- build (a slice of) the bin's neighbour list
-        int cnt = 0;
-        if (sorted) {
-            // class order; within a class, by slice, wave and lane: deterministic
-            for (int cb = 0; cb < 8; ++cb) {
-                if (!combo_enabled(cb)) continue;
-                for (int p0 = 0; p0 < natoms; p0 += BIN_WG) {
-                    const int n = p0 + tid;
-                    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-                    int et = 0, cls = 0;
-                    const bool inc = n < natoms && make_entry(n, cb, e, et, cls);
-                    // lanes of the same class: the four class bits agree
-                    uint64_t same = __builtin_amdgcn_ballot_w64(inc);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const uint64_t m = __builtin_amdgcn_ballot_w64(inc && ((cls >> i) & 1));
-                        same &= ((cls >> i) & 1) ? m : ~m;
-                    }
-                    const uint32_t rank =
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
-                    if (lane < BIN_NCLS) wcls[wave * BIN_NCLS + lane] = 0;
-                    wave_lds_fence();
-                    if (inc && rank == 0) wcls[wave * BIN_NCLS + cls] = __popcll(same);
-                    __syncthreads();
-                    if (tid < BIN_NCLS) {
-                        int run = cls_cur[tid];
-                        for (int w = 0; w < BIN_NW; ++w) {
-                            const int c = wcls[w * BIN_NCLS + tid];
-                            wcls[w * BIN_NCLS + tid] = run;
-                            run += c;
-                        }
-                        cls_cur[tid] = run;
-                    }
-                    __syncthreads();
-                    if (inc) {
-                        const int slot = wcls[wave * BIN_NCLS + cls] + (int)rank;
-                        ent[slot] = e;
-                        etype[slot] = (unsigned short)et;
-                    }
-                    __syncthreads();
-                }
-            }
-            cnt = total;
-            combo = 8;
-        } else {
-            bool full = false;
-            while (combo < 8 && !full) {
-                if (!combo_enabled(combo)) {
-                    ++combo;
-                    p = 0;
-                    continue;
-                }
-                while (p < natoms) {
-                    const int n = p + tid;
-                    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-                    int et = 0, cls = 0;
-                    const bool inc = n < natoms && make_entry(n, combo, e, et, cls);
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
-                    if (lane == 0) wcount[wave] = __popcll(m);
-                    __syncthreads();
-                    int base = 0, tot = 0;
-                    for (int w = 0; w < BIN_NW; ++w) {
-                        const int c = wcount[w];
-                        base += (w < wave) ? c : 0;
-                        tot += c;
-                    }
-                    __syncthreads();
-                    // a slice adds at most BIN_WG entries and cap >= BIN_WG (checked at launch),
-                    // so a pass that starts with an empty list always takes its first slice:
-                    // every pass advances (combo, p)
-                    if (cnt + tot > cap) {
-                        full = true;
-                        break;
-                    }
-                    if (inc) {
-                        const int slot = cnt + base + __popcll(m & below);
-                        ent[slot] = e;
-                        etype[slot] = (unsigned short)et;
-                    }
-                    cnt += tot;
-                    p += BIN_WG;
-                }
-                if (!full) {
-                    ++combo;
-                    p = 0;
-                }
-            }
-        }
+        const int lo = pass * cap;
+        const int cnt = min(cap, total - lo);
+        // ---- build the window
+        if (lane < BIN_NCLS) tally[lane] = wcls[wave * BIN_NCLS + lane];
+        wave_lds_fence();
+        scan(true, lo);
+        __syncthreads();
         // pad the list to a multiple of 64 with far-away sentinels (no lane of the sweep needs a
         // bounds test); listcap is a multiple of 64, so the padding stays inside the list
         const int cnt64 = (cnt + 63) & ~63;
@@ -628,7 +593,34 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             if (CHARGED && pass == 0) {
                 const int4 *kn = a.kn + F.k_off;
                 const float2 *kc = a.kc + F.k_off;
-                for (int kb = 0; kb < F.nk; kb += 64) {
+                bool folded = false;
+                if constexpr (NS == 3) {
+                    if (a.guest.mirror) {
+                        // sites 1 and 2 mirror each other through site 0 with equal charges:
+                        // q0 e^{i t0} + q1 (e^{i (t0 + p)} + e^{i (t0 - p)}) = e^{i t0} (q0 + 2 q1 cos p),
+                        // p the phase of site 1 relative to site 0 (site 2's own fixed-point phase
+                        // differs from t0 - p by at most |n| 2^-31 revolutions)
+                        const uint32_t d0 = sfx[1][0] - sfx[0][0], d1 = sfx[1][1] - sfx[0][1],
+                                       d2 = sfx[1][2] - sfx[0][2];
+                        const float q0 = qke_lds[0], q1x2 = 2.0f * qke_lds[1];
+                        for (int kb = 0; kb < F.nk; kb += 64) {
+                            const int k = kb + lane;
+                            if (k < F.nk) {
+                                const int4 nn = kn[k];
+                                const float2 c = kc[k];
+                                const uint32_t ph = (uint32_t)nn.x * sfx[0][0] + (uint32_t)nn.y * sfx[0][1] +
+                                                    (uint32_t)nn.z * sfx[0][2];
+                                const uint32_t pd = (uint32_t)nn.x * d0 + (uint32_t)nn.y * d1 + (uint32_t)nn.z * d2;
+                                const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
+                                const float rvd = (float)(int32_t)pd * 2.3283064365386963e-10f;
+                                const float w = q0 + q1x2 * __builtin_amdgcn_cosf(rvd);
+                                acc += (double)(w * (c.x * __builtin_amdgcn_cosf(rev) + c.y * __builtin_amdgcn_sinf(rev)));
+                            }
+                        }
+                        folded = true;
+                    }
+                }
+                for (int kb = 0; kb < (folded ? 0 : F.nk); kb += 64) {
                     const int k = kb + lane;
                     if (k < F.nk) {
                         const int4 nn = kn[k];
@@ -654,7 +646,7 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                 else *o += Etot;
             }
         }
-        if (combo >= 8) break;
+        if (lo + cap >= total) break;
         __syncthreads();
         ++pass;
     }
@@ -678,8 +670,9 @@ static hipError_t launch_binned_ns(const BinnedArgs &a, bool upper, hipStream_t 
 template <int NS>
 static hipError_t launch_binned_energy_ns(const BinnedArgs &a, hipStream_t st)
 {
-    // the sweep reads whole 64-entry rows; a capacity below one slice of the list build could
-    // never take that slice and would never finish
+    // the sweep reads whole 64-entry rows; every pass takes the next `listcap` entries of the
+    // class-ordered list, so any positive capacity finishes (the library keeps it at least one
+    // build slice)
     if (a.listcap < BIN_WG || a.listcap % 64 != 0 || !a.extra_passes) return hipErrorInvalidValue;
     const size_t lds = binned_lds_bytes(a.listcap, NS);
     const dim3 grid((unsigned)a.nbins), block(BIN_WG);
