@@ -1383,6 +1383,7 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         B.cls_d0 = (float)d0;
         B.cls_inv_w = (float)((binned_nclasses() - 1) / std::max(R - d0, 1e-3));
         B.reach[0] = B.reach[1] = B.reach[2] = -1;
+        B.hc_rf[0] = B.hc_rf[1] = B.hc_rf[2] = 0u;
         for (int d = 0; d < 3; ++d) B.alt[d] = (float)std::max(0.0, 1.0 - R / h.perp[d]);
         B.bin_base = base[s];
         base[s + 1] = base[s] + (int)nb;
@@ -1419,8 +1420,11 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
             const FwHost &h = b->info[b->fw_of[s]];
             const FwDesc &D = b->desc[b->fw_of[s]];
             if (rmax2 > 0.0)
-                for (int d = 0; d < 3; ++d)
+                for (int d = 0; d < 3; ++d) {
                     bins[s].reach[d] = (int)std::ceil(std::sqrt(rmax2) * D.cg_nc[d] / h.perp[d]);
+                    bins[s].hc_rf[d] = (uint32_t)std::min(4294967295.0,
+                                                          std::ceil(std::sqrt(rmax2) / h.perp[d] * 4294967296.0) + 2.0);
+                }
         }
     }
     const int nbins = base[nsys];

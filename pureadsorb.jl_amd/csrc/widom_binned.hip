@@ -63,32 +63,42 @@ __device__ inline void wave_lds_fence()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Grid cells, along one axis with nc cells, that hold the coordinates within +-rf (32-bit fixed
+// point) of x: `cnt` cells from `start` upwards, wrapping (the whole axis when the interval
+// reaches around it). Atoms are binned as (x nc) >> 32.
+__device__ inline void cell_range(uint32_t x, uint32_t rf, int nc, int &start, int &cnt)
+{
+    if ((uint64_t)2 * rf + ((uint64_t)1 << 32) / (uint64_t)nc >= ((uint64_t)1 << 32)) {
+        start = 0;
+        cnt = nc;
+        return;
+    }
+    const int lo = (int)(((uint64_t)(uint32_t)(x - rf) * (uint64_t)nc) >> 32);
+    const int hi = (int)(((uint64_t)(uint32_t)(x + rf) * (uint64_t)nc) >> 32);
+    const int span = hi >= lo ? hi - lo : hi - lo + nc;
+    start = lo;
+    cnt = min(span + 1, nc);
+}
+
 // Phase-0 test of one insertion (src/widom.jl:99-151): true when some site lies closer than
 // rho_at to an atom of type t (the radii guarantee exp(-dU/kT) == 0.0 in FP64). The atoms come
-// from the hard-core grid cells within (h0, h1, h2) cells of each site's home cell (h_d < nc_d),
-// with the exact fixed-point minimum image. Atoms are sorted by cell with c0 fastest, so the
-// stencil's cells of one (c1, c2) row are one contiguous atom range (two when the row wraps):
-// one dependent cell_start lookup per row.
+// from the hard-core grid cells that hold coordinates within the largest radius of the site,
+// +-rf[d] along each axis (the radius over the perpendicular width bounds the fractional
+// offset), with the exact fixed-point minimum image. Atoms are sorted by cell with c0 fastest,
+// so the stencil's cells of one (c1, c2) row are one contiguous atom range (two when the row
+// wraps): one dependent cell_start lookup per row.
 template <int NS>
-__device__ inline bool stencil_hit(const BinnedArgs &a, const FwDesc &F, int s, const uint32_t (*sp)[3], int h0, int h1,
-                                   int h2)
+__device__ inline bool stencil_hit(const BinnedArgs &a, const FwDesc &F, int s, const uint32_t (*sp)[3],
+                                   const uint32_t rf[3])
 {
     const int *cs = a.cell_start + F.cg_off;
     const uint4 *cat = a.corefx + F.atom_off;
     const float *rho = a.rho2 + (size_t)s * NS * a.ntypes_c;
     const int nc[3] = {F.cg_nc[0], F.cg_nc[1], F.cg_nc[2]};
-    const int hr[3] = {h0, h1, h2};
     int cnt[3], start[3];
     bool hit = false;
     for (int t = 0; t < NS && !hit; ++t) {
-        for (int d = 0; d < 3; ++d) {
-            const int home = (int)(((uint64_t)sp[t][d] * (uint64_t)nc[d]) >> 32);
-            cnt[d] = min(2 * hr[d] + 1, nc[d]);
-            // start in [0, nc) without a division: home - h lies in (-nc, nc) because h < nc
-            int st = home - hr[d];
-            st = st < 0 ? st + nc[d] : st;
-            start[d] = cnt[d] == nc[d] ? 0 : st;
-        }
+        for (int d = 0; d < 3; ++d) cell_range(sp[t][d], rf[d], nc[d], start[d], cnt[d]);
         // wrapped indices: start + i < 2 nc, so one conditional subtraction replaces the modulo
         const int a0 = start[0], a1 = start[0] + cnt[0];
         const int e0 = a1 < nc[0] ? a1 : nc[0], w1 = a1 > nc[0] ? a1 - nc[0] : 0;
@@ -174,17 +184,35 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
                 const float dx = (float)(int32_t)(sp[t][0] - at.x);
                 const float dy = (float)(int32_t)(sp[t][1] - at.y);
                 const float dz = (float)(int32_t)(sp[t][2] - at.z);
-                const float ex = F.Af[0] * dx + F.Af[3] * dy + F.Af[6] * dz;
-                const float ey = F.Af[1] * dx + F.Af[4] * dy + F.Af[7] * dz;
-                const float ez = F.Af[2] * dx + F.Af[5] * dy + F.Af[8] * dz;
-                if (ex * ex + ey * ey + ez * ez < rho[t * a.ntypes_c + (int)at.w]) hit = true;
+                // FP32 distance with fused multiply-adds (the kernel's FP64 pose above keeps
+                // contraction off; the radii carry the FP32 rounding allowance either way)
+                const float ex = __builtin_fmaf(F.Af[6], dz, __builtin_fmaf(F.Af[3], dy, F.Af[0] * dx));
+                const float ey = __builtin_fmaf(F.Af[7], dz, __builtin_fmaf(F.Af[4], dy, F.Af[1] * dx));
+                const float ez = __builtin_fmaf(F.Af[8], dz, __builtin_fmaf(F.Af[5], dy, F.Af[2] * dx));
+                const float r2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+                if (r2 < rho[t * a.ntypes_c + (int)at.w]) hit = true;
             }
         if (hit) {
             a.key[g - a.g0] = -1;
             a.dU[g - a.out_base] = INFINITY;
             return;
         }
-        pending = B.reach[0] > 1 || B.reach[1] > 1 || B.reach[2] > 1;
+        // the full test is pending when some site's cells within the largest radius reach
+        // beyond the home cell's neighbours
+        if (B.reach[0] > 1 || B.reach[1] > 1 || B.reach[2] > 1) {
+#pragma unroll
+            for (int t = 0; t < NS; ++t)
+                for (int d = 0; d < 3; ++d) {
+                    const int nc = F.cg_nc[d];
+                    if (nc <= 3) continue;
+                    const int home = (int)(((uint64_t)sp[t][d] * (uint64_t)nc) >> 32);
+                    int st, cnt;
+                    cell_range(sp[t][d], B.hc_rf[d], nc, st, cnt);
+                    // cells [st, st + cnt) against [home - 1, home + 1], both wrapping
+                    const int below = home - st < 0 ? home - st + nc : home - st;
+                    if (below > 1 || cnt - below > 2) pending = true;
+                }
+        }
     }
     a.key[g - a.g0] = key;
     if (!pending) {
@@ -218,7 +246,7 @@ __global__ __launch_bounds__(256) void bin_reject_kernel(BinnedArgs a)
 #pragma unroll
         for (int t = 0; t < NS; ++t)
             for (int d = 0; d < 3; ++d) sp[t][d] = sf[3 * t + d];
-        if (stencil_hit<NS>(a, F, s, sp, B.reach[0], B.reach[1], B.reach[2])) {
+        if (stencil_hit<NS>(a, F, s, sp, B.hc_rf)) {
             a.key[rel] = -1;
             a.dU[g - a.out_base] = INFINITY;
         } else {
