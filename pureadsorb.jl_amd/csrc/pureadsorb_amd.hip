@@ -1360,21 +1360,38 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
         const double kappa = b->bin_scale > 0.0 ? b->bin_scale : 1.0;
         const double edge = kappa * std::max(BIN_EDGE_MIN, std::cbrt(BIN_TARGET_INSERTIONS * h.volume / nins));
         BinSys &B = bins[s];
-        int64_t nb = 1;
-        for (int d = 0; d < 3; ++d) {
-            B.nb[d] = std::max(1, std::min(1024, (int)std::floor(h.perp[d] / edge)));
-            nb *= B.nb[d];
-        }
-        if (nb > (1 << 22)) return fail(PA_ERR_INTERNAL, "system %d would need %lld bins", s + 1, (long long)nb);
+        for (int d = 0; d < 3; ++d) B.nb[d] = std::max(1, std::min(1024, (int)std::floor(h.perp[d] / edge)));
         // circumradius of one bin: half its longest body diagonal
-        double rb = 0.0;
-        for (int sy = -1; sy <= 1; sy += 2)
-            for (int sz = -1; sz <= 1; sz += 2) {
-                const double f[3] = {0.5 / B.nb[0], 0.5 * sy / B.nb[1], 0.5 * sz / B.nb[2]};
-                double v[3];
-                matvec(D.A, f, v);
-                rb = std::max(rb, norm3(v));
+        auto circumradius = [&]() {
+            double r = 0.0;
+            for (int sy = -1; sy <= 1; sy += 2)
+                for (int sz = -1; sz <= 1; sz += 2) {
+                    const double f[3] = {0.5 / B.nb[0], 0.5 * sy / B.nb[1], 0.5 * sz / B.nb[2]};
+                    double v[3];
+                    matvec(D.A, f, v);
+                    r = std::max(r, norm3(v));
+                }
+            return r;
+        };
+        double rb = circumradius();
+        // the list keeps each atom's nearest image and the next one along every axis, which holds
+        // every image within the list radius only while that radius is below every perpendicular
+        // width: refine the longest bin edge until it is (few insertions per system make bins as
+        // large as the cell)
+        const double pmin = std::min(h.perp[0], std::min(h.perp[1], h.perp[2]));
+        while (rcmax + b->r_guest + rb + 0.02 >= pmin * (1.0 - 1e-9)) {
+            int dm = 0;
+            double em = -1.0;
+            for (int d = 0; d < 3; ++d) {
+                const double e = norm3(D.A + 3 * d) / B.nb[d];
+                if (e > em && B.nb[d] < 1024) { em = e; dm = d; }
             }
+            if (em < 0.0) return fail(PA_ERR_INTERNAL, "system %d: no bin grid keeps the list radius below the cell widths", s + 1);
+            ++B.nb[dm];
+            rb = circumradius();
+        }
+        const int64_t nb = (int64_t)B.nb[0] * B.nb[1] * B.nb[2];
+        if (nb > (1 << 22)) return fail(PA_ERR_INTERNAL, "system %d would need %lld bins", s + 1, (long long)nb);
         const double R = rcmax + b->r_guest + rb + 0.02;
         B.rlist2 = (float)(R * R);
         // radial classes of the list: everything closer than rc - r_bin - r_guest (in range of
