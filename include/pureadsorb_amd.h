@@ -282,9 +282,24 @@ int pa_state_sk(const pa_state *state, int32_t sys, double *sk);
 /* total_energy (src/guest.jl:293-343) of every system from its poses, FP64, with the structure
  * factor rebuilt from the poses. */
 int pa_total_energy(pa_state *state, double *energy);
-/* audit_energy! (src/audit.jl:97-165) on every system; on success resets the running structure
- * factor and energy to the rebuilt values. discrepancy (nullable): recomputed - running, eV. */
+/* audit_energy! (src/audit.jl:97-165) on every system, the from-scratch rebuild on the device;
+ * errors read as the reference's ("energy audit failed for system n (Float64): ..."). On
+ * success resets the running structure factor and energy to the rebuilt values.
+ * discrepancy (nullable): recomputed - running, eV. */
 int pa_audit_energy(pa_state *state, double *discrepancy);
+/* Test-only (no counterpart in the reference; for the audit's fault-injection tests, after
+ * test/audit_tests.jl:60-415): overwrites one element of system `sys`'s running state.
+ * PA_FIELD_ENERGY: running energy = re (eV). PA_FIELD_SK: running structure factor at k-vector
+ * `index` = re + i im. PA_FIELD_HOST_ENERGY: cached host-guest energy of live guest `index` = re
+ * (eV). PA_FIELD_OCCUPANCY: occupancy = re (an integer; raising it within the capacity is refused,
+ * it would expose unset slots). An occupancy above its capacity makes every entry point that
+ * reads the guest slots refuse the state (PA_ERR_ARG) until it is set back; pa_audit_energy
+ * reports it. */
+#define PA_FIELD_ENERGY 1
+#define PA_FIELD_SK 2
+#define PA_FIELD_HOST_ENERGY 3
+#define PA_FIELD_OCCUPANCY 4
+int pa_state_set_value(pa_state *state, int32_t field, int32_t sys, int64_t index, double re, double im);
 
 /* GCMC driver (src/gcmc.jl:161-337). A cycle of chain n is max(its occupancy, min_cycle_length)
  * attempts; attempt c of chain n draws its move type from ChainRNG(chain seed, c, 1) word 0 with

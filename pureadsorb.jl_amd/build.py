@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "pureadsorb_amd", "libpureadsorb_amd.so")
 SOURCES = ["pureadsorb_amd.hip", "widom_kernels.hip", "widom_binned.hip", "widom_grad.hip", "mc_kernels.hip",
-           "mc_host.hip", "nvt_kernels.hip", "batch_kernels.hip", "io_readers.cpp"]
+           "mc_host.hip", "audit_kernels.hip", "nvt_kernels.hip", "batch_kernels.hip", "io_readers.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -14,6 +14,7 @@ namespace pa {
 hipError_t launch_mc(const McArgs &a, hipStream_t st);
 hipError_t launch_chain_widom(const ChainWidomArgs &a, hipStream_t st);
 hipError_t launch_chain_snapshot(const ChainSnap &S, int64_t slot, const McArgs &m, hipStream_t st);
+hipError_t launch_audit(const AuditArgs &a, hipStream_t st);
 }  // namespace pa
 
 using namespace pa;
@@ -35,6 +36,13 @@ struct pa_state {
     DevAlloc d_atomf64, d_atype_s, d_chunk, d_chunk_off, d_lines, d_rounds, d_round_off, d_perm, d_coff,
         d_half_perp, d_ct, d_snap_pos, d_snap_quat, d_snap_sk, d_snap_occ, d_item_off, d_cw_dU, d_pose_pos,
         d_pose_quat;
+    // the audit's device rebuild: per-axis power tables, structure factor, per-slot host-guest
+    // energies, per-system sums
+    DevAlloc d_aud_tab, d_aud_sk, d_aud_he, d_aud_sums;
+    // set when pa_state_set_value put an occupancy above its capacity: every entry point that
+    // reads the guest slots refuses the state until the occupancy is set back
+    bool occ_over_capacity = false;
+    std::vector<int> occ_live;  // live guests per system when an occupancy was first set over its capacity
     double cull_extra = 0.0;
     int64_t ctot = 0;       // entries of one snapshot slot's weighted structure factors
     int64_t snap_slots = 0; // slots allocated
@@ -53,6 +61,16 @@ struct pa_state {
 namespace {
 
 int ensure_chain_tables(pa_state *st);
+
+// Entry points that read the guest slots (or launch kernels over them) refuse a state whose
+// occupancy a test put above a capacity.
+int refuse_over_capacity(const pa_state *st)
+{
+    if (st->occ_over_capacity)
+        return fail(PA_ERR_ARG, "a system's occupancy exceeds its capacity (set through pa_state_set_value); only "
+                                "pa_audit_energy, pa_state_info and pa_state_set_value take this state until it is set back");
+    return PA_OK;
+}
 
 // ---- FP64 host energies (the same terms the kernel sums, in one serial order) ----------------
 
@@ -486,6 +504,7 @@ static int run_moves(pa_state *st, int movetype, int64_t nmoves, const double *s
                      const double *fugacity_pa, double T, double *trace)
 {
     if (!st) return fail(PA_ERR_ARG, "NULL state");
+    if (int g = refuse_over_capacity(st)) return g;
     if (movetype < MOVE_TRANSLATION || movetype > MOVE_DELETION)
         return fail(PA_ERR_ARG, "movetype=%d must be one of PA_MOVE_TRANSLATION .. PA_MOVE_DELETION", movetype);
     if (nmoves < 0) return fail(PA_ERR_ARG, "nmoves=%lld must be >= 0", (long long)nmoves);
@@ -593,6 +612,7 @@ extern "C" int pa_state_guests(const pa_state *st, int32_t sys, double *refpoint
                                double *host_energy)
 {
     if (!st) return fail(PA_ERR_ARG, "NULL state");
+    if (int g = refuse_over_capacity(st)) return g;
     if (sys < 0 || sys >= st->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, st->nsys - 1);
     int occ = 0;
     HIP_TRY(hipMemcpy(&occ, st->d_occ.as<int>() + sys, sizeof(int), hipMemcpyDeviceToHost));
@@ -645,6 +665,7 @@ static int state_snapshot(pa_state *st, std::vector<double4> &pos, std::vector<d
 extern "C" int pa_total_energy(pa_state *st, double *E)
 {
     if (!st || !E) return fail(PA_ERR_ARG, "NULL argument");
+    if (int g = refuse_over_capacity(st)) return g;
     std::vector<double4> pos, quat;
     std::vector<int> occ;
     std::vector<std::vector<double2>> sk;
@@ -658,61 +679,117 @@ extern "C" int pa_total_energy(pa_state *st, double *E)
 // audit_energy! (src/audit.jl:97-165) on every system: occupancy within capacity, the running
 // structure factor against one rebuilt from the poses, then the running energy against a
 // from-scratch total; on success both are reset to the rebuilt values and the rounding
-// accumulators cleared.
+// accumulators cleared. The rebuild runs on the device (audit_kernels.hip); pa_total_energy
+// keeps the serial host rebuild, which the tests hold it to.
 extern "C" int pa_audit_energy(pa_state *st, double *discrepancy)
 {
     if (!st) return fail(PA_ERR_ARG, "NULL state");
     pa_batch *b = st->b;
-    std::vector<double4> pos, quat;
+    const int nsys = st->nsys, NS = b->nsites;
     std::vector<int> occ, acc;
-    std::vector<std::vector<double2>> skr;
-    std::vector<double> E, Erun, Eabs, skabs;
-    std::vector<double2> skrun;
-    int s = state_snapshot(st, pos, quat, occ, skr, E);
-    if (s) return s;
-    HIP_TRY(download(Erun, st->d_energy, st->nsys));
-    HIP_TRY(download(Eabs, st->d_energy_abs, st->nsys));
-    HIP_TRY(download(skrun, st->d_Sk, std::max(1, st->sk_off[st->nsys])));
-    HIP_TRY(download(skabs, st->d_skabs, std::max(1, st->sk_off[st->nsys])));
-    HIP_TRY(download(acc, st->d_acc, (size_t)st->nsys * NMOVES));
+    HIP_TRY(download(occ, st->d_occ, nsys));
+    // occupancy first: the rebuild reads every live guest's slot
+    for (int n = 0; n < nsys; ++n) {
+        const int cap = st->guest_off[n + 1] - st->guest_off[n];
+        if (occ[n] > cap)
+            return fail(PA_ERR_ARG, "energy audit failed for system %d (Float64): occupancy %d exceeds capacity %d", n + 1,
+                        occ[n], cap);
+    }
+    const int nslots = st->guest_off[nsys], nsk = st->sk_off[nsys];
+    int tw = 1;
+    for (int n = 0; n < nsys; ++n) {
+        const FwDesc &F = b->desc[b->fw_of[n]];
+        tw = std::max(tw, F.kn_max[0] + F.kn_max[1] + F.kn_max[2] + 3);
+    }
+    HIP_TRY(st->d_aud_tab.ensure(sizeof(double2) * std::max<size_t>(1, (size_t)nslots * NS * tw)));
+    HIP_TRY(st->d_aud_sk.ensure(sizeof(double2) * std::max(1, nsk)));
+    HIP_TRY(st->d_aud_he.ensure(sizeof(double) * std::max(1, nslots)));
+    HIP_TRY(st->d_aud_sums.ensure(sizeof(double) * 3 * nsys));
+    AuditArgs A{};
+    A.fw = b->d_fw.as<FwDesc>();
+    A.fw_of = b->d_fw_of.as<int>();
+    A.nsys = nsys;
+    A.atom64 = b->d_atom64.as<double4>();
+    A.atype = b->d_atype.as<int>();
+    A.k64 = b->d_k64.as<double4>();
+    A.S64 = b->d_S64.as<double2>();
+    A.kn = b->d_kn.as<int4>();
+    A.pair64 = b->d_pair64.as<double2>();
+    A.ntypes_c = b->ntypes_c;
+    A.rc_lj2 = b->rc_lj * b->rc_lj;
+    A.rc_ew2 = b->gdev.charged ? b->rc_ew * b->rc_ew : 0.0;
+    A.guest = b->gdev;
+    A.guest_off = st->d_guest_off.as<int>();
+    A.sk_off = st->d_sk_off.as<int>();
+    A.occ = st->d_occ.as<int>();
+    A.pos = st->d_refpos.as<double4>();
+    A.quat = st->d_quat.as<double4>();
+    A.nslots = nslots;
+    A.nsk = nsk;
+    A.tw = tw;
+    A.tab = st->d_aud_tab.as<double2>();
+    A.sk = st->d_aud_sk.as<double2>();
+    A.host_e = st->d_aud_he.as<double>();
+    A.sums = st->d_aud_sums.as<double>();
+    HIP_TRY(launch_audit(A, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+
+    std::vector<double> sums, Erun, Eabs, skabs;
+    std::vector<double2> skrun, skr;
+    HIP_TRY(download(sums, st->d_aud_sums, (size_t)3 * nsys));
+    HIP_TRY(download(skr, st->d_aud_sk, std::max(1, nsk)));
+    HIP_TRY(download(Erun, st->d_energy, nsys));
+    HIP_TRY(download(Eabs, st->d_energy_abs, nsys));
+    HIP_TRY(download(skrun, st->d_Sk, std::max(1, nsk)));
+    HIP_TRY(download(skabs, st->d_skabs, std::max(1, nsk)));
+    HIP_TRY(download(acc, st->d_acc, (size_t)nsys * NMOVES));
     const double eps = DBL_EPSILON;
     auto tol = [&](double abs_accum, double mag, double nm) {
         return nm * eps * std::max(abs_accum, 1.0) + eps * std::max(mag, 1.0);
     };
-    for (int n = 0; n < st->nsys; ++n) {
-        const int cap = st->guest_off[n + 1] - st->guest_off[n];
-        if (occ[n] > cap)
-            return fail(PA_ERR_ARG, "energy audit failed for system %d: occupancy %d exceeds capacity %d", n + 1, occ[n], cap);
+    double qabs = 0.0;
+    for (int a = 0; a < NS; ++a) qabs += std::fabs(b->gq[a]);
+    for (int n = 0; n < nsys; ++n) {
         int nacc = 0;
         for (int t = 0; t < NMOVES; ++t) nacc += acc[n * NMOVES + t];
         const double nmoves = std::max(1, nacc - st->acc_at_audit[n]);
         const int u = b->fw_of[n];
-        double qabs = 0.0;
-        for (int a = 0; a < b->nsites; ++a) qabs += std::fabs(b->gq[a]);
-        const double nterms = (double)occ[n] * b->nsites;
-        for (size_t k = 0; k < skr[n].size(); ++k) {
-            const double2 r = skrun[st->sk_off[n] + k], w = skr[n][k], h = b->S64_h[b->desc[u].k_off + k];
+        const FwDesc &F = b->desc[u];
+        const double nterms = (double)occ[n] * NS;
+        for (int k = 0; k < F.nk; ++k) {
+            const int j = st->sk_off[n] + k;
+            const double2 r = skrun[j], w = skr[j], h = b->S64_h[F.k_off + k];
             const double mag = nterms * (std::hypot(h.x, h.y) + occ[n] * qabs);
             const double d = std::hypot(r.x - w.x, r.y - w.y);
-            if (d > tol(skabs[st->sk_off[n] + k], mag, nmoves))
+            const double tk = tol(skabs[j], mag, nmoves);
+            if (!(d <= tk))
                 return fail(PA_ERR_ARG,
-                            "energy audit failed for system %d: structure factor at k-vector %zu disagrees with the poses: "
-                            "running=(%g, %g), rebuilt=(%g, %g), discrepancy=%g over %g accepted move(s) since the last "
-                            "audit",
-                            n + 1, k + 1, r.x, r.y, w.x, w.y, d, nmoves);
+                            "energy audit failed for system %d (Float64): structure factor at k-vector %d disagrees with "
+                            "the poses: running=%.17g %c %.17gim, rebuilt=%.17g %c %.17gim, discrepancy=%.17g exceeds "
+                            "tolerance %.17g over %.0f accepted move(s) since the last audit",
+                            n + 1, k + 1, r.x, r.y < 0 ? '-' : '+', std::fabs(r.y), w.x, w.y < 0 ? '-' : '+',
+                            std::fabs(w.y), d, tk, nmoves);
         }
+        // total_energy (src/guest.jl:293-343) from the device sums: host-guest, guest-guest,
+        // reciprocal of the rebuilt structure factor, then the per-guest self terms, the tail
+        // correction and the net-charge term
+        double gse = 0.0;
+        if (int e = guest_self_h(b, F.alpha, &gse)) return e;
+        const double E = sums[3 * n] + sums[3 * n + 1] + KE * sums[3 * n + 2] + occ[n] * gse + guest_tail_h(b, u, occ[n]) +
+                         net_h(b, u, occ[n]);
         const double nte = occ[n] + 0.5 * occ[n] * (occ[n] - 1);
-        const double mag = std::max({std::fabs(Erun[n]), std::fabs(E[n]), nte * b->hc_bs[u]});
-        const double d = std::fabs(E[n] - Erun[n]);
-        if (discrepancy) discrepancy[n] = E[n] - Erun[n];
-        if (d > tol(Eabs[n], std::isfinite(mag) ? mag : std::max(std::fabs(Erun[n]), std::fabs(E[n])), nmoves))
+        const double mag = std::max({std::fabs(Erun[n]), std::fabs(E), nte * b->hc_bs[u]});
+        const double d = std::fabs(E - Erun[n]);
+        if (discrepancy) discrepancy[n] = E - Erun[n];
+        const double te = tol(Eabs[n], std::isfinite(mag) ? mag : std::max(std::fabs(Erun[n]), std::fabs(E)), nmoves);
+        if (!(d <= te))
             return fail(PA_ERR_ARG,
-                        "energy audit failed for system %d: accumulated=%.17g, recomputed=%.17g, discrepancy=%g over %g "
-                        "accepted move(s) since the last audit",
-                        n + 1, Erun[n], E[n], d, nmoves);
-        std::copy(skr[n].begin(), skr[n].end(), skrun.begin() + st->sk_off[n]);
+                        "energy audit failed for system %d (Float64): accumulated=%.17g, recomputed=%.17g, "
+                        "discrepancy=%.17g exceeds tolerance %.17g over %.0f accepted move(s) since the last audit",
+                        n + 1, Erun[n], E, d, te, nmoves);
+        std::copy(skr.begin() + st->sk_off[n], skr.begin() + st->sk_off[n + 1], skrun.begin() + st->sk_off[n]);
         std::fill(skabs.begin() + st->sk_off[n], skabs.begin() + st->sk_off[n + 1], 0.0);
-        Erun[n] = E[n];
+        Erun[n] = E;
         Eabs[n] = 0.0;
         st->acc_at_audit[n] = nacc;
     }
@@ -721,6 +798,57 @@ extern "C" int pa_audit_energy(pa_state *st, double *discrepancy)
     HIP_TRY(upload(st->d_energy, Erun));
     HIP_TRY(upload(st->d_energy_abs, Eabs));
     return ok();
+}
+
+// Test-only overwrite of one element of a system's running state (the audit's fault-injection
+// tests, after test/audit_tests.jl:60-415).
+extern "C" int pa_state_set_value(pa_state *st, int32_t field, int32_t sys, int64_t index, double re, double im)
+{
+    if (!st) return fail(PA_ERR_ARG, "NULL state");
+    if (sys < 0 || sys >= st->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, st->nsys - 1);
+    const int cap = st->guest_off[sys + 1] - st->guest_off[sys];
+    switch (field) {
+    case PA_FIELD_ENERGY:
+        HIP_TRY(hipMemcpy(st->d_energy.as<double>() + sys, &re, sizeof(double), hipMemcpyHostToDevice));
+        return ok();
+    case PA_FIELD_SK: {
+        const int nk = st->sk_off[sys + 1] - st->sk_off[sys];
+        if (index < 0 || index >= nk) return fail(PA_ERR_ARG, "k-vector index %lld out of range 0..%d", (long long)index, nk - 1);
+        const double2 v = make_double2(re, im);
+        HIP_TRY(hipMemcpy(st->d_Sk.as<double2>() + st->sk_off[sys] + index, &v, sizeof(double2), hipMemcpyHostToDevice));
+        return ok();
+    }
+    case PA_FIELD_HOST_ENERGY: {
+        int occ = 0;
+        HIP_TRY(hipMemcpy(&occ, st->d_occ.as<int>() + sys, sizeof(int), hipMemcpyDeviceToHost));
+        if (index < 0 || index >= std::min(occ, cap))
+            return fail(PA_ERR_ARG, "guest index %lld out of range 0..%d", (long long)index, std::min(occ, cap) - 1);
+        HIP_TRY(hipMemcpy(st->d_host_e.as<double>() + st->guest_off[sys] + index, &re, sizeof(double),
+                          hipMemcpyHostToDevice));
+        return ok();
+    }
+    case PA_FIELD_OCCUPANCY: {
+        if (!(re >= 0.0 && re <= 2147483647.0 && re == std::floor(re)))
+            return fail(PA_ERR_ARG, "occupancy %g must be a non-negative integer", re);
+        std::vector<int> occ;
+        HIP_TRY(download(occ, st->d_occ, st->nsys));
+        // live guests: the occupancy itself, or what it was before it was set over a capacity
+        if (!st->occ_over_capacity) st->occ_live = occ;
+        const int v = (int)re, live = st->occ_live[sys];
+        if (v > live && v <= cap)
+            return fail(PA_ERR_ARG, "occupancy %d would expose unset guest slots (live guests %d)", v, live);
+        occ[sys] = v;
+        if (v <= cap) st->occ_live[sys] = v;
+        HIP_TRY(upload(st->d_occ, occ));
+        bool over = false;
+        for (int n = 0; n < st->nsys; ++n) over = over || occ[n] > st->guest_off[n + 1] - st->guest_off[n];
+        st->occ_over_capacity = over;
+        return ok();
+    }
+    default:
+        return fail(PA_ERR_ARG, "field=%d must be PA_FIELD_ENERGY, PA_FIELD_SK, PA_FIELD_HOST_ENERGY or PA_FIELD_OCCUPANCY",
+                    field);
+    }
 }
 
 // Zeroes the move counters at the start of production. The audit's "accepted moves since the
@@ -802,6 +930,9 @@ static void fluctuation_qst(const double *U, const double *N, int64_t ns, double
 
 extern "C" int pa_gcmc_run(pa_state *st, const pa_gcmc_params *p, pa_gcmc_result *out)
 {
+    if (st) {
+        if (int g = refuse_over_capacity(st)) return g;
+    }
     if (!st || !p || !out) return fail(PA_ERR_ARG, "NULL argument");
     const int nsys = st->nsys;
     if (p->n_warmup < 0) return fail(PA_ERR_ARG, "n_warmup=%lld must be >= 0", (long long)p->n_warmup);
@@ -1194,6 +1325,9 @@ int upload_const_terms(pa_state *st, std::vector<double> &ct)
 extern "C" int pa_widom_chain_energies(pa_state *st, int32_t sys, int64_t n, const double *pos, const double *quat,
                                        double *dU)
 {
+    if (st) {
+        if (int g = refuse_over_capacity(st)) return g;
+    }
     if (!st) return fail(PA_ERR_ARG, "NULL state");
     if (sys < 0 || sys >= st->nsys) return fail(PA_ERR_ARG, "system index %d out of range 0..%d", sys, st->nsys - 1);
     if (n < 0) return fail(PA_ERR_ARG, "n=%lld must be >= 0", (long long)n);
@@ -1234,6 +1368,9 @@ extern "C" int pa_widom_chain_energies(pa_state *st, int32_t sys, int64_t n, con
 
 extern "C" int pa_nvt_run(pa_state *st, const pa_guest *guest, const pa_nvt_params *p, pa_nvt_result *out)
 {
+    if (st) {
+        if (int g = refuse_over_capacity(st)) return g;
+    }
     if (!st || !guest || !p || !out) return fail(PA_ERR_ARG, "NULL argument");
     pa_batch *b = st->b;
     const int nsys = st->nsys;

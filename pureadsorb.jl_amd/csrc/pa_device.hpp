@@ -90,6 +90,33 @@ struct BinSys {
     float cls_d0, cls_inv_w;
 };
 
+// The audit's from-scratch rebuild (audit_kernels.hip). Guest slot g of system n lies in
+// [guest_off[n], guest_off[n + 1]); its live guests are the first occ[n].
+struct AuditArgs {
+    const FwDesc *fw;
+    const int *fw_of;
+    int nsys;
+    const double4 *atom64;     // framework atoms, Cartesian (Å) and charge
+    const int *atype;
+    const double4 *k64;        // kx, ky, kz, kprefactor
+    const double2 *S64;        // framework structure factor
+    const int4 *kn;
+    const double2 *pair64;     // sigma, epsilon per (site, compact type)
+    int ntypes_c;
+    double rc_lj2, rc_ew2;     // rc_ew2 = 0 for an uncharged guest
+    GuestDev guest;
+    const int *guest_off;      // nsys + 1
+    const int *sk_off;         // nsys + 1: each system's slice of the running structure factor
+    const int *occ;
+    const double4 *pos, *quat; // per slot
+    int nslots, nsk;
+    int tw;                    // per-axis power table entries per site (largest over frameworks)
+    double2 *tab;              // nslots * nsites * tw
+    double2 *sk;               // rebuilt structure factor, nsk
+    double *host_e;            // per slot: rebuilt host-guest energy (0 for an empty slot)
+    double *sums;              // per system: host-guest sum, guest-guest pairs, reciprocal sum
+};
+
 struct BinnedArgs {
     const FwDesc *fw;
     const int *fw_of;

@@ -13,7 +13,7 @@ from .isotherm import IsothermResult, combine_replicas, run_isotherm
 from .io_native import cif_types_native, read_cif_native, read_forcefield_native, read_guest_native
 from .mc import (MOVE_DELETION, MOVE_INSERTION, MOVE_REINSERTION, MOVE_ROTATION, MOVE_TRANSLATION, GCMCResult,
                  NVTResult, SystemState, audit_energy, gcmc_samples, mc_delete, mc_exchange, mc_insert, mc_moves, mc_step,
-                 nvt_last_timing, nvt_samples, run_gcmc, run_nvt, total_energy, widom_chain_energies)
+                 nvt_last_timing, nvt_samples, run_gcmc, run_nvt, set_state_value, total_energy, widom_chain_energies)
 from .structure import (Framework, cartesian, cell_matrix, min_multiplicity, natoms, perpendicular_lengths, read_cif,
                         reciprocal_basis, replicate, total_charge, volume)
 from .widom import (KB, KE, PASCAL, WidomGrad, WidomResult, default_run, hardcore_flags, last_timing, sys_of_index,
@@ -26,7 +26,7 @@ __all__ = [
     "replicate", "total_charge", "volume", "KB", "KE", "PASCAL", "WidomResult", "default_run", "last_timing",
     "sys_of_index", "system_counts", "widom", "widom_energies", "widom_grad", "widom_poses", "WidomGrad", "hardcore_flags",
     "MOVE_TRANSLATION", "MOVE_ROTATION", "MOVE_REINSERTION", "MOVE_INSERTION", "MOVE_DELETION", "SystemState",
-    "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "run_gcmc",
+    "mc_step", "mc_insert", "mc_delete", "mc_exchange", "mc_moves", "total_energy", "audit_energy", "set_state_value", "run_gcmc",
     "GCMCResult", "gcmc_samples", "NVTResult", "run_nvt", "nvt_samples", "widom_chain_energies", "nvt_last_timing",
     "Fugacity", "cubic_roots", "peng_robinson_fugacity", "IsothermResult", "combine_replicas", "run_isotherm",
     "read_cif_native", "read_forcefield_native", "read_guest_native", "cif_types_native",

@@ -21,6 +21,12 @@ PA_PREC_F64 = 0
 PA_PREC_MIXED = 1
 PA_SINGLE_PHASE = 0x100
 
+# pa_state_set_value fields (test-only overwrites of one element of a chain's running state)
+PA_FIELD_ENERGY = 1
+PA_FIELD_SK = 2
+PA_FIELD_HOST_ENERGY = 3
+PA_FIELD_OCCUPANCY = 4
+
 PA_BATCH_FULLK = 1
 PA_BATCH_HOST_ONLY = 2
 
@@ -30,7 +36,7 @@ EXPORTS = (
     "pa_batch_framework_info", "pa_widom", "pa_widom_noalloc", "pa_widom_grad", "pa_widom_energies", "pa_hardcore_flags", "pa_widom_poses", "pa_last_timing",
     "pa_last_error", "pa_device_name", "pa_synchronize",
     "pa_state_create", "pa_state_destroy", "pa_mc_step", "pa_mc_insert", "pa_mc_delete", "pa_mc_moves", "pa_state_info",
-    "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
+    "pa_state_guests", "pa_state_sk", "pa_total_energy", "pa_audit_energy", "pa_state_set_value", "pa_gcmc_run", "pa_gcmc_samples", "pa_batch_set_bin_scale", "pa_batch_bin_scale",
     "pa_batch_set_list_capacity", "pa_batch_list_stats",
     "pa_widom_chain_energies", "pa_nvt_run", "pa_nvt_samples", "pa_nvt_last_timing",
     "pa_read_cif", "pa_cif_info", "pa_cif_atoms", "pa_cif_strings", "pa_cif_types", "pa_cif_destroy",
@@ -238,6 +244,8 @@ def lib():
     L.pa_total_energy.argtypes = [vp, vp]
     L.pa_audit_energy.restype = C.c_int
     L.pa_audit_energy.argtypes = [vp, vp]
+    L.pa_state_set_value.restype = C.c_int
+    L.pa_state_set_value.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, C.c_double, C.c_double]
     L.pa_gcmc_run.restype = C.c_int
     L.pa_gcmc_run.argtypes = [vp, C.POINTER(PaGcmcParams), vp]
     L.pa_gcmc_samples.restype = C.c_int

@@ -146,10 +146,24 @@ def total_energy(state: SystemState) -> np.ndarray:
 
 
 def audit_energy(state: SystemState) -> np.ndarray:
-    """`audit_energy!` (src/audit.jl:97-165) on every system; returns recomputed - running (eV)."""
+    """`audit_energy!` (src/audit.jl:97-165) on every system, rebuilt on the device; returns
+    recomputed - running (eV). A failed stage raises ArgumentError with the reference's message."""
     d = np.zeros(state.nsys)
     _lib.check(_lib.lib().pa_audit_energy(state.handle, d.ctypes.data))
     return d
+
+
+def set_state_value(state: SystemState, field: str, n: int, value, index: int = 0):
+    """Test-only: overwrite one element of system n's running state, for fault injection.
+    field: "energy" (eV), "sk" (complex, k-vector `index`), "host_energy" (eV, live guest
+    `index`) or "occupancy" (an integer; above the capacity, the state is refused by every call
+    that reads its guest slots until it is set back)."""
+    f = {"energy": _lib.PA_FIELD_ENERGY, "sk": _lib.PA_FIELD_SK, "host_energy": _lib.PA_FIELD_HOST_ENERGY,
+         "occupancy": _lib.PA_FIELD_OCCUPANCY}.get(field)
+    if f is None:
+        raise _lib.ArgumentError(f"field={field!r}: use 'energy', 'sk', 'host_energy' or 'occupancy'")
+    v = complex(value)
+    _lib.check(_lib.lib().pa_state_set_value(state.handle, f, int(n), int(index), v.real, v.imag))
 
 
 @dataclass(frozen=True)
