@@ -5,7 +5,8 @@
 //                          minimum image per pair (src/energy.jl:42-82, src/reference.jl:39-56).
 //  (the mixed-precision pipeline is in widom_binned.hip)
 //  * boltzmann_tiles_kernel per-tile log-sum-exp Boltzmann sums {max, sum w, sum U w} in FP64,
-//                          deterministic (fixed per-thread order, fixed tree).
+//                          deterministic (fixed per-thread order, wavefront butterfly,
+//                          wavefronts merged in order).
 //  * widom_pose_kernel     device-drawn poses, for bit-exact placement checks.
 #include "pa_device.hpp"
 
@@ -128,13 +129,15 @@ __device__ inline Lse lse_merge(Lse x, Lse y)
 constexpr int TILE_THREADS = 256;
 
 // One workgroup per tile of consecutive insertions (tiles never straddle a block). out[t] =
-// {max log w, sum w e^-max, sum dU w e^-max, NaN count}.
+// {max log w, sum w e^-max, sum dU w e^-max, NaN count}. Each work-item accumulates its
+// strided insertions with a running maximum, each wavefront merges its 64 partial sums with
+// shuffles (butterfly, fixed order), and work-item 0 merges the wavefronts' results in order.
 __global__ __launch_bounds__(TILE_THREADS) void boltzmann_tiles_kernel(const double *dU, const int64_t *tile_g0,
                                                                        const int *tile_n, double kT, double4 *out)
 {
-    __shared__ double sm[TILE_THREADS], sw[TILE_THREADS], su[TILE_THREADS];
-    __shared__ int snan[TILE_THREADS];
-    const int t = blockIdx.x, tid = threadIdx.x;
+    constexpr int NW = TILE_THREADS / 64;
+    __shared__ double4 part[NW];
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t g0 = tile_g0[t];
     const int n = tile_n[t];
     Lse acc = {-INFINITY, 0.0, 0.0};
@@ -145,18 +148,23 @@ __global__ __launch_bounds__(TILE_THREADS) void boltzmann_tiles_kernel(const dou
         if (lw != lw) { ++nan; continue; }
         lse_add(acc, lw, du);
     }
-    sm[tid] = acc.m; sw[tid] = acc.w; su[tid] = acc.u; snan[tid] = nan;
-    __syncthreads();
-    for (int stride = TILE_THREADS / 2; stride >= 1; stride >>= 1) {
-        if (tid < stride) {
-            Lse x = {sm[tid], sw[tid], su[tid]}, y = {sm[tid + stride], sw[tid + stride], su[tid + stride]};
-            Lse r = lse_merge(x, y);
-            sm[tid] = r.m; sw[tid] = r.w; su[tid] = r.u;
-            snan[tid] += snan[tid + stride];
-        }
-        __syncthreads();
+    for (int off = 32; off >= 1; off >>= 1) {
+        const Lse y = {__shfl_xor(acc.m, off, 64), __shfl_xor(acc.w, off, 64), __shfl_xor(acc.u, off, 64)};
+        nan += __shfl_xor(nan, off, 64);
+        // both lanes of a pair merge (lower lane's value first), so they hold the same result
+        acc = (lane & off) ? lse_merge(y, acc) : lse_merge(acc, y);
     }
-    if (tid == 0) out[t] = make_double4(sm[0], sw[0], su[0], (double)snan[0]);
+    if (lane == 0) part[w] = make_double4(acc.m, acc.w, acc.u, (double)nan);
+    __syncthreads();
+    if (tid == 0) {
+        Lse r = {part[0].x, part[0].y, part[0].z};
+        double nn = part[0].w;
+        for (int i = 1; i < NW; ++i) {
+            r = lse_merge(r, Lse{part[i].x, part[i].y, part[i].z});
+            nn += part[i].w;
+        }
+        out[t] = make_double4(r.m, r.w, r.u, nn);
+    }
 }
 
 // ------------------------------------------------------------------------------------------

@@ -166,27 +166,28 @@ def test_nvt_cycle_length_is_one_value_for_the_batch():
 
 def test_nvt_empty_box_acceptance_and_error_shrink():
     """test/nvt_tests.jl:289-330: acceptance rates away from 0 and 1 for 50 CO2 in the empty box;
-    a 40x longer run at the same block count shrinks the Widom errors by more than 2 and the
-    energy error. The energy bar is weaker than the reference's factor 2: the total energy stays
-    correlated over tens of cycles, so the short run's 10-cycle blocks understate its error (on
-    the first device run the ratio came out at 1.95)."""
+    a 40x longer run at the same block count shrinks every block error (mu_ex, q_st, energy) by
+    more than the reference's factor 2. The runs are twice the reference's length (100 and 4,000
+    production cycles): the total energy stays correlated over tens of cycles, so the 10-cycle
+    blocks of a 50-cycle run understate its energy error, and over four independent seed sets
+    the energy ratio came out at 1.88-6.15 with the reference's lengths but 3.15-9.93 with these."""
     fw, ff, g, ew, b = _empty_box()
     st = pa.SystemState(b, g, [50], T=T, seed=3)
     r = pa.run_nvt(b, st, g, ff, T=T, n_warmup=20, n_production=200, n_widom_per_cycle=5, n_audit=200,
                    step_trans=0.3, step_rot=0.3, widom_seed=1003, nblocks=4)[0]
     print(f"acceptance {r.acceptance}", flush=True)
     assert all(0.05 < x < 0.95 for x in r.acceptance)
-    short = pa.run_nvt(b, pa.SystemState(b, g, [20], T=T, seed=11), g, ff, T=T, n_warmup=5, n_production=50,
+    short = pa.run_nvt(b, pa.SystemState(b, g, [20], T=T, seed=11), g, ff, T=T, n_warmup=5, n_production=100,
                        n_widom_per_cycle=4, n_audit=1_000_000, step_trans=0.3, step_rot=0.3, widom_seed=2001,
                        nblocks=5)[0]
-    long = pa.run_nvt(b, pa.SystemState(b, g, [20], T=T, seed=11), g, ff, T=T, n_warmup=5, n_production=2000,
+    long = pa.run_nvt(b, pa.SystemState(b, g, [20], T=T, seed=11), g, ff, T=T, n_warmup=5, n_production=4000,
                       n_widom_per_cycle=4, n_audit=1_000_000, step_trans=0.3, step_rot=0.3, widom_seed=2002,
                       nblocks=5)[0]
     for name in ("mu_ex_err", "q_st_err", "energy_err"):
         a, c = getattr(short, name), getattr(long, name)
         print(f"{name}: short {a:.3e}, long {c:.3e}", flush=True)
         assert a > 0 and np.isfinite(a) and c > 0 and np.isfinite(c)
-        assert c < (a / 2 if name != "energy_err" else a)
+        assert c < a / 2
 
 
 def test_nvt_rejects_mismatched_guest_and_bad_arguments():
