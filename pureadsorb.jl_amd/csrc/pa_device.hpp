@@ -135,8 +135,8 @@ struct BinnedArgs {
     int *order;                // chunk-relative insertion indices grouped by bin
     double *dU;                // indexed by g - out_base
     int64_t out_base;
-    const uint4 *atomfx;
-    const int *atype;
+    const uint4 *atomfx;       // the framework's atoms in Morton order: fixed point, q bits
+    const int *atype;          // their compact types, same order
     const int4 *kn;
     const float2 *kc;
     const float2 *pair32;

@@ -201,6 +201,7 @@ struct pa_batch {
     std::vector<double2> S64_h;
     std::vector<int> atype_h;
     pa::DevAlloc d_corefx, d_coreidx, d_cell_start, d_rho2, d_cand, d_ncand, d_nbr_start, d_nbr_atoms, d_passes;
+    pa::DevAlloc d_binfx, d_bintype;  // the atoms in Morton order, for the mixed path's list build
     // gradient scratch
     pa::DevAlloc d_list_g, d_msys, d_flag, d_count, d_sel_tmp, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg,
         d_loff, d_lsite, d_lw, d_blk, d_kblk;

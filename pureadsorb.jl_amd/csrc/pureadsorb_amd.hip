@@ -462,6 +462,8 @@ struct BatchTables {
     std::vector<float2> kc;
     std::vector<uint4> corefx;
     std::vector<int> coreidx, cell_start;
+    std::vector<uint4> binfx;           // the atoms in Morton order (mixed path's list build)
+    std::vector<int> bintype;
     std::vector<int> nbr_start;         // per framework cell (cell_start layout): first entry in nbr_atoms
     int64_t nbr_total = 0;              // entries of nbr_atoms (filled on the device from corefx)
     std::vector<int> cell_fw_off{0};    // prefix over frameworks of their cell counts
@@ -493,6 +495,26 @@ int add_atoms(BatchTables &T, FwDesc &D, const double *frac, const double *cart,
         uint32_t qb;
         std::memcpy(&qb, &qf, 4);
         T.atomfx.push_back(make_uint4(to_fixed(f[0]), to_fixed(f[1]), to_fixed(f[2]), qb));
+    }
+    // the mixed path's list build reads the atoms in Morton order of their fixed-point coordinates
+    // (the top 10 bits per axis, ties by index): a wavefront's 64 atoms then lie close together,
+    // so an image across a cell boundary is either needed by some of them or skipped by all
+    {
+        auto spread = [](uint32_t v) {
+            uint64_t x = 0;
+            for (int i = 0; i < 10; ++i) x |= (uint64_t)((v >> i) & 1u) << (3 * i);
+            return x;
+        };
+        std::vector<std::pair<uint64_t, int>> key((size_t)n);
+        for (int j = 0; j < n; ++j) {
+            const uint4 &x = T.atomfx[D.atom_off + j];
+            key[j] = {spread(x.x >> 22) | (spread(x.y >> 22) << 1) | (spread(x.z >> 22) << 2), j};
+        }
+        std::sort(key.begin(), key.end());
+        for (int i = 0; i < n; ++i) {
+            T.binfx.push_back(T.atomfx[D.atom_off + key[i].second]);
+            T.bintype.push_back(T.atype[D.atom_off + key[i].second]);
+        }
     }
     // hard-core cell grid: cells of at least HC_CELL Å, at most 64 per axis; atoms sorted by cell
     {
@@ -583,6 +605,8 @@ int upload_batch(pa_batch *b, const BatchTables &T)
     HIP_TRY(upload(b->d_pair64, T.pair64));
     HIP_TRY(upload(b->d_pair32, T.pair32));
     HIP_TRY(upload(b->d_corefx, T.corefx));
+    HIP_TRY(upload(b->d_binfx, T.binfx));
+    HIP_TRY(upload(b->d_bintype, T.bintype));
     HIP_TRY(upload(b->d_coreidx, T.coreidx));
     HIP_TRY(upload(b->d_cell_start, T.cell_start));
     HIP_TRY(upload(b->d_nbr_start, T.nbr_start));
@@ -1495,8 +1519,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.seed = seed;
     A.dU = dU;
     A.out_base = out_base;
-    A.atomfx = b->d_atomfx.as<uint4>();
-    A.atype = b->d_atype.as<int>();
+    A.atomfx = b->d_binfx.as<uint4>();
+    A.atype = b->d_bintype.as<int>();
     A.kn = b->d_kn.as<int4>();
     A.kc = b->d_kc.as<float2>();
     A.pair32 = b->d_pair32.as<float2>();

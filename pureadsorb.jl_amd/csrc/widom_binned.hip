@@ -425,6 +425,11 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
     int combos = 0;
     for (int c = 0; c < 8; ++c)
         if (combo_enabled(c)) combos |= 1 << c;
+    // the image test of make_entry on the fixed-point offset: |f_d| >= alt[d] exactly when
+    // |(int32)(x - c)| >= ceil(alt[d] 2^32)
+    uint32_t altfx[3];
+    for (int d = 0; d < 3; ++d)
+        altfx[d] = (uint32_t)fmin(4294967295.0, __builtin_ceil((double)B.alt[d] * 4294967296.0));
     // running position per class of this wave's entries: 16 ints of the wave's ring, which is
     // idle while the list is built
     int *tally = ring_info;
@@ -442,8 +447,19 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             const bool live = n < natoms;
             const uint4 at = live ? atoms[n] : make_uint4(0u, 0u, 0u, 0u);
             const int et = (live && write) ? types[n] : 0;
+            // |fractional offset| of the atom from the bin centre per axis, fixed point
+            auto absfx = [](uint32_t v) { return (int32_t)v < 0 ? 0u - v : v; };
+            const uint32_t ad[3] = {absfx(at.x - cx), absfx(at.y - cy), absfx(at.z - cz)};
             for (int cb = 0; cb < 8; ++cb) {
                 if (!((combos >> cb) & 1)) continue;
+                // an image across a boundary that no atom of the wavefront can take is skipped
+                // before any FP64 work (the wavefront's atoms are neighbours in Morton order)
+                if (cb != 0) {
+                    bool img = live;
+                    for (int d = 0; d < 3; ++d)
+                        if ((cb >> d) & 1) img = img && ad[d] >= altfx[d];
+                    if (__builtin_amdgcn_ballot_w64(img) == 0) continue;
+                }
                 float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
                 int cls = 0;
                 const bool inc = live && make_entry(at, cb, e, cls);
