@@ -131,7 +131,7 @@ struct BinnedArgs {
     int *key;                  // (g - g0) global bin id
     int *bin_count;            // nbins + 1
     int *bin_start;            // nbins + 1
-    int *cursor;               // nbins
+    int *rank;                 // (g - g0) position of the insertion within its bin (from the count)
     int *order;                // chunk-relative insertion indices grouped by bin
     double *dU;                // indexed by g - out_base
     int64_t out_base;

@@ -1488,8 +1488,6 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.bin_count = (int *)p;
     SCR_TRY(b, scr.take(b->d_bin_start, sizeof(int) * (nbins + 1), &p));
     A.bin_start = (int *)p;
-    SCR_TRY(b, scr.take(b->d_cursor, sizeof(int) * nbins, &p));
-    A.cursor = (int *)p;
     size_t scan_bytes = 0;
     HIP_TRY(binned_scan_bytes(nbins, &scan_bytes));
     void *scan_tmp = nullptr;
@@ -1501,6 +1499,8 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.key = (int *)p;
     SCR_TRY(b, scr.take(b->d_order, sizeof(int) * chunk_n, &p));
     A.order = (int *)p;
+    SCR_TRY(b, scr.take(b->d_rank, sizeof(int) * chunk_n, &p));
+    A.rank = (int *)p;
     SCR_TRY(b, scr.take(b->d_cand, sizeof(int) * chunk_n, &p));
     A.cand = (int *)p;
     SCR_TRY(b, scr.take(b->d_ncand, sizeof(int), &p));
@@ -1642,20 +1642,29 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     // a call into a fixed scratch region never tunes: it uses the batch's bin scale as it stands
     if ((precision & ~PA_SINGLE_PHASE) == PA_PREC_MIXED && b->bin_scale == 0.0 && G >= ((int64_t)1 << 18) &&
         !scr.base) {
-        // autotune the bin edge multiplier once per batch: time the energy pipeline on the first
-        // (up to) 2^22 insertions, one default chunk, for each candidate and keep the fastest.
-        // Insertions run system-major, so the sample holds whole systems at the call's own
-        // insertion density
+        // autotune the bin edge multiplier once per batch: time the energy pipeline for each
+        // candidate and keep the fastest. Candidates differ by a few percent, and which one wins
+        // depends on the batch's mix of framework sizes, so the sample is the whole call up to
+        // 2^25 insertions; a longer call is sampled in four windows of 2^22 spread over it
+        // (insertions run system-major: each window holds whole systems at the call's density)
         const double cand[] = {0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5};
-        const int64_t ge = std::min<int64_t>(G, (int64_t)1 << 22);
+        const int64_t W = (int64_t)1 << 22;
+        const int nwin = G > ((int64_t)1 << 25) ? 4 : 1;
+        int64_t wb[4], we[4];
+        for (int r = 0; r < nwin; ++r) {
+            wb[r] = nwin == 1 ? 0 : (G - W) * r / (nwin - 1);
+            we[r] = nwin == 1 ? G : wb[r] + W;
+        }
         double best = 1.0, best_ms = INFINITY;
         for (int rep = 0; rep < 2; ++rep)
             for (double kappa : cand) {
                 b->bin_scale = kappa;
-                int64_t nl = 0;
                 HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-                st = run_energies(b, seg, 0, ge, 0, seed, precision, chunk, &nl, kT_phase0, dU);
-                if (st) { b->bin_scale = 0.0; return st; }
+                for (int r = 0; r < nwin; ++r) {
+                    int64_t nl = 0;
+                    st = run_energies(b, seg, wb[r], we[r], 0, seed, precision, chunk, &nl, kT_phase0, dU);
+                    if (st) { b->bin_scale = 0.0; return st; }
+                }
                 HIP_TRY(hipEventRecord(b->ev[1], b->stream));
                 HIP_TRY(hipEventSynchronize(b->ev[1]));
                 float ms = 0.0f;

@@ -5,7 +5,7 @@
 //                           as 32-bit fixed-point fractional coordinates, and counts insertions
 //                           per spatial bin of the system's cell;
 //  2. exclusive scan of the bin counts (hipCUB);
-//  3. bin_scatter_kernel    groups the chunk's insertions by bin;
+//  3. bin_scatter_kernel    groups the chunk's insertions by bin (positions from step 1's counts);
 //  4. widom_binned_kernel   one workgroup per bin: builds the bin's neighbour list in LDS (every
 //                           atom image within rc + r_guest + r_bin of the bin centre, FP32
 //                           coordinates relative to that centre, computed in FP64, in order of
@@ -44,6 +44,20 @@ __device__ inline int find_sys64(const int64_t *seg, int nsys, int64_t g)
         if (seg[mid] <= g) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+// The system of insertion g for a wavefront of consecutive insertions: one binary search for the
+// first active lane's g (uniform, so on scalar loads), then each lane steps over the system
+// boundaries between that g and its own (none unless a system ends inside the wavefront).
+__device__ inline int find_sys64_wave(const int64_t *seg, int nsys, int64_t g)
+{
+    const uint64_t gu = (uint64_t)g;
+    const int64_t g0 = (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)gu) |
+                                 ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(gu >> 32)) << 32));
+    int s = find_sys64(seg, nsys, g0);
+    while (s + 1 < nsys && seg[s + 1] <= g) ++s;
+    while (s > 0 && seg[s] > g) --s;
+    return s;
 }
 
 __device__ inline int find_sys32(const int *base, int nsys, int b)
@@ -138,7 +152,7 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
 #pragma clang fp contract(off)
     const int64_t g = a.g0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= a.g1) return;
-    const int s = find_sys64(a.seg, a.nsys, g);
+    const int s = find_sys64_wave(a.seg, a.nsys, g);
     const int64_t j = g - a.seg[s];
     const FwDesc &F = a.fw[a.fw_of[s]];
     double f[3], q[4];
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(256) void bin_pose_kernel(BinnedArgs a)
     }
     a.key[g - a.g0] = key;
     if (!pending) {
-        atomicAdd(&a.bin_count[key], 1);
+        a.rank[g - a.g0] = atomicAdd(&a.bin_count[key], 1);
         return;
     }
     // wavefront-aggregated append to the candidate list
@@ -250,28 +264,29 @@ __global__ __launch_bounds__(256) void bin_reject_kernel(BinnedArgs a)
             a.key[rel] = -1;
             a.dU[g - a.out_base] = INFINITY;
         } else {
-            atomicAdd(&a.bin_count[a.key[rel]], 1);
+            a.rank[rel] = atomicAdd(&a.bin_count[a.key[rel]], 1);
         }
     }
 }
 
-// Clears the bin counts (nbins + 1) and scatter cursors (nbins) of one chunk in one launch.
+// Clears the bin counts (nbins + 1) and the candidate counter of one chunk in one launch.
 __global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i <= a.nbins) a.bin_count[i] = 0;
-    if (i < a.nbins) a.cursor[i] = 0;
     if (i == 0) *a.ncand = 0;
     if (i == 0 && a.zero_passes) *a.extra_passes = 0;
 }
 
+// Each insertion's position within its bin is the count its atomic increment returned (pose or
+// reject kernel); the order within a bin does not change any insertion's energy.
 __global__ __launch_bounds__(256) void bin_scatter_kernel(BinnedArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.g1 - a.g0) return;
     const int k = a.key[i];
     if (k < 0) return;  // rejected in phase 0
-    a.order[a.bin_start[k] + atomicAdd(&a.cursor[k], 1)] = (int)i;
+    a.order[a.bin_start[k] + a.rank[i]] = (int)i;
 }
 
 // Cutoff classes of the binned energy kernel. The ring only holds pairs inside the larger of the
@@ -753,8 +768,7 @@ int binned_listcap_max(int ns)
     return cap;
 }
 
-// Steps 1-3 of the pipeline (pose + bin counts, scan, scatter), after clearing bin_count and
-// cursor.
+// Steps 1-3 of the pipeline (pose + bin counts, scan, scatter), after clearing bin_count.
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st)
 {
     const int64_t n = a.g1 - a.g0;
