@@ -232,6 +232,35 @@ def test_rubtak_triclinic_energies(rubtak, trappe, co2):
     assert np.max(np.abs(dm[w] - ref[w])) < 2e-4
 
 
+def test_mirror_fold_matches_the_generic_reciprocal_sum(rubtak, trappe, co2):
+    """CO2 listed C, O, O (sites 1 and 2 mirror images through site 0, equal charges) takes the
+    mixed path's folded reciprocal sum e^{i t0}(q0 + 2 q1 cos p); the same molecule listed O, C, O
+    takes the generic per-site sum. Same poses, so the energies agree to FP32 rounding, and both
+    agree with the oracle; Widom results within the precision gate of each other."""
+    sc = pa.replicate(rubtak, (3, 3, 3))
+    ew = pa.EwaldParams(12.0, 1e-6)
+    perm = [1, 0, 2]
+    ocq = pa.Guest(np.asarray(co2.sites)[perm], np.asarray(co2.types)[perm], np.asarray(co2.charges)[perm],
+                   co2.tc, co2.pc, co2.omega)
+    bf = pa.FrameworkBatch([sc], trappe, co2, ew)
+    bg = pa.FrameworkBatch([sc], trappe, ocq, ew)
+    bf.bin_scale = bg.bin_scale = 1.0
+    n = 4096
+    ef = pa.widom_energies(bf, 0, 9, 0, n, precision="mixed")
+    eg = pa.widom_energies(bg, 0, 9, 0, n, precision="mixed")
+    ref = oracle_for(sc, trappe, co2, ew).energies(9, 0, 0, n, nthreads=oracle_threads())
+    _check_relative(ef, ref, "folded CO2")
+    _check_relative(eg, ref, "CO2 listed O, C, O")
+    w = ref < 30 * pa.KB * 298.15
+    assert w.sum() > 10
+    d = np.max(np.abs(ef[w] - eg[w]))
+    print(f"folded vs generic reciprocal sum: max |d dU| {d:.2e} eV over {w.sum()} accessible poses", flush=True)
+    assert d < 2e-6
+    rf = pa.widom(bf, co2, T=298.15, ninsert=200_000, seed=9, nblocks=5)[0]
+    rg = pa.widom(bg, ocq, T=298.15, ninsert=200_000, seed=9, nblocks=5)[0]
+    assert abs(rf.ln_K_H - rg.ln_K_H) <= 1e-4 and abs(rf.q_st - rg.q_st) <= 1e-4
+
+
 def test_general_cell_orientation(c1p):
     """A cell that is not upper triangular (a rotated copy) runs the general minimum-image
     variant; energies are rotation invariant, so both agree with the oracle."""
