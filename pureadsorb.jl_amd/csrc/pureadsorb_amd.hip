@@ -1344,7 +1344,13 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     pa::CallScratch &scr = b->scr;
     int64_t *d_seg = nullptr;
     SCR_TRY(b, scr.put(b->d_seg, seg, &d_seg));
-    const int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
+    // the default cuts the range into equal chunks of at most 2^22 (a short last chunk would run
+    // the energy kernel on a fraction of the device); an explicit chunk is taken as given
+    int64_t CH = chunk > 0 ? std::max<int64_t>(chunk, 64) : ((int64_t)1 << 22);
+    if (chunk <= 0 && ge - gb > CH) {
+        const int64_t n = (ge - gb + CH - 1) / CH;
+        CH = ((ge - gb + n - 1) / n + 255) / 256 * 256;
+    }
     const int64_t nchunks = (ge - gb + CH - 1) / CH;
     // event pairs around the energy launches; a call into a fixed scratch region creates none
     // and records none
