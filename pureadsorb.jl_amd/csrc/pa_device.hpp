@@ -84,7 +84,8 @@ struct BinSys {
                     // across the cell boundary on that axis (alt >= 0.5: never)
     int reach[3];   // hard-core stencil half-width in grid cells per axis (reach[0] < 0: no check)
     uint32_t hc_rf[3];  // largest rejection radius over the perpendicular width, 32-bit fixed point
-                        // and rounded up: a site's hard-core partners lie within +-hc_rf[d]
+                        // and rounded up: a site's hard-core partners lie within +-hc_rf[d];
+                        // all ones: that range (plus one cell) reaches around the axis
     // radial classes of the neighbour list: class 0 holds the atoms closer than cls_d0 to the bin
     // centre, classes 1..BIN_NCLS-1 split [cls_d0, sqrt(rlist2)] into shells of width 1/cls_inv_w
     float cls_d0, cls_inv_w;

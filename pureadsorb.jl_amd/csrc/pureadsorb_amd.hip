@@ -1473,6 +1473,10 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
                     bins[s].reach[d] = (int)std::ceil(std::sqrt(rmax2) * D.cg_nc[d] / h.perp[d]);
                     bins[s].hc_rf[d] = (uint32_t)std::min(4294967295.0,
                                                           std::ceil(std::sqrt(rmax2) / h.perp[d] * 4294967296.0) + 2.0);
+                    // +-hc_rf plus one cell reaching around the axis: every cell, marked by the
+                    // all-ones value so that the kernels need no division
+                    const uint64_t rf = bins[s].hc_rf[d], nc = (uint64_t)D.cg_nc[d];
+                    if (2 * rf + ((uint64_t)1 << 32) / nc >= ((uint64_t)1 << 32)) bins[s].hc_rf[d] = 0xFFFFFFFFu;
                 }
         }
     }

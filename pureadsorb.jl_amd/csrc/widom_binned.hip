@@ -78,11 +78,12 @@ __device__ inline void wave_lds_fence()
 }
 
 // Grid cells, along one axis with nc cells, that hold the coordinates within +-rf (32-bit fixed
-// point) of x: `cnt` cells from `start` upwards, wrapping (the whole axis when the interval
-// reaches around it). Atoms are binned as (x nc) >> 32.
+// point) of x: `cnt` cells from `start` upwards, wrapping. Atoms are binned as (x nc) >> 32. An
+// interval that reaches around the axis (2 rf + 2^32 / nc >= 2^32, decided once per system on the
+// host) comes as rf = all ones and covers every cell.
 __device__ inline void cell_range(uint32_t x, uint32_t rf, int nc, int &start, int &cnt)
 {
-    if ((uint64_t)2 * rf + ((uint64_t)1 << 32) / (uint64_t)nc >= ((uint64_t)1 << 32)) {
+    if (rf == 0xFFFFFFFFu) {
         start = 0;
         cnt = nc;
         return;
