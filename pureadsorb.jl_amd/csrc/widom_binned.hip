@@ -299,6 +299,11 @@ constexpr int CUT_NONE = 0, CUT_EW = 1, CUT_LJ = 2;
 // LDS (address space 3) views: a 32-bit LDS address held in a register is dereferenced
 // directly, with no base added.
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// Fourth word of a raw (untyped) buffer resource on gfx9-family GPUs: 32-bit data format, no
+// swizzle; offsets are bounds-checked against the record count.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
 typedef __attribute__((address_space(3))) const f32x2 lds_f32x2;
 typedef __attribute__((address_space(3))) unsigned char lds_byte;
 
@@ -651,8 +656,23 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
             }
             wave_lds_fence();
             if (CHARGED && pass == 0) {
-                const int4 *kn = a.kn + F.k_off;
-                const float2 *kc = a.kc + F.k_off;
+                // the reciprocal cross term: full rounds of 64 k-vectors from a uniform base (one
+                // lane per k-vector, no bounds test) and one partial round; each lane sums its
+                // ~nk/64 terms in FP32 before the FP64 accumulator takes them
+                const int nk = F.nk, nfull = nk & ~63;
+                // buffer loads: the round's offset in a scalar register, the lane's in a fixed
+                // vector register (no per-lane pointer arithmetic)
+                const __amdgpu_buffer_rsrc_t rkn =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(a.kn + F.k_off), (short)0, nk * 16, BUF_RSRC_WORD3);
+                const __amdgpu_buffer_rsrc_t rkc =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(a.kc + F.k_off), (short)0, nk * 8, BUF_RSRC_WORD3);
+                auto kload = [&](int kb, int4 &nn, float2 &c) {
+                    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rkn, lane * 16, kb * 16, 0);
+                    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rkc, lane * 8, kb * 8, 0);
+                    nn = make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+                    c = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+                };
+                float kacc = 0.0f;
                 bool folded = false;
                 if constexpr (NS == 3) {
                     if (a.guest.mirror) {
@@ -663,28 +683,32 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                         const uint32_t d0 = sfx[1][0] - sfx[0][0], d1 = sfx[1][1] - sfx[0][1],
                                        d2 = sfx[1][2] - sfx[0][2];
                         const float q0 = qke_lds[0], q1x2 = 2.0f * qke_lds[1];
-                        for (int kb = 0; kb < F.nk; kb += 64) {
-                            const int k = kb + lane;
-                            if (k < F.nk) {
-                                const int4 nn = kn[k];
-                                const float2 c = kc[k];
-                                const uint32_t ph = (uint32_t)nn.x * sfx[0][0] + (uint32_t)nn.y * sfx[0][1] +
-                                                    (uint32_t)nn.z * sfx[0][2];
-                                const uint32_t pd = (uint32_t)nn.x * d0 + (uint32_t)nn.y * d1 + (uint32_t)nn.z * d2;
-                                const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
-                                const float rvd = (float)(int32_t)pd * 2.3283064365386963e-10f;
-                                const float w = q0 + q1x2 * __builtin_amdgcn_cosf(rvd);
-                                acc += (double)(w * (c.x * __builtin_amdgcn_cosf(rev) + c.y * __builtin_amdgcn_sinf(rev)));
-                            }
+                        auto term = [&](const int4 nn, const float2 c) {
+                            const uint32_t ph =
+                                (uint32_t)nn.x * sfx[0][0] + (uint32_t)nn.y * sfx[0][1] + (uint32_t)nn.z * sfx[0][2];
+                            const uint32_t pd = (uint32_t)nn.x * d0 + (uint32_t)nn.y * d1 + (uint32_t)nn.z * d2;
+                            const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
+                            const float rvd = (float)(int32_t)pd * 2.3283064365386963e-10f;
+                            const float w = q0 + q1x2 * __builtin_amdgcn_cosf(rvd);
+                            return w * (c.x * __builtin_amdgcn_cosf(rev) + c.y * __builtin_amdgcn_sinf(rev));
+                        };
+                        for (int kb = 0; kb < nfull; kb += 64) {
+                            int4 nn;
+                            float2 c;
+                            kload(kb, nn, c);
+                            kacc += term(nn, c);
+                        }
+                        if (lane < nk - nfull) {
+                            int4 nn;
+                            float2 c;
+                            kload(nfull, nn, c);
+                            kacc += term(nn, c);
                         }
                         folded = true;
                     }
                 }
-                for (int kb = 0; kb < (folded ? 0 : F.nk); kb += 64) {
-                    const int k = kb + lane;
-                    if (k < F.nk) {
-                        const int4 nn = kn[k];
-                        const float2 c = kc[k];
+                if (!folded) {
+                    auto term = [&](const int4 nn, const float2 c) {
                         float re = 0.0f, im = 0.0f;
 #pragma unroll
                         for (int t = 0; t < NS; ++t) {
@@ -695,9 +719,22 @@ __global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
                             re += qs * __builtin_amdgcn_cosf(rev);
                             im += qs * __builtin_amdgcn_sinf(rev);
                         }
-                        acc += (double)(c.x * re + c.y * im);
+                        return c.x * re + c.y * im;
+                    };
+                    for (int kb = 0; kb < nfull; kb += 64) {
+                        int4 nn;
+                        float2 c;
+                        kload(kb, nn, c);
+                        kacc += term(nn, c);
+                    }
+                    if (lane < nk - nfull) {
+                        int4 nn;
+                        float2 c;
+                        kload(nfull, nn, c);
+                        kacc += term(nn, c);
                     }
                 }
+                acc += (double)kacc;
             }
             const double Etot = wave_sum64(acc);
             if (lane == 0) {
