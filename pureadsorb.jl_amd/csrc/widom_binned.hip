@@ -361,7 +361,7 @@ __host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
 }
 
 template <int NS, bool CHARGED, int CUT>
-__global__ __launch_bounds__(BIN_WG) void widom_binned_kernel(BinnedArgs a)
+__global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = blockIdx.x;
