@@ -50,6 +50,23 @@ struct DevAlloc {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// Page-locked host memory of at least n bytes, kept and grown on demand (copies into it need no
+// staging through the runtime's bounce buffers).
+struct PinnedAlloc {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~PinnedAlloc() { if (p) (void)hipHostFree(p); }
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipHostFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipHostMalloc(&p, n ? n : 16, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
 struct FwHost {
     int natoms, nk, nk_full;
     double volume, alpha, const_off, halfrange;
@@ -203,6 +220,7 @@ struct pa_batch {
     pa::DevAlloc d_corefx, d_coreidx, d_cell_start, d_rho2, d_cand, d_ncand, d_nbr_start, d_nbr_atoms, d_passes;
     pa::DevAlloc d_binfx, d_bintype;  // the atoms in Morton order, for the mixed path's list build
     // gradient scratch
+    pa::PinnedAlloc h_acc_q;  // gradient path: the per-atom charge accumulators read back
     pa::DevAlloc d_list_g, d_msys, d_flag, d_count, d_sel_tmp, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg,
         d_loff, d_lsite, d_lw, d_blk, d_kblk;
     hipStream_t stream = nullptr;

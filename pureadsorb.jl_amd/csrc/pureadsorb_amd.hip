@@ -2025,11 +2025,14 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     HIP_TRY(launch_grad(A, b->upper_all, (int)blk.size(), (int)kblk.size(), b->stream));
     if (charged && want_q) HIP_TRY(launch_grad_recip(A, b->d_k64.as<double4>(), qoff[nsys], b->stream));
     HIP_TRY(hipEventRecord(g1, b->stream));
-    std::vector<double> acc_lj(nlj), acc_q(2 * qoff[nsys]);
+    std::vector<double> acc_lj(nlj);
     HIP_TRY(hipMemcpyAsync(acc_lj.data(), b->d_acc_lj.p, sizeof(double) * nlj, hipMemcpyDeviceToHost, b->stream));
-    if (!acc_q.empty())
-        HIP_TRY(hipMemcpyAsync(acc_q.data(), b->d_acc_q.p, sizeof(double) * acc_q.size(), hipMemcpyDeviceToHost,
-                               b->stream));
+    // the per-atom charge accumulators (2 per atom, MBs for large batches) into page-locked memory
+    const size_t nacc_q = (size_t)2 * qoff[nsys];
+    HIP_TRY(b->h_acc_q.ensure(sizeof(double) * std::max<size_t>(nacc_q, 1)));
+    double *acc_q = b->h_acc_q.as<double>();
+    if (nacc_q)
+        HIP_TRY(hipMemcpyAsync(acc_q, b->d_acc_q.p, sizeof(double) * nacc_q, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     float gms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&gms, g0, g1));
@@ -2105,7 +2108,7 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
         }
         if (want_q) {
             const int n = b->info[u].natoms;
-            double *gq = acc_q.data() + 2 * qoff[s];
+            double *gq = acc_q + 2 * qoff[s];
             // the real-space part of atom j moves with its replication images: average over them
             const int nimg = gh.rep[0] * gh.rep[1] * gh.rep[2];
             if (nimg > 1 && n > 0) {
