@@ -33,7 +33,7 @@ int binned_nclasses();
 hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, int nsys, int64_t *loff, hipStream_t st);
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
                        unsigned char *flag, int64_t *list, int64_t *count, void *tmp, size_t *tmp_bytes, hipStream_t st);
-hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st);
+hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, const int4 *rblk, int nrblk, hipStream_t st);
 hipError_t launch_sk(const double4 *atoms, int natoms, const int4 *kreq, int nreq, double2 *out, hipStream_t st);
 hipError_t launch_nbr_fill(const FwDesc *fw, int nfw, const int *cell_fw_off, int ncells, const int *cell_start,
                            const int *nbr_start, const uint4 *corefx, uint4 *nbr_atoms, hipStream_t st);
@@ -2023,7 +2023,17 @@ extern "C" int pa_widom_grad(pa_batch *b, const pa_guest *guest, double T, int64
     // the Coulomb and k-space sums feed only the charge derivatives
     if (!want_q) A.guest.charged = 0;
     HIP_TRY(launch_grad(A, b->upper_all, (int)blk.size(), (int)kblk.size(), b->stream));
-    if (charged && want_q) HIP_TRY(launch_grad_recip(A, b->d_k64.as<double4>(), qoff[nsys], b->stream));
+    if (charged && want_q) {
+        // the per-atom reciprocal term in workgroups of (system, 256 atoms, 256 k-vectors)
+        std::vector<int4> rblk;
+        for (int s = 0; s < nsys; ++s) {
+            const FwDesc &D = b->desc[b->fw_of[s]];
+            for (int a0 = 0; a0 < D.natoms; a0 += 256)
+                for (int k0 = 0; k0 < D.nk; k0 += 256) rblk.push_back(make_int4(s, a0, k0, 0));
+        }
+        HIP_TRY(upload(b->d_rblk, rblk));
+        HIP_TRY(launch_grad_recip(A, b->d_k64.as<double4>(), b->d_rblk.as<int4>(), (int)rblk.size(), b->stream));
+    }
     HIP_TRY(hipEventRecord(g1, b->stream));
     std::vector<double> acc_lj(nlj);
     HIP_TRY(hipMemcpyAsync(acc_lj.data(), b->d_acc_lj.p, sizeof(double) * nlj, hipMemcpyDeviceToHost, b->stream));

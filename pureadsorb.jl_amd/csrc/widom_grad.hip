@@ -227,40 +227,49 @@ __global__ __launch_bounds__(256) void grad_flag_kernel(const double *dU, const 
     flag[g] = x > -40.0 ? 1 : 0;
 }
 
-// Reciprocal part of every framework atom's charge derivative, one work-item per (system, atom):
+// Reciprocal part of every framework atom's charge derivative:
 // KE sum_k kprefactor 2 (cos(k.r_j) A_re(k) + sin(k.r_j) A_im(k)), with k.r_j = 2 pi n.f_j taken
-// from the fixed-point coordinates (an exact wrapping integer phase), FP64 sin/cos of pi x.
-__global__ __launch_bounds__(256) void grad_recip_kernel(GradArgs a, const double4 *k64, int64_t natoms_total)
+// from the fixed-point coordinates (an exact wrapping integer phase), FP64 sin/cos of pi x. One
+// workgroup per (system, 256 atoms, 256 k-vectors): the chunk's k-vectors, prefactors and
+// accumulated A(k) are staged in LDS once and read by every lane at the same address, and each
+// chunk's partial sums join the atom's accumulator atomically.
+constexpr int RECIP_KCH = 256;
+__global__ __launch_bounds__(256) void grad_recip_kernel(GradArgs a, const double4 *k64, const int4 *rblk)
 {
 #pragma clang fp contract(off)
-    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (x >= natoms_total) return;
-    int lo = 0, hi = a.nsys - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.qoff[mid] <= x) lo = mid; else hi = mid - 1;
-    }
-    const int s = lo;
-    const int j = (int)(x - a.qoff[s]);
+    __shared__ int4 skn[RECIP_KCH];
+    __shared__ double sf[RECIP_KCH];
+    __shared__ double4 sacc[RECIP_KCH];
+    const int4 w = rblk[blockIdx.x];
+    const int s = w.x, a0 = w.y, k0 = w.z;
     const FwDesc &F = a.fw[a.fw_of[s]];
+    const int nkc = min(RECIP_KCH, F.nk - k0);
+    const int t = threadIdx.x;
+    if (t < nkc) {
+        skn[t] = a.kn[F.k_off + k0 + t];
+        sf[t] = 2.0 * k64[F.k_off + k0 + t].w;
+        const double *acck = a.acc_k + 4 * (a.koff[s] + k0 + t);
+        sacc[t] = make_double4(acck[0], acck[1], acck[2], acck[3]);
+    }
+    __syncthreads();
+    const int j = a0 + t;
+    if (j >= F.natoms) return;
     const uint4 at = a.atomfx[F.atom_off + j];
-    const int4 *kn = a.kn + F.k_off;
-    const double4 *kp = k64 + F.k_off;
-    const double *acck = a.acc_k + 4 * a.koff[s];
     double gw = 0.0, gu = 0.0;
-    for (int k = 0; k < F.nk; ++k) {
-        const int4 nn = kn[k];
+    for (int k = 0; k < nkc; ++k) {
+        const int4 nn = skn[k];
         const uint32_t ph = (uint32_t)nn.x * at.x + (uint32_t)nn.y * at.y + (uint32_t)nn.z * at.z;
         const double rev = (double)(int32_t)ph * 2.3283064365386963e-10;
         double sn, cs;
         sincospi(2.0 * rev, &sn, &cs);
-        const double f = 2.0 * kp[k].w;
-        gw += f * (cs * acck[4 * k] + sn * acck[4 * k + 1]);
-        gu += f * (cs * acck[4 * k + 2] + sn * acck[4 * k + 3]);
+        const double f = sf[k];
+        const double4 c = sacc[k];
+        gw += f * (cs * c.x + sn * c.y);
+        gu += f * (cs * c.z + sn * c.w);
     }
     double *accq = a.acc_q + 2 * a.qoff[s];
-    accq[2 * j] += KE * gw;
-    accq[2 * j + 1] += KE * gu;
+    unsafeAtomicAdd(accq + 2 * j, KE * gw);
+    unsafeAtomicAdd(accq + 2 * j + 1, KE * gu);
 }
 
 hipError_t grad_select(const double *dU, const int64_t *seg, int nsys, const double *msys, double beta, int64_t G,
@@ -304,11 +313,12 @@ hipError_t grad_offsets(const int64_t *list, int64_t nlist, const int64_t *seg, 
     return hipGetLastError();
 }
 
-hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, int64_t natoms_total, hipStream_t st)
+// rblk: (system, first atom, first k-vector, 0) of each workgroup; atoms and k-vectors in
+// steps of 256 within each system's ranges.
+hipError_t launch_grad_recip(const GradArgs &a, const double4 *k64, const int4 *rblk, int nrblk, hipStream_t st)
 {
-    if (natoms_total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_recip_kernel, dim3((unsigned)((natoms_total + 255) / 256)), dim3(256), 0, st, a, k64,
-                       natoms_total);
+    if (nrblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(grad_recip_kernel, dim3((unsigned)nrblk), dim3(256), 0, st, a, k64, rblk);
     return hipGetLastError();
 }
 
