@@ -191,7 +191,33 @@ __global__ __launch_bounds__(256) void grad_kspace_kernel(GradArgs a)
         for (int i = tid; i < nt; i += 256) t_w[i] = a.lw[base + i];
         __syncthreads();
         if (!valid) continue;
-        for (int i = 0; i < nt; ++i) {
+        auto add = [&](int i, float re, float im) {
+            const double2 w = t_w[i];
+            ar += w.x * (double)re;
+            ai += w.x * (double)im;
+            br += w.y * (double)re;
+            bi += w.y * (double)im;
+        };
+        bool folded = false;
+        if constexpr (NS == 3) {
+            if (a.guest.mirror) {
+                // sites 1 and 2 mirror each other through site 0 with equal charges:
+                // S_g = e^{i t0} (q0 + 2 q1 cos p), p the phase of site 1 relative to site 0
+                const float q0 = qs[0], q1x2 = 2.0f * qs[1];
+                for (int i = 0; i < nt; ++i) {
+                    const uint32_t *s0 = t_site + i * 9;
+                    const uint32_t ph = (uint32_t)nn.x * s0[0] + (uint32_t)nn.y * s0[1] + (uint32_t)nn.z * s0[2];
+                    const uint32_t pd = (uint32_t)nn.x * (s0[3] - s0[0]) + (uint32_t)nn.y * (s0[4] - s0[1]) +
+                                        (uint32_t)nn.z * (s0[5] - s0[2]);
+                    const float rev = (float)(int32_t)ph * 2.3283064365386963e-10f;
+                    const float rvd = (float)(int32_t)pd * 2.3283064365386963e-10f;
+                    const float wq = q0 + q1x2 * __builtin_amdgcn_cosf(rvd);
+                    add(i, wq * __builtin_amdgcn_cosf(rev), wq * __builtin_amdgcn_sinf(rev));
+                }
+                folded = true;
+            }
+        }
+        for (int i = 0; i < (folded ? 0 : nt); ++i) {
             float re = 0.0f, im = 0.0f;
 #pragma unroll
             for (int t = 0; t < NS; ++t) {
@@ -201,11 +227,7 @@ __global__ __launch_bounds__(256) void grad_kspace_kernel(GradArgs a)
                 re += qs[t] * __builtin_amdgcn_cosf(rev);
                 im += qs[t] * __builtin_amdgcn_sinf(rev);
             }
-            const double2 w = t_w[i];
-            ar += w.x * (double)re;
-            ai += w.x * (double)im;
-            br += w.y * (double)re;
-            bi += w.y * (double)im;
+            add(i, re, im);
         }
     }
     if (!valid) return;
