@@ -221,6 +221,7 @@ struct pa_batch {
     pa::DevAlloc d_binfx, d_bintype;  // the atoms in Morton order, for the mixed path's list build
     // gradient scratch
     pa::PinnedAlloc h_acc_q;  // gradient path: the per-atom charge accumulators read back
+    pa::PinnedAlloc h_tiles;  // Widom calls: the Boltzmann tile sums read back
     pa::DevAlloc d_rblk;      // gradient path: workgroups of the per-atom reciprocal term
     pa::DevAlloc d_list_g, d_msys, d_flag, d_count, d_sel_tmp, d_acc_lj, d_acc_q, d_acc_k, d_qoff, d_koff, d_pairg,
         d_loff, d_lsite, d_lw, d_blk, d_kblk;

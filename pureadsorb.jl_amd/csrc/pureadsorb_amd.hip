@@ -1694,8 +1694,18 @@ static int widom_core(pa_batch *b, const pa_guest *guest, double T, int64_t nins
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(launch_tiles(dU, d_tile_g0, d_tile_n, ntiles, kT, d_tiles, b->stream));
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
-    std::vector<double4> tiles(ntiles);
-    HIP_TRY(hipMemcpyAsync(tiles.data(), d_tiles, sizeof(double4) * ntiles, hipMemcpyDeviceToHost, b->stream));
+    // the tile sums back into page-locked memory (a call into a fixed scratch region allocates
+    // nothing, so it reads them into ordinary host memory)
+    std::vector<double4> tiles_pageable;
+    double4 *tiles = nullptr;
+    if (scr.base) {
+        tiles_pageable.resize(ntiles);
+        tiles = tiles_pageable.data();
+    } else {
+        HIP_TRY(b->h_tiles.ensure(sizeof(double4) * std::max(ntiles, 1)));
+        tiles = b->h_tiles.as<double4>();
+    }
+    HIP_TRY(hipMemcpyAsync(tiles, d_tiles, sizeof(double4) * ntiles, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     float e01 = 0, e12 = 0, e02 = 0;
     HIP_TRY(hipEventElapsedTime(&e01, b->ev[0], b->ev[1]));
