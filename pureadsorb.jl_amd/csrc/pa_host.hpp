@@ -201,7 +201,7 @@ struct pa_batch {
     // per-call scratch
     pa::DevAlloc d_seg, d_dU, d_tile_g0, d_tile_n, d_tiles, d_pos, d_quat;
     // mixed-path binning scratch
-    pa::DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_rank, d_order, d_scan_tmp;
+    pa::DevAlloc d_bins, d_bin_base, d_sitefx, d_key, d_bin_count, d_bin_start, d_rank, d_order, d_scan_tmp, d_bin_perm;
     // hard-core rejection: per unique framework the bound B_s (src/reject.jl:101-131) and
     // kmin[site][compact type] (src/reject.jl:141-153); device cell grids
     std::vector<double> hc_bs, hc_kmin;

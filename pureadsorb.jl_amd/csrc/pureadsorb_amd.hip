@@ -24,6 +24,7 @@ hipError_t launch_tiles(const double *dU, const int64_t *tile_g0, const int *til
 hipError_t launch_poses(const FwDesc *fw, const int *fw_of, int sys, uint64_t seed, int64_t j0, int64_t n, double *pos,
                         double *quat, hipStream_t st);
 hipError_t binned_scan_bytes(int nbins, size_t *bytes);
+hipError_t binned_sort_bytes(int nbins, int bits, size_t *bytes);
 hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_binned_energy(const BinnedArgs &a, hipStream_t st);
 hipError_t launch_grad(const GradArgs &a, bool upper, int nblk, int nkblk, hipStream_t st);
@@ -1500,11 +1501,24 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     A.bin_count = (int *)p;
     SCR_TRY(b, scr.take(b->d_bin_start, sizeof(int) * (nbins + 1), &p));
     A.bin_start = (int *)p;
+    const int64_t chunk_n = std::min<int64_t>(CH, ge - gb);
     size_t scan_bytes = 0;
     HIP_TRY(binned_scan_bytes(nbins, &scan_bytes));
+    if (scr.base == nullptr) {
+        // launch order by descending bin count (a fixed region keeps the plain order: its size is
+        // stated without the sort's storage, and the order changes no insertion's energy)
+        A.sort_bits = 1;
+        while (A.sort_bits < 31 && ((int64_t)1 << A.sort_bits) <= chunk_n) ++A.sort_bits;
+        size_t sort_bytes = 0;
+        HIP_TRY(binned_sort_bytes(nbins, A.sort_bits, &sort_bytes));
+        scan_bytes = std::max(scan_bytes, sort_bytes);
+        SCR_TRY(b, scr.take(b->d_bin_perm, sizeof(int) * 3 * (size_t)nbins, &p));
+        A.bin_perm = (int *)p;
+        A.bin_iota = A.bin_perm + nbins;
+        A.bin_ksort = (unsigned *)(A.bin_perm + 2 * (size_t)nbins);
+    }
     void *scan_tmp = nullptr;
     SCR_TRY(b, scr.take(b->d_scan_tmp, scan_bytes, &scan_tmp));
-    const int64_t chunk_n = std::min<int64_t>(CH, ge - gb);
     SCR_TRY(b, scr.take(b->d_sitefx, sizeof(uint32_t) * 3 * b->nsites * chunk_n, &p));
     A.sitefx = (uint32_t *)p;
     SCR_TRY(b, scr.take(b->d_key, sizeof(int) * chunk_n, &p));

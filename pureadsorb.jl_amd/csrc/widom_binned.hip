@@ -275,6 +275,7 @@ __global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i <= a.nbins) a.bin_count[i] = 0;
+    if (a.bin_perm && i < a.nbins) a.bin_iota[i] = i;
     if (i == 0) *a.ncand = 0;
     if (i == 0 && a.zero_passes) *a.extra_passes = 0;
 }
@@ -364,7 +365,8 @@ template <int NS, bool CHARGED, int CUT>
 __global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int b = blockIdx.x;
+    // heaviest bins first, so that the launch does not end on a few crowded bins
+    const int b = a.bin_perm ? a.bin_perm[blockIdx.x] : (int)blockIdx.x;
     const int i0 = a.bin_start[b];
     const int nins = a.bin_start[b + 1] - i0;
     if (nins == 0) return;
@@ -792,6 +794,14 @@ hipError_t binned_scan_bytes(int nbins, size_t *bytes)
     return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (int *)nullptr, (int *)nullptr, nbins + 1);
 }
 
+// Temporary storage of the descending sort of `nbins` counts of `bits` bits (the launch order).
+hipError_t binned_sort_bytes(int nbins, int bits, size_t *bytes)
+{
+    *bytes = 0;
+    return hipcub::DeviceRadixSort::SortPairsDescending(nullptr, *bytes, (unsigned *)nullptr, (unsigned *)nullptr,
+                                                        (int *)nullptr, (int *)nullptr, nbins, 0, bits);
+}
+
 // Largest neighbour-list capacity (entries, a multiple of 64) that keeps a bin workgroup's LDS
 // within BIN_LDS_BUDGET, so that two workgroups share a CU's 160 KB.
 constexpr size_t BIN_LDS_BUDGET = 80 * 1024;
@@ -827,6 +837,14 @@ hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes
     size_t bytes = scan_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, a.bin_count, a.bin_start, a.nbins + 1, st);
     if (e != hipSuccess) return e;
+    if (a.bin_perm) {
+        // the energy kernel's launch order: bins by descending count (the same temporary storage,
+        // sized for both, after the scan on the same stream)
+        bytes = scan_bytes;
+        e = hipcub::DeviceRadixSort::SortPairsDescending(scan_tmp, bytes, (const unsigned *)a.bin_count, a.bin_ksort,
+                                                         a.bin_iota, a.bin_perm, a.nbins, 0, a.sort_bits, st);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
