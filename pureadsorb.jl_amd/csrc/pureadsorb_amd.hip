@@ -1504,9 +1504,10 @@ static int run_energies(pa_batch *b, const std::vector<int64_t> &seg, int64_t gb
     const int64_t chunk_n = std::min<int64_t>(CH, ge - gb);
     size_t scan_bytes = 0;
     HIP_TRY(binned_scan_bytes(nbins, &scan_bytes));
-    if (scr.base == nullptr) {
+    if (scr.base == nullptr && !keys_out) {
         // launch order by descending bin count (a fixed region keeps the plain order: its size is
-        // stated without the sort's storage, and the order changes no insertion's energy)
+        // stated without the sort's storage, and the order changes no insertion's energy; a call
+        // that only returns the bin keys launches no energy kernel)
         A.sort_bits = 1;
         while (A.sort_bits < 31 && ((int64_t)1 << A.sort_bits) <= chunk_n) ++A.sort_bits;
         size_t sort_bytes = 0;
