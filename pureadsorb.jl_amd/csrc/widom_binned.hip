@@ -4,14 +4,15 @@
 //  1. bin_pose_kernel       draws each pose (FP64, bit-identical to the oracle's), stores its sites
 //                           as 32-bit fixed-point fractional coordinates, and counts insertions
 //                           per spatial bin of the system's cell;
-//  2. exclusive scan of the bin counts (hipCUB);
+//  2. exclusive scan of the bin counts (hipCUB), and their descending sort (the launch order of
+//     step 4: the most crowded bins first);
 //  3. bin_scatter_kernel    groups the chunk's insertions by bin (positions from step 1's counts);
 //  4. widom_binned_kernel   one workgroup per bin: builds the bin's neighbour list in LDS (every
 //                           atom image within rc + r_guest + r_bin of the bin centre, FP32
 //                           coordinates relative to that centre, computed in FP64, in order of
 //                           16 radial classes; a longer list than the LDS holds is taken in
 //                           windows of that order over several passes), then each
-//                           wavefront takes one insertion at a time, sweeps the list with lanes
+//                           wavefront claims one insertion at a time, sweeps the list with lanes
 //                           over atoms, compacts in-cutoff (site, atom) pairs into a per-wave LDS
 //                           ring, and evaluates the FP32 LJ + screened-Coulomb terms on full
 //                           wavefronts into FP64 accumulators; the reciprocal cross term uses
@@ -354,7 +355,7 @@ __host__ __device__ inline size_t binned_lds_bytes(int cap, int ns)
     off += ((size_t)cap * 2 + 15) & ~(size_t)15;        // uint16 types
     off += (size_t)BIN_NW * BRING * 12;                 // rings
     off += (size_t)ns * BIN_MAX_TYPES * 8;              // pair table
-    off += 16 * 4;                                      // KE * q per site
+    off += 16 * 4;                                      // KE * q per site; slot 8: insertion counter
     off += BIN_NCLS * 4;                                // class sizes
     off += (size_t)BIN_NW * BIN_NCLS * 4;               // first list position per (wave, class)
     off += (size_t)(cap / 64) * 16;                     // row bounds: (dmin, dmax), (prefix max, suffix min)
@@ -405,6 +406,8 @@ __global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
     const uint32_t cz = to_fixed((bz + 0.5) / B.nb[2]);
     for (int i = tid; i < NS * ntc; i += BIN_WG) pair_lds[(i / ntc) * ntc + (i % ntc)] = a.pair32[i];
     if (tid < NS) qke_lds[tid] = a.guest.qke[tid];
+    static_assert(NS <= 8, "the insertion counter shares the per-site charge slots");
+    int *next_ins = (int *)(qke_lds + 8);  // the bin's insertion counter (slot 8 of 16)
 
     const double A0 = F.A[0], A1 = F.A[1], A2 = F.A[2], A3 = F.A[3], A4 = F.A[4], A5 = F.A[5], A6 = F.A[6],
                  A7 = F.A[7], A8 = F.A[8];
@@ -577,10 +580,12 @@ __global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
             for (int q = r; q < nrows; ++q) sm = fminf(sm, rowb[q].x);
             rowc[r] = make_float2(pm, sm);
         }
+        if (tid == 0) *next_ins = BIN_NW;
         __syncthreads();
 
-        // ---- evaluate the bin's insertions against the list
-        for (int ii = wave; ii < nins; ii += BIN_NW) {
+        // ---- evaluate the bin's insertions against the list: each wavefront starts with its own
+        // index and claims the next unclaimed one when it finishes (costs differ per insertion)
+        for (int ii = wave; ii < nins;) {
             const int grel = a.order[i0 + ii];
             const uint32_t *sf = a.sitefx + (size_t)grel * 3 * NS;
             uint32_t sfx[NS][3];
@@ -744,6 +749,9 @@ __global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
                 if (pass == 0) *o = Etot + F.const_off;
                 else *o += Etot;
             }
+            int claim = 0;
+            if (lane == 0) claim = atomicAdd(next_ins, 1);
+            ii = __builtin_amdgcn_readfirstlane(claim);
         }
         if (lo + cap >= total) break;
         __syncthreads();
