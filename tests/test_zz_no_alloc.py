@@ -1,7 +1,9 @@
 """pa_widom_noalloc (no device allocation; per-call buffers from one static device array) against
 pa_widom on the same batch, seeds and insertion counts: every WidomResult field equal, on two
 calls in a row, the second smaller. The bin scale is fixed first, because pa_widom may tune it
-and pa_widom_noalloc never does."""
+and pa_widom_noalloc never does. pa_widom launches the bins by descending count and
+pa_widom_noalloc in spatial order, so equality also shows that the launch order changes no
+energy."""
 from __future__ import annotations
 
 import ctypes as C
