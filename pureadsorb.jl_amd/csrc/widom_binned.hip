@@ -4,8 +4,8 @@
 //  1. bin_pose_kernel       draws each pose (FP64, bit-identical to the oracle's), stores its sites
 //                           as 32-bit fixed-point fractional coordinates, and counts insertions
 //                           per spatial bin of the system's cell;
-//  2. exclusive scan of the bin counts (hipCUB), and their descending sort (the launch order of
-//     step 4: the most crowded bins first);
+//  2. exclusive scan of the bin counts and the launch order of step 4, the most crowded bins
+//     first (one workgroup for up to BIN_ORDER_MAX bins, else hipCUB's scan and radix sort);
 //  3. bin_scatter_kernel    groups the chunk's insertions by bin (positions from step 1's counts);
 //  4. widom_binned_kernel   one workgroup per bin: builds the bin's neighbour list in LDS (every
 //                           atom image within rc + r_guest + r_bin of the bin centre, FP32
@@ -269,6 +269,65 @@ __global__ __launch_bounds__(256) void bin_reject_kernel(BinnedArgs a)
             a.rank[rel] = atomicAdd(&a.bin_count[a.key[rel]], 1);
         }
     }
+}
+
+// Launch order of the energy kernel for a grid of at most BIN_ORDER_MAX bins, and the exclusive
+// scan of the counts, in one workgroup (a larger grid takes hipCUB's scan and radix sort): the
+// counts are staged in LDS, each thread scans a contiguous range, and the bins are placed by
+// count class, most crowded first. Classes are half octaves of the count (empty bins last), so
+// within a class the counts differ by less than 1.5x and their order is arbitrary: enough for
+// the heaviest-first rule, and no insertion's energy depends on the order.
+constexpr int BIN_ORDER_WG = 1024;
+constexpr int BIN_ORDER_MAX = 14336;  // 56 KB of counts in LDS
+constexpr int BIN_ORDER_NCLS = 64;
+
+__device__ inline int order_class(int c)
+{
+    if (c <= 0) return 0;
+    const int e = 31 - __clz(c);                      // floor(log2 c), 0..30
+    const int half = e > 0 ? (c >> (e - 1)) & 1 : 0;  // upper or lower half of the octave
+    return 1 + 2 * e + half;                          // 1..62
+}
+
+__global__ __launch_bounds__(BIN_ORDER_WG) void bin_order_kernel(BinnedArgs a)
+{
+    __shared__ int cs[BIN_ORDER_MAX + 1];  // counts, then starts
+    __shared__ int part[BIN_ORDER_WG];
+    __shared__ int ncls[BIN_ORDER_NCLS], cursor[BIN_ORDER_NCLS];
+    const int tid = threadIdx.x, nb = a.nbins, n1 = nb + 1;
+    for (int i = tid; i < n1; i += BIN_ORDER_WG) cs[i] = a.bin_count[i];
+    if (tid < BIN_ORDER_NCLS) ncls[tid] = 0;
+    __syncthreads();
+    const int per = (n1 + BIN_ORDER_WG - 1) / BIN_ORDER_WG;
+    const int r0 = min(n1, tid * per), r1 = min(n1, r0 + per);
+    int sum = 0;
+    for (int i = r0; i < r1; ++i) sum += cs[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < BIN_ORDER_WG; off <<= 1) {  // inclusive scan of the range sums
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int i = r0; i < r1; ++i) {
+        const int c = cs[i];
+        cs[i] = run;
+        run += c;
+        if (i < nb) atomicAdd(&ncls[order_class(c)], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int k = BIN_ORDER_NCLS - 1; k >= 0; --k) {
+            cursor[k] = acc;
+            acc += ncls[k];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n1; i += BIN_ORDER_WG) a.bin_start[i] = cs[i];
+    for (int i = tid; i < nb; i += BIN_ORDER_WG) a.bin_perm[atomicAdd(&cursor[order_class(cs[i + 1] - cs[i])], 1)] = i;
 }
 
 // Clears the bin counts (nbins + 1) and the candidate counter of one chunk in one launch.
@@ -842,6 +901,14 @@ hipError_t launch_binning(const BinnedArgs &a, void *scan_tmp, size_t scan_bytes
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
+    if (a.bin_perm && a.nbins <= BIN_ORDER_MAX) {
+        // scan and launch order in one workgroup
+        hipLaunchKernelGGL(bin_order_kernel, dim3(1), dim3(BIN_ORDER_WG), 0, st, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
     size_t bytes = scan_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, a.bin_count, a.bin_start, a.nbins + 1, st);
     if (e != hipSuccess) return e;
