@@ -50,6 +50,26 @@ def test_noalloc_equals_widom_on_two_calls(rubtak_batch, precision):
         assert sum(n.nsamples for n in got) == ninsert
 
 
+@pytest.mark.parametrize("kappa", [0.5, 2.0])
+def test_both_launch_order_paths_match_the_spatial_order(rubtak_batch, kappa):
+    """pa_widom orders the energy launch in one workgroup for grids of up to 14,336 bins and with
+    hipCUB's radix sort above that. 200,000 insertions per system in two 36 A cells give
+    2 x 26^3 = 35,152 bins at bin scale 0.5 (the sort) and 2 x 6^3 = 432 at 2.0 (one workgroup);
+    pa_widom_noalloc takes the same bins in spatial order: every field equal."""
+    b, g = rubtak_batch
+    kept = b.bin_scale
+    b.bin_scale = kappa
+    try:
+        print(f"launch order vs spatial order: bin scale {kappa}", flush=True)
+        ref = pa.widom(b, g, T=298.15, ninsert=400_000, seed=7, nblocks=5, precision="mixed")
+        got = pa.widom(b, g, T=298.15, ninsert=400_000, seed=7, nblocks=5, precision="mixed", no_alloc=True)
+    finally:
+        b.bin_scale = kept
+    for r, n in zip(ref, got):
+        assert _fields(n) == _fields(r)
+        assert np.isfinite(n.ln_K_H)
+
+
 def test_noalloc_single_phase_and_chunked(rubtak_batch):
     """Single-phase and a small chunk (several energy launches) through the fixed array."""
     b, g = rubtak_batch
