@@ -132,8 +132,8 @@ struct BinnedArgs {
     int *key;                  // (g - g0) global bin id
     int *bin_count;            // nbins + 1
     int *bin_start;            // nbins + 1
-    int *bin_perm;             // nbins: launch order of the energy kernel (bins by descending count), or null
-    int *bin_iota;             // nbins: 0, 1, 2, ... (the sort's input values)
+    int *bin_perm;             // nbins: launch order of the energy kernel (most crowded bins first), or null
+    int *bin_iota;             // nbins: 0, 1, 2, ... (the radix sort's input values, grids beyond one workgroup's)
     unsigned *bin_ksort;       // nbins: the sorted counts (the sort's key output)
     int sort_bits;             // bits of the largest possible count
     int *rank;                 // (g - g0) position of the insertion within its bin (from the count)

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void bin_zero_kernel(BinnedArgs a)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i <= a.nbins) a.bin_count[i] = 0;
-    if (a.bin_perm && i < a.nbins) a.bin_iota[i] = i;
+    if (a.bin_perm && a.nbins > BIN_ORDER_MAX && i < a.nbins) a.bin_iota[i] = i;  // the radix sort's values
     if (i == 0) *a.ncand = 0;
     if (i == 0 && a.zero_passes) *a.extra_passes = 0;
 }
