@@ -632,12 +632,28 @@ __global__ __launch_bounds__(BIN_WG, 6) void widom_binned_kernel(BinnedArgs a)
         }
         __syncthreads();
         // prefix maximum of the upper bounds and suffix minimum of the lower bounds: the rows
-        // wholly inside a site's cutoff form a prefix, those wholly outside a suffix
-        for (int r = tid; r < nrows; r += BIN_WG) {
-            float pm = 0.0f, sm = INFINITY;
-            for (int q = 0; q <= r; ++q) pm = fmaxf(pm, rowb[q].y);
-            for (int q = r; q < nrows; ++q) sm = fminf(sm, rowb[q].x);
-            rowc[r] = make_float2(pm, sm);
+        // wholly inside a site's cutoff form a prefix, those wholly outside a suffix. Up to 64
+        // rows: one wavefront, a lane per row, six shuffle steps (max and min are exact, so any
+        // order gives the same bounds); longer lists: a thread per row
+        if (nrows <= 64) {
+            if (wave == 0) {
+                const float2 rb = lane < nrows ? rowb[lane] : make_float2(INFINITY, 0.0f);
+                float pm = rb.y, sm = rb.x;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const float up = __shfl_up(pm, off, 64), dn = __shfl_down(sm, off, 64);
+                    if (lane >= off) pm = fmaxf(pm, up);
+                    if (lane + off < 64) sm = fminf(sm, dn);
+                }
+                if (lane < nrows) rowc[lane] = make_float2(fmaxf(pm, 0.0f), sm);
+            }
+        } else {
+            for (int r = tid; r < nrows; r += BIN_WG) {
+                float pm = 0.0f, sm = INFINITY;
+                for (int q = 0; q <= r; ++q) pm = fmaxf(pm, rowb[q].y);
+                for (int q = r; q < nrows; ++q) sm = fminf(sm, rowb[q].x);
+                rowc[r] = make_float2(pm, sm);
+            }
         }
         if (tid == 0) *next_ins = BIN_NW;
         __syncthreads();
